@@ -1,0 +1,86 @@
+"""Generate the reference fixtures under tests/golden/ from the compiled reference.
+
+Runs oracle/_ref/ref_harness (the reference Fortran of gabortimar/CPFEM_code,
+src/fasolt/, compiled by oracle/Makefile with flang in this container) on
+seeded inputs and stores inputs and outputs as .npz.  The reference computes
+in REAL(4), so every stored real is float32.
+
+    python tests/golden/gen_fixtures.py        # (re)writes the .npz files
+
+Fixtures:
+  dstrss_fcc.npz  DSTRSS on 256 FCC points (seed 11, 10% elastic elements)
+  dstrss_hcp.npz  DSTRSS on 128 HCP points with twin systems (seed 12)
+  nrstep_fcc.npz  ELVALS+SUVALS then FACSLS on a 3^3 one-point brick mesh
+  incs_fcc.npz    two increments of the reference loop on a 2^3 one-point mesh
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, HERE)
+
+import refio  # noqa: E402
+from gpbatch import make_gp_batch  # noqa: E402
+from cpfem_code_amd.problem import (LT_BRICK_1GP, Control, fcc_aluminium, fcc_tension,  # noqa: E402
+                                    hcp_magnesium, f32)
+
+DT = 1.0e-3
+
+
+def dstrss_fixture(name, mat, n, seed, **kw):
+    b = make_gp_batch(mat, n, seed=seed, **kw)
+    r = refio.dstrss([mat], Control(), DT, b["iel"], b["de"], b["r"], b["st"], b["ds"], b["sv"], b["dsv"])
+    out = {f"in_{k}": v for k, v in b.items()}
+    out.update({f"ref_{k}": np.asarray(v) for k, v in r.items()})
+    np.savez_compressed(os.path.join(HERE, name), dt=DT, **{k: (v.astype(np.float32) if v.dtype == np.float64 else v)
+                                                            for k, v in out.items()})
+
+
+def nrstep_state(p, seed):
+    """A plastic-looking state: small smooth displacement increments and stresses."""
+    rng = np.random.default_rng(seed)
+    s = p.zero_state()
+    x = p.x.reshape(-1, 3)
+    grad = rng.normal(scale=1e-3, size=(3, 3))
+    s["x"] = p.x.copy()
+    s["dx"] = f32((x @ grad.T).reshape(-1) + rng.normal(scale=1e-5, size=p.ndof))
+    s["st"] = f32(rng.normal(scale=30.0, size=(p.ngps, 6)))
+    s["ds"] = f32(rng.normal(scale=2.0, size=(p.ngps, 6)))
+    return s
+
+
+def main():
+    assert refio.available(), "build the reference first: make -C oracle ref"
+    dstrss_fixture("dstrss_fcc.npz", fcc_aluminium(), 256, 11, elastic_frac=0.1, stress=50.0)
+    dstrss_fixture("dstrss_hcp.npz", hcp_magnesium(), 128, 12, elastic_frac=0.1, stress=25.0)
+
+    p = fcc_tension(3, ninc=4, lt=LT_BRICK_1GP, seed=5)
+    s = nrstep_state(p, 6)
+    r = refio.nrstep(p, s, inc=1)
+    np.savez_compressed(os.path.join(HERE, "nrstep_fcc.npz"),
+                        in_dx=s["dx"].astype(np.float32), in_st=s["st"].astype(np.float32),
+                        in_ds=s["ds"].astype(np.float32),
+                        err=r["err"], sumcpiter=np.float32(r["sumcpiter"]), fc=r["fc"].astype(np.float32),
+                        rhs=r["rhs"].astype(np.float32), is_=r["is"], ss=r["ss"].astype(np.float32),
+                        ds=r["state"]["ds"].astype(np.float32), dsv=r["state"]["dsv"].astype(np.float32),
+                        de=r["state"]["de"].astype(np.float32), dep=r["state"]["dep"].astype(np.float32),
+                        err_sor=r["err_sor"], ddx=r["ddx"].astype(np.float32))
+
+    p = fcc_tension(2, ninc=2, lt=LT_BRICK_1GP, seed=7, control=Control(maxit=20))
+    s = p.zero_state()
+    s["x"], s["dx"] = p.x.copy(), p.dx.copy()
+    res = refio.increments(p, s, 2)
+    arrs = {}
+    for k, st in enumerate(res):
+        for name in ("x", "dx", "fc", "st", "sv", "ep", "eps", "rt", "cumslip", "srate"):
+            arrs[f"inc{k + 1}_{name}"] = st[name].astype(np.float32)
+        arrs[f"inc{k + 1}_err"] = np.int32(st["err"])
+    np.savez_compressed(os.path.join(HERE, "incs_fcc.npz"), **arrs)
+
+
+if __name__ == "__main__":
+    main()
