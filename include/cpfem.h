@@ -1,0 +1,94 @@
+/*
+ * cpfem.h — C ABI of the MI355X-native FASOLT3 hot path.
+ *
+ * Plain pointers and sizes only; every entry point returns an int that is
+ * either 0 or a FASOLT3 error code of the reference's FAFERR table
+ * (src/fasolt/Fasolt3.f90:299-323): 903 zero diagonal, 904 SOR not
+ * converged, 905/906 rotation/strain increment too large, 910 singular LU,
+ * 920 bad element weight, 955 row too long, 987 slip Newton residual > 0.5.
+ * Codes below 0 are errors of this library (CPFEM_E_*).  The host keeps
+ * the reference behaviour by calling its own FAFERR(code) on a nonzero
+ * return.  Arrays follow the reference's Fortran layout (column-major per
+ * point, e.g. st(1:6, gp) -> st[6*gp + i]); index arrays are 1-based.
+ * The context is not re-entrant; calls are stream-ordered and return when
+ * the results the host asked for are in host memory.
+ */
+#ifndef CPFEM_H
+#define CPFEM_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CPFEM_MAXSS 60 /* fas-com-mod.f90:55 maxss */
+#define CPFEM_MAXSV 63 /* fas-com-mod.f90:14 maxsvs */
+
+#define CPFEM_E_ARG (-1)       /* bad argument or unsupported size */
+#define CPFEM_E_HIP (-2)       /* HIP runtime error */
+#define CPFEM_E_NSS (-3)       /* number of slip systems not compiled in */
+#define CPFEM_E_NOGPU (-4)     /* no gfx950 device */
+
+/* One material set as held in the .fsd file (Faslt_io_nowin.f90:77-107). */
+typedef struct {
+    int32_t nss;
+    int32_t twinlim;
+    double twinprob;
+    double e[9];                              /* S11 S22 S33 S23 S31 S12 S44 S55 S66 */
+    double ssang[CPFEM_MAXSS];                /* 0 slip, 1 unidirectional, |.|>1.5 twin (deg) */
+    double ab[CPFEM_MAXSS][3];                /* ab(1:3,i) slip direction */
+    double an[CPFEM_MAXSS][3];                /* an(1:3,i) plane normal */
+    double ampar[CPFEM_MAXSS][10];            /* ampar(1:10,i) */
+    double smatrx[CPFEM_MAXSS][CPFEM_MAXSS];  /* smatrx[i][j] == Fortran smatrx(j,i) */
+} cpfem_material;
+
+/* fafcdf.dat (Fasolt3.f90:216-218) plus the solver settings of FACSLS. */
+typedef struct {
+    double toler;
+    int32_t maxit;
+    int32_t sor_maxit;    /* Stif_sol.f90:173 maxitl */
+    double alpha, afactor;
+    double efix, smalp, smalf;
+    double smald, smale, smalr;
+    double sor_omega;     /* Stif_sol.f90:173 alphal */
+    double sor_tol;       /* tolerance of the SOR stopping test */
+    int32_t sor_kind;     /* 0: |dx|^2/|x|^2 < tol^2 (Stif_sol.f90:208-210); 1: |b-Ax|/|b| < tol */
+    int32_t precision;    /* 0: FP64 throughout; 1: FP32 element products and SOR sweeps, FP64 residuals */
+} cpfem_control;
+
+/*
+ * Per-point constitutive update on a batch of independent points.
+ * Replaces DSTRSS(iel, jgp, deff, es, de, st, ds, t, r, chi)
+ * (src/fasolt/Sig_eps.f90:84), which the reference calls once per point
+ * from ELVALS (src/fasolt/Elvals.f90:113), together with the module state
+ * it reads and writes: sv/dsv(:,jgp), srate(:,jgp), twinsys(jgp), tflag and
+ * COMMON /CPITER/ sumcpiter.
+ *   iel[n]            element material code (negative: elastic only)
+ *   de[n][6], r[n][3] strain increment (eng. shear) and rotation increment
+ *   st[n][6]          stress at the start of the increment
+ *   ds[n][6]          in: stress-increment guess, out: stress increment
+ *   sv[n][ldsv]       state: Euler angles 1-3, slip resistances 4..3+nss
+ *   dsv[n][ldsv]      in/out: state increments
+ *   t[n][6]           out: updated stress
+ *   chi[n][36]        out: tangent d(sigma)/d(eps), Fortran chi(6,6) order
+ *   deff[n]           out: effective plastic strain increment
+ *   srate[n][60]      in/out: slip rates (unchanged on the elastic branch)
+ *   twinsys[n]        in/out; timestwinned[n] in
+ *   tflag, sumcpiter  in/out accumulators (max / sum)
+ *   flops[n], err[n]  optional (may be NULL): per-point algorithmic FLOPs and codes
+ * Host pointers.  Returns the largest error code over the batch.
+ */
+int cpfem_dstrss_batch(const cpfem_material *mat, const cpfem_control *ctl, double dt, int32_t n,
+                       const int32_t *iel, const double *de, const double *r, const double *st,
+                       double *ds, const double *sv, double *dsv, int32_t ldsv,
+                       double *t, double *chi, double *deff, double *srate,
+                       int32_t *twinsys, const int32_t *timestwinned,
+                       int32_t *tflag, double *sumcpiter, int64_t *flops, int32_t *err);
+
+/* Device name and compute units of the current device (diagnostics). */
+int cpfem_device_info(char *name, int32_t len, int32_t *cus);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CPFEM_H */
