@@ -136,3 +136,202 @@ def dstrss_batch(material, control, dt, iel, de, r, st, ds, sv, dsv, twinsys=Non
         raise FafErr(code, "cpfem_dstrss_batch")
     return dict(t=t, ds=ds, chi=chi.reshape(n, 6, 6).transpose(0, 2, 1), deff=deff, dsv=dsv, srate=srate,
                 twinsys=tw, tflag=int(tflag[0]), sumcpiter=float(sumcp[0]), err=err, flops=flops, code=code)
+
+
+# ---------------------------------------------------------------------------
+# device-resident problem (NRITER call sequence)
+# ---------------------------------------------------------------------------
+class CpfemMesh(C.Structure):
+    _fields_ = [("nlmnt", C.c_int32), ("nnod", C.c_int32), ("ngps", C.c_int32), ("nsv", C.c_int32),
+                ("le", _P), ("lt", _P), ("nd", _P), ("np", _P), ("ln", _P), ("lg", _P), ("ib", _P),
+                ("ninc1", C.c_int32), ("nxs", C.c_int32), ("nfs", C.c_int32),
+                ("tiem", _P), ("x_spec", _P), ("f_spec", _P)]
+
+
+class CpfemStats(C.Structure):
+    _fields_ = [("nit", C.c_int32), ("sor_sweeps", C.c_int32), ("sor_sweeps_total", C.c_int32), ("err", C.c_int32),
+                ("sdx", C.c_double), ("sfc", C.c_double), ("sumcpiter", C.c_double), ("alp2", C.c_double),
+                ("sor_error", C.c_double), ("t_const_ms", C.c_double), ("t_asm_ms", C.c_double),
+                ("t_suvals_ms", C.c_double), ("t_sor_ms", C.c_double), ("t_update_ms", C.c_double),
+                ("t_total_ms", C.c_double), ("flops_const", C.c_int64), ("transc_const", C.c_int64),
+                ("gp_updates", C.c_int64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_CTX = C.c_void_p
+
+
+def _bind_mesh_api(L):
+    L.cpfem_create.argtypes = [C.POINTER(CpfemMesh), C.POINTER(CpfemMaterial), C.c_int32, C.POINTER(CpfemControl),
+                               C.POINTER(_CTX)]
+    L.cpfem_destroy.argtypes = [_CTX]
+    L.cpfem_set_state.argtypes = [_CTX] + [_P] * 6 + [C.c_int32] + [_P] * 8
+    L.cpfem_get_state.argtypes = [_CTX] + [_P] * 7 + [C.c_int32] + [_P] * 10
+    L.cpfem_update_node.argtypes = [_CTX]
+    L.cpfem_elvals.argtypes = [_CTX, C.c_int32, _P, _P]
+    L.cpfem_suvals.argtypes = [_CTX, C.c_int32, _P]
+    L.cpfem_facsls.argtypes = [_CTX, _P, _P]
+    L.cpfem_nr_update.argtypes = [_CTX, _P, _P]
+    L.cpfem_nriter.argtypes = [_CTX, C.c_int32, C.POINTER(CpfemStats)]
+    L.cpfem_update_gptv.argtypes = [_CTX]
+    L.cpfem_reorient.argtypes = [_CTX, _P]
+    L.cpfem_increment.argtypes = [_CTX, C.c_int32, C.POINTER(CpfemStats)]
+    L.cpfem_get_system.argtypes = [_CTX, _P, _P]
+    L.cpfem_get_matrix.argtypes = [_CTX, _P, _P, _P, _P]
+    L.cpfem_get_stats.argtypes = [_CTX, C.POINTER(CpfemStats)]
+
+
+def _check(code, where, ok_codes=()):
+    if code < 0:
+        raise RuntimeError(f"{where} failed ({code})")
+    if code and code not in ok_codes and code > 500:
+        raise FafErr(code, where)
+    return code
+
+
+class Solver:
+    """A FASOLT3 problem resident on the GPU, driven through the C ABI.
+
+    Methods follow the reference's NRITER call sequence (Stif_sol.f90:10-88):
+    ``elvals``, ``suvals``, ``facsls``, ``nr_update``; ``nriter`` runs the whole
+    damped Newton loop of one increment and ``increment`` adds UPDATE_NODE,
+    UPDATE_GPTV and DoReorient (Fasolt3.f90:38-46).
+    """
+
+    def __init__(self, prob, precision: int = 0, control=None):
+        L = lib()
+        if not hasattr(L, "_mesh_bound"):
+            _bind_mesh_api(L)
+            L._mesh_bound = True
+        self.L = L
+        self.prob = prob
+        p = prob
+        keep = self._keep = {}
+
+        def arr(name, a, dt):
+            a = np.ascontiguousarray(a, dt)
+            keep[name] = a
+            return a.ctypes.data_as(C.c_void_p)
+
+        m = CpfemMesh()
+        m.nlmnt, m.nnod, m.ngps, m.nsv = p.nlmnt, p.nnod, p.ngps, p.nsv
+        m.le, m.lt, m.nd = arr("le", p.le, np.int32), arr("lt", p.lt, np.int32), arr("nd", p.nd, np.int32)
+        m.np = arr("np", p.np_, np.int32)
+        m.ln, m.lg, m.ib = arr("ln", p.ln, np.int32), arr("lg", p.lg, np.int32), arr("ib", p.ib, np.int32)
+        m.ninc1, m.nxs, m.nfs = p.ninc1, p.nxs, p.nfs
+        m.tiem = arr("tiem", p.tiem, np.float64)
+        m.x_spec = arr("x_spec", p.x_spec, np.float64)
+        m.f_spec = arr("f_spec", p.f_spec, np.float64)
+        self._mesh = m
+        self._mats = (CpfemMaterial * len(p.materials))(*[material_struct(mt) for mt in p.materials])
+        self._ctl = control_struct(control or p.control, precision)
+        h = _CTX()
+        _check(L.cpfem_create(C.byref(m), self._mats, len(p.materials), C.byref(self._ctl), C.byref(h)),
+               "cpfem_create")
+        self.h = h
+        st = p.zero_state()
+        st["x"], st["dx"] = p.x, p.dx
+        self.set_state(st)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cpfem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, s):
+        f = lambda k, dt=np.float64: None if k not in s else np.ascontiguousarray(s[k], dt)  # noqa: E731
+        a = {k: f(k) for k in ("x", "dx", "eps", "dep", "sv", "dsv", "ep", "de", "st", "ds", "rt", "dr")}
+        a["twinsys"] = f("twinsys", np.int32)
+        a["timestwinned"] = f("timestwinned", np.int32)
+        ldsv = a["sv"].shape[1] if a["sv"] is not None else MAXSV
+        _check(self.L.cpfem_set_state(self.h, _ptr(a["x"]), _ptr(a["dx"]), _ptr(a["eps"]), _ptr(a["dep"]),
+                                      _ptr(a["sv"]), _ptr(a["dsv"]), ldsv, _ptr(a["ep"]), _ptr(a["de"]),
+                                      _ptr(a["st"]), _ptr(a["ds"]), _ptr(a["rt"]), _ptr(a["dr"]),
+                                      _ptr(a["twinsys"]), _ptr(a["timestwinned"])), "cpfem_set_state")
+
+    def get_state(self):
+        p = self.prob
+        g, d = p.ngps, p.ndof
+        s = dict(x=np.zeros(d), dx=np.zeros(d), fc=np.zeros(d), eps=np.zeros(g), dep=np.zeros(g),
+                 sv=np.zeros((g, MAXSV)), dsv=np.zeros((g, MAXSV)), ep=np.zeros((g, 6)), de=np.zeros((g, 6)),
+                 st=np.zeros((g, 6)), ds=np.zeros((g, 6)), rt=np.zeros((g, 3)), dr=np.zeros((g, 3)),
+                 srate=np.zeros((g, MAXSS)), cumslip=np.zeros((g, MAXSS)),
+                 twinsys=np.zeros(g, np.int32), timestwinned=np.zeros(g, np.int32))
+        _check(self.L.cpfem_get_state(self.h, _ptr(s["x"]), _ptr(s["dx"]), _ptr(s["fc"]), _ptr(s["eps"]),
+                                      _ptr(s["dep"]), _ptr(s["sv"]), _ptr(s["dsv"]), MAXSV, _ptr(s["ep"]),
+                                      _ptr(s["de"]), _ptr(s["st"]), _ptr(s["ds"]), _ptr(s["rt"]), _ptr(s["dr"]),
+                                      _ptr(s["srate"]), _ptr(s["cumslip"]), _ptr(s["twinsys"]),
+                                      _ptr(s["timestwinned"])), "cpfem_get_state")
+        return s
+
+    def update_node(self):
+        _check(self.L.cpfem_update_node(self.h), "cpfem_update_node")
+
+    def elvals(self, inc):
+        sp = np.zeros(1)
+        tf = np.zeros(1, np.int32)
+        code = _check(self.L.cpfem_elvals(self.h, inc, _ptr(sp), _ptr(tf)), "cpfem_elvals")
+        return code, float(sp[0]), int(tf[0])
+
+    def suvals(self, inc):
+        sfc = np.zeros(1)
+        _check(self.L.cpfem_suvals(self.h, inc, _ptr(sfc)), "cpfem_suvals")
+        return float(sfc[0])
+
+    def facsls(self):
+        sw = np.zeros(1, np.int32)
+        err = np.zeros(1)
+        code = _check(self.L.cpfem_facsls(self.h, _ptr(sw), _ptr(err)), "cpfem_facsls", ok_codes=(904,))
+        return code, int(sw[0]), float(err[0])
+
+    def nr_update(self):
+        sdx = np.zeros(1)
+        alp2 = np.zeros(1)
+        _check(self.L.cpfem_nr_update(self.h, _ptr(sdx), _ptr(alp2)), "cpfem_nr_update")
+        return float(sdx[0]), float(alp2[0])
+
+    def nriter(self, inc):
+        st = CpfemStats()
+        code = _check(self.L.cpfem_nriter(self.h, inc, C.byref(st)), "cpfem_nriter", ok_codes=(904,))
+        d = st.as_dict()
+        d["code"] = code
+        return d
+
+    def update_gptv(self):
+        _check(self.L.cpfem_update_gptv(self.h), "cpfem_update_gptv")
+
+    def reorient(self):
+        n = np.zeros(1, np.int32)
+        _check(self.L.cpfem_reorient(self.h, _ptr(n)), "cpfem_reorient")
+        return int(n[0])
+
+    def increment(self, inc):
+        st = CpfemStats()
+        code = _check(self.L.cpfem_increment(self.h, inc, C.byref(st)), "cpfem_increment", ok_codes=(904,))
+        d = st.as_dict()
+        d["code"] = code
+        return d
+
+    def system(self):
+        d = self.prob.ndof
+        rhs, ddx = np.zeros(d), np.zeros(d)
+        _check(self.L.cpfem_get_system(self.h, _ptr(rhs), _ptr(ddx)), "cpfem_get_system")
+        return rhs, ddx
+
+    def matrix(self):
+        import scipy.sparse as sp
+        nnz = np.zeros(1, np.int64)
+        _check(self.L.cpfem_get_matrix(self.h, _ptr(nnz), None, None, None), "cpfem_get_matrix")
+        n = int(nnz[0])
+        rows, cols, vals = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n)
+        _check(self.L.cpfem_get_matrix(self.h, _ptr(nnz), _ptr(rows), _ptr(cols), _ptr(vals)), "cpfem_get_matrix")
+        d = self.prob.ndof
+        return sp.csr_matrix((vals, (rows, cols)), shape=(d, d))

@@ -1,0 +1,1460 @@
+// Device-resident increment of FASOLT3: point kernels, colour-ordered element
+// assembly, penalty boundary conditions, multicolour block SOR and the damped
+// Newton update, behind the C ABI of include/cpfem.h.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <vector>
+
+#include "common.hpp"
+#include "constitutive.hpp"
+#include "kernels.hpp"
+#include "mesh.hpp"
+
+struct cpfem_ctx : cpfem::Ctx {};
+
+namespace cpfem {
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// deterministic reductions: per-block partials, then one block in fixed order
+// ---------------------------------------------------------------------------
+template <int Q>
+__device__ __forceinline__ void block_partials(double v[Q], double* partial, int slot) {
+    __shared__ double sh[kBlock / 64][Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        double x = v[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        v[q] = x;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) sh[wave][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            double s = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w][q];
+            partial[(size_t)slot * 4 + q] = s;
+        }
+    }
+}
+
+__global__ void k_reduce_final(const double* __restrict__ partial, int nblocks, double* __restrict__ out) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] += partial[(size_t)b * 4 + q];
+    __shared__ double sh[kBlock][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sh[threadIdx.x][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < 4; ++q) {
+            double s = 0.0;
+            for (int t = 0; t < (int)blockDim.x; ++t) s += sh[t][q];
+            out[q] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// simple vector kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, size_t n, T v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_axpy_nodes(double* __restrict__ y, const double* __restrict__ x, double a, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = y[i] + a * x[i];
+}
+
+// ---------------------------------------------------------------------------
+// point kernel: kinematics (SHAPE + DSTRNS) and DSTRSS for every point
+// ---------------------------------------------------------------------------
+struct PointMeshArgs {
+    int ngps;
+    const int* gp_elem;
+    const int* gp_local;
+    const int* elem_nodes;
+    const int* elem_lt;
+    const int* elem_le;
+    const double* x;
+    const double* dx;
+    double smale, smalr;
+    double* de;      // [6][n]
+    double* dr;      // [3][n]
+    double* dep;     // [n]
+    PointArrays P;
+};
+
+template <int N>
+__global__ __launch_bounds__(64) void k_points_mesh(const DevMaterial<N>* __restrict__ Mp, const DevControl C,
+                                                    const double dt, const PointMeshArgs A) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= A.ngps) return;
+    const DevMaterial<N>& M = *Mp;
+    const size_t n = (size_t)A.ngps;
+    const int e = A.gp_elem[g], ig = A.gp_local[g];
+    const int lt = A.elem_lt[e];
+    PointIO io;
+    io.err = 0;
+    io.tflag = 0;
+    io.deff = 0.0;
+    {
+        double xl[8][3], dxl[8][3], sfd[8][3];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const int nd = A.elem_nodes[8 * e + a];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                xl[a][k] = A.x[3 * nd + k];
+                dxl[a][k] = A.dx[3 * nd + k];
+            }
+        }
+        const double w = brick_shape(lt, ig, xl, sfd);
+        if (w < 1.e-7) io.err = 920;
+        double deff;
+        const int e2 = dstrns(sfd, dxl, A.smale, A.smalr, io.de, io.r, deff);
+        if (e2 > io.err) io.err = e2;
+        io.deff = deff;
+    }
+    io.iel = A.elem_le[e];
+    const PointArrays& P = A.P;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        io.st[i] = P.st[i * n + g];
+        io.ds[i] = P.ds[i * n + g];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        io.sv0[i] = P.sv[i * n + g];
+        io.dsv0[i] = P.dsv[i * n + g];
+    }
+    io.twinsys = P.twinsys[g];
+    io.timestwinned = P.timestwinned[g];
+    io.rng_key = (P.rng_epoch << 32) ^ (uint64_t)g;
+    double s0[N], dsv[3 + N], srate[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        s0[i] = P.sv[(3 + i) * n + g];
+        srate[i] = P.srate[i * n + g];
+    }
+#pragma unroll
+    for (int i = 0; i < 3 + N; ++i) dsv[i] = P.dsv[i * n + g];
+    Count cnt{0, 0, 0, 0};
+    const int kin_err = io.err;
+    io.err = 0;
+    dstrss<N>(M, C, dt, io, s0, dsv, srate, cnt);
+    if (kin_err > io.err) io.err = kin_err;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        A.de[i * n + g] = io.de[i];
+        P.t[i * n + g] = io.t[i];
+        P.ds[i * n + g] = io.ds[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) A.dr[i * n + g] = io.r[i];
+    A.dep[g] = io.deff;
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+#pragma unroll
+        for (int i = 0; i < 6; ++i) P.chi[(i + 6 * j) * n + g] = io.chi[i][j];
+    P.erepmd[g] = io.erepmd;
+#pragma unroll
+    for (int i = 0; i < 3 + N; ++i) P.dsv[i * n + g] = dsv[i];
+#pragma unroll
+    for (int i = 0; i < N; ++i) P.srate[i * n + g] = srate[i];
+    P.twinsys[g] = io.twinsys;
+    P.flops[g] = cnt.flops;
+    P.transc[g] = cnt.transc;
+    P.err[g] = io.err;
+    if (io.err) atomicMax(P.errmax, io.err);
+    if (io.tflag) atomicOr(P.tflag, 1);
+}
+
+// Sums of erepmd, flops and transcendentals over the points (report values).
+__global__ void k_point_sums(const double* __restrict__ erepmd, const long long* __restrict__ flops,
+                             const long long* __restrict__ transc, int n, double* partial) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        v[0] += erepmd[i];
+        v[1] += (double)flops[i];
+        v[2] += (double)transc[i];
+    }
+    block_partials<4>(v, partial, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// element assembly: one 64-lane wave per element, lane = node pair (i, j)
+// ---------------------------------------------------------------------------
+struct AsmArgs {
+    int R;                       // matrix rows
+    int nel_color;
+    const int* elems;            // elements of this colour
+    const int* elem_nodes;
+    const int* elem_gps;
+    const int* elem_lt;
+    const int* elem_np;
+    const int* scatter;          // [nel][64]
+    const double* x;
+    const double* chi;           // [36][ngps]
+    const double* t;             // [6][ngps]
+    int ngps;
+    double* fc;                  // [3*nnod]
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_assemble(const AsmArgs A, T* __restrict__ vals) {
+    constexpr int EPB = 4;  // elements per block
+    __shared__ T sh_dn[EPB][8][8][3];
+    __shared__ T sh_w[EPB][8];
+    __shared__ T sh_d[EPB][8][36];
+    __shared__ T sh_s[EPB][8][6];
+    __shared__ double sh_x[EPB][8][3];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = blockIdx.x * EPB + wave;
+    const bool live = k < A.nel_color;
+    const int e = live ? A.elems[k] : 0;
+    const int ng = live ? A.elem_np[e] : 0;
+    const int lt = live ? A.elem_lt[e] : 13;
+    if (live && lane < 24) {
+        const int a = lane / 3, c = lane % 3;
+        sh_x[wave][a][c] = A.x[3 * A.elem_nodes[8 * e + a] + c];
+    }
+    __syncthreads();
+    const size_t n = (size_t)A.ngps;
+    if (live && lane < ng) {
+        double xl[8][3], sfd[8][3];
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) xl[a][c] = sh_x[wave][a][c];
+        const double w = brick_shape(lt, lane, xl, sfd);
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sh_dn[wave][lane][a][c] = (T)sfd[a][c];
+        sh_w[wave][lane] = (T)w;
+        const int gp = A.elem_gps[8 * e + lane];
+#pragma unroll
+        for (int q = 0; q < 36; ++q) sh_d[wave][lane][q] = (T)A.chi[q * n + gp];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sh_s[wave][lane][q] = (T)A.t[q * n + gp];
+    }
+    __syncthreads();
+    if (!live) return;
+    const int i = lane >> 3, j = lane & 7;
+    T acc[3][3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) acc[p][q] = (T)0;
+    for (int g = 0; g < ng; ++g) {
+        T a[3], b[3], s[6], gm[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            a[c] = sh_dn[wave][g][i][c];
+            b[c] = sh_dn[wave][g][j][c];
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) s[q] = sh_s[wave][g][q];
+        pair_block<T>(a, b, &sh_d[wave][g][0], s, gm);
+        const T w = sh_w[wave][g];
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) acc[p][q] = acc[p][q] + w * gm[p][q];
+    }
+    const size_t R = (size_t)A.R;
+    const int off = A.scatter[64 * e + lane];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            T* dst = vals + off + (size_t)(3 * p + q) * R;
+            *dst = *dst + acc[p][q];
+        }
+    // internal forces of node i (Elvals.f90:122-126), lanes with j == 0
+    if (j == 0) {
+        double fl[3] = {0.0, 0.0, 0.0};
+        for (int g = 0; g < ng; ++g) {
+            const int gp = A.elem_gps[8 * e + g];
+            double xl[8][3], sfd[8][3];
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) xl[a][c] = sh_x[wave][a][c];
+            const double w = brick_shape(lt, g, xl, sfd);
+            double s[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) s[q] = A.t[q * n + gp];
+            fl[0] = fl[0] + w * (s[0] * sfd[i][0] + s[5] * sfd[i][1] + s[4] * sfd[i][2]);
+            fl[1] = fl[1] + w * (s[1] * sfd[i][1] + s[3] * sfd[i][2] + s[5] * sfd[i][0]);
+            fl[2] = fl[2] + w * (s[2] * sfd[i][2] + s[4] * sfd[i][0] + s[3] * sfd[i][1]);
+        }
+        const int nd = A.elem_nodes[8 * e + i];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) A.fc[3 * nd + c] = A.fc[3 * nd + c] + fl[c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SUVALS: rhs = fc + penalty terms; penalty stiffness on the diagonal blocks
+// ---------------------------------------------------------------------------
+struct BcArgs {
+    int nnod, R;
+    const int* ib;
+    const int* node_row;
+    const double* x;
+    const double* dx;
+    const double* fc;
+    const double* xs;   // x_spec of this increment [99][15]
+    const double* fs;   // f_spec of this increment [99][3]
+    double efix;
+    double* rhs;
+};
+
+template <typename T>
+__global__ void k_suvals(const BcArgs A, T* __restrict__ vals, double* __restrict__ vals64, double* partial) {
+    const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (nd < A.nnod) {
+        const int idof = 3 * nd;
+        double r[3] = {A.fc[idof], A.fc[idof + 1], A.fc[idof + 2]};
+        const int ibc = A.ib[nd];
+        if (ibc >= 100) {
+            const int i_kin = (ibc / 100) % 100;
+            if (i_kin > 0) {
+                const double* xs = A.xs + (size_t)(i_kin - 1) * 15;
+                double xspec[3];
+                if (i_kin < 50) {
+                    xspec[0] = xs[6];
+                    xspec[1] = xs[10];
+                    xspec[2] = xs[14];
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        double s = 0.0;
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) s = s + xs[6 + 3 * j + k] * (A.x[idof + k] + A.dx[idof + k]);
+                        xspec[j] = s;
+                    }
+                }
+                const int point[3][3] = {{0, 1, 3}, {1, 2, 4}, {3, 4, 5}};
+                const size_t row = (size_t)A.node_row[nd], R = (size_t)A.R;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        const double add = A.efix * xs[point[i][j]];
+                        T* dst = vals + (size_t)(3 * i + j) * R + row;
+                        *dst = *dst + (T)add;
+                        if (vals64) vals64[(size_t)(3 * i + j) * R + row] += add;
+                    }
+                    double sum = 0.0;
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) sum = sum + A.efix * xs[point[i][j]] * (A.dx[idof + j] - xspec[j]);
+                    r[i] = r[i] + sum;
+                }
+            }
+            const int i_spf = (ibc / 10000) % 100;
+            if (i_spf > 0) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i) r[i] = r[i] + A.fs[(size_t)(i_spf - 1) * 3 + i];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) A.rhs[idof + i] = r[i];
+        if (ibc < 10) v[0] = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        v[1] = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    }
+    block_partials<4>(v, partial, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// multicolour block SOR
+// ---------------------------------------------------------------------------
+struct SorArgs {
+    int R, S, r0, r1;
+    const int* row_node;
+    const int* cols;     // [S][R]
+    double omega;
+    const int* stop;     // device flag: nonzero -> skip the sweep
+};
+
+// One colour of one sweep on A y = b (y, b nodal vectors in DOF order).  The
+// three DOFs of a node are relaxed in x, y, z order with the fresh values of
+// the earlier ones, as the reference's natural-order sweep does within a node.
+// Accumulates |y_new - y_old|^2 and |y_new|^2 for the relative-update test.
+template <typename T, typename V>
+__global__ __launch_bounds__(256) void k_sor_color(const SorArgs A, const T* __restrict__ vals,
+                                                   const V* __restrict__ b, V* __restrict__ y, double* partial,
+                                                   int pslot0) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    const int r = A.r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (*A.stop == 0 && r < A.r1) {
+        const size_t R = (size_t)A.R;
+        T sum[3] = {(T)0, (T)0, (T)0};
+        for (int sl = 1; sl < A.S; ++sl) {
+            const int c = A.cols[(size_t)sl * R + r];
+            const T y0 = (T)y[3 * c], y1 = (T)y[3 * c + 1], y2 = (T)y[3 * c + 2];
+            const T* vb = vals + (size_t)sl * 9 * R + r;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                sum[a] = sum[a] + vb[(3 * a + 0) * R] * y0 + vb[(3 * a + 1) * R] * y1 + vb[(3 * a + 2) * R] * y2;
+        }
+        const int nd = A.row_node[r];
+        T d[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) d[q] = vals[(size_t)q * R + r];
+        T yn[3] = {(T)y[3 * nd], (T)y[3 * nd + 1], (T)y[3 * nd + 2]};
+        const T om = (T)A.omega;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            T s = sum[a];
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                if (q != a) s = s + d[3 * a + q] * yn[q];
+            const T old = yn[a];
+            const T nw = old + om * (((T)b[3 * nd + a] - s) / d[3 * a + a] - old);
+            yn[a] = nw;
+            const double dd = (double)nw - (double)old;
+            v[0] += dd * dd;
+            v[1] += (double)nw * (double)nw;
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) y[3 * nd + a] = (V)yn[a];
+    }
+    block_partials<4>(v, partial, pslot0 + blockIdx.x);
+}
+
+// Residual r = b - A y in FP64 over all rows (A read in its stored precision).
+template <typename T>
+__global__ __launch_bounds__(256) void k_residual(const SorArgs A, const T* __restrict__ vals,
+                                                  const double* __restrict__ b, const double* __restrict__ y,
+                                                  double* __restrict__ res, double* partial) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < A.R) {
+        const size_t R = (size_t)A.R;
+        double sum[3] = {0.0, 0.0, 0.0};
+        for (int sl = 0; sl < A.S; ++sl) {
+            const int c = (sl == 0) ? A.row_node[r] : A.cols[(size_t)sl * R + r];
+            const double y0 = y[3 * c], y1 = y[3 * c + 1], y2 = y[3 * c + 2];
+            const T* vb = vals + (size_t)sl * 9 * R + r;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                sum[a] += (double)vb[(3 * a + 0) * R] * y0 + (double)vb[(3 * a + 1) * R] * y1 +
+                          (double)vb[(3 * a + 2) * R] * y2;
+        }
+        const int nd = A.row_node[r];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double ra = b[3 * nd + a] - sum[a];
+            if (res) res[3 * nd + a] = ra;
+            v[0] += ra * ra;
+            v[1] += b[3 * nd + a] * b[3 * nd + a];
+        }
+    }
+    block_partials<4>(v, partial, blockIdx.x);
+}
+
+__global__ void k_to_float(const double* __restrict__ a, float* __restrict__ b, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+__global__ void k_add_float(double* __restrict__ y, const float* __restrict__ d, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = y[i] + (double)d[i];
+}
+
+// Stop flag of the relative-update test (Stif_sol.f90:208-210) after a sweep.
+__global__ void k_sor_check(const double* __restrict__ partial, int nblocks, double tol2, int* stop, int* sweeps,
+                            double* err_out, int maxit) {
+    __shared__ double sh[kBlock][2];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
+        a += partial[(size_t)i * 4];
+        b += partial[(size_t)i * 4 + 1];
+    }
+    sh[threadIdx.x][0] = a;
+    sh[threadIdx.x][1] = b;
+    __syncthreads();
+    if (threadIdx.x == 0 && *stop == 0) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int t = 0; t < (int)blockDim.x; ++t) {
+            s0 += sh[t][0];
+            s1 += sh[t][1];
+        }
+        const double err = s1 > 0.0 ? s0 / s1 : 0.0;
+        *err_out = err;
+        *sweeps += 1;
+        if (err < tol2 || *sweeps >= maxit) *stop = 1;
+    }
+}
+
+// NR update of Stif_sol.f90:70-74.
+__global__ void k_nr_update(double* __restrict__ x, double* __restrict__ dx, const double* __restrict__ ddx,
+                            double alp2, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double xc = alp2 * ddx[i];
+        x[i] = x[i] - xc;
+        dx[i] = dx[i] - xc;
+    }
+}
+
+__global__ void k_norms2(const double* __restrict__ a, const double* __restrict__ b, int n, double* partial) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        v[0] += a[i] * a[i];
+        v[1] += b[i] * b[i];
+    }
+    block_partials<4>(v, partial, blockIdx.x);
+}
+
+// UPDATE_GPTV (Fasolt3.f90:254-284) on field-major arrays.
+__global__ void k_commit(int n, int nsv, double* eps, const double* dep, double* rt, const double* dr, double* ep,
+                         const double* de, double* st, const double* ds, double* sv, const double* dsv) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const size_t N = (size_t)n;
+    eps[g] = eps[g] + dep[g];
+    for (int j = 0; j < 3; ++j) rt[j * N + g] = rt[j * N + g] + dr[j * N + g];
+    for (int j = 0; j < 6; ++j) {
+        ep[j * N + g] = ep[j * N + g] + de[j * N + g];
+        st[j * N + g] = st[j * N + g] + ds[j * N + g];
+    }
+    for (int j = 0; j < nsv; ++j) sv[j * N + g] = sv[j * N + g] + dsv[j * N + g];
+}
+
+// Cumulative slip of Stif_sol.f90:80-84.
+__global__ void k_cumslip(int n, int nss, double* cumslip, const double* srate, double dt) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const size_t N = (size_t)n;
+    for (int k = 0; k < nss; ++k) cumslip[k * N + g] = cumslip[k * N + g] + (srate[k * N + g] * dt);
+}
+
+// DoReorient (Fasolt3.f90:79-178) with the material-1 twin tables.
+struct TwinTables {
+    double ssang[60];
+    double ab[60][3], an[60][3];
+};
+
+__global__ void k_reorient(int n, double* sv, int* twinsys, int* timestwinned, const TwinTables* __restrict__ T,
+                           int* count) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || twinsys[j] == 0) return;
+    const size_t N = (size_t)n;
+    const int i = twinsys[j] - 1;
+    const double e1 = sv[j], e2 = sv[N + j], e3 = sv[2 * N + j];
+    const double ang = T->ssang[i] * 3.14159 / 180.;
+    const double c1 = cos(e1), c2 = cos(e2), c3 = cos(e3), ss1 = sin(e1), s2 = sin(e2), s3 = sin(e3);
+    double rm[3][3];
+    rm[0][0] = c1 * c3 - c2 * ss1 * s3;
+    rm[0][1] = -c1 * s3 - c2 * c3 * ss1;
+    rm[0][2] = ss1 * s2;
+    rm[1][0] = c3 * ss1 + c1 * c2 * s3;
+    rm[1][1] = c1 * c2 * c3 - ss1 * s3;
+    rm[1][2] = -c1 * s2;
+    rm[2][0] = s2 * s3;
+    rm[2][1] = c3 * s2;
+    rm[2][2] = c2;
+    double nv[3], dv[3];
+    for (int p = 0; p < 3; ++p) {
+        double sn = 0.0, sd = 0.0;
+        for (int q = 0; q < 3; ++q) {
+            sn = sn + rm[p][q] * T->an[i][q];
+            sd = sd + rm[p][q] * T->ab[i][q];
+        }
+        nv[p] = sn;
+        dv[p] = sd;
+    }
+    const double u1 = nv[1] * dv[2] - nv[2] * dv[1];
+    const double u2 = nv[2] * dv[0] - nv[0] * dv[2];
+    const double u3 = nv[0] * dv[1] - nv[1] * dv[0];
+    const double c = cos(ang), s = sin(ang);
+    double erm[3][3];
+    erm[0][0] = c + u1 * u1 * (1. - c);
+    erm[0][1] = u1 * u2 * (1. - c) - u3 * s;
+    erm[0][2] = u1 * u3 * (1. - c) + u2 * s;
+    erm[1][0] = u1 * u2 * (1. - c) + u3 * s;
+    erm[1][1] = c + u2 * u2 * (1. - c);
+    erm[1][2] = u2 * u3 * (1. - c) - u1 * s;
+    erm[2][0] = u1 * u3 * (1. - c) - u2 * s;
+    erm[2][1] = u2 * u3 * (1. - c) + u1 * s;
+    erm[2][2] = c + u3 * u3 * (1. - c);
+    double frm[3][3];
+    for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) {
+            double sum = 0.0;
+            for (int k = 0; k < 3; ++k) sum = sum + erm[p][k] * rm[k][q];
+            frm[p][q] = sum;
+        }
+    const double e1n = atan2(frm[0][2], -frm[1][2]);
+    if (fabs(frm[2][2]) > 1.) frm[2][2] = frm[2][2] / fabs(frm[2][2]);
+    const double e2n = acos(frm[2][2]);
+    const double e3n = atan2(frm[2][0], frm[2][1]);
+    sv[j] = e1n;
+    sv[N + j] = e2n;
+    sv[2 * N + j] = e3n;
+    twinsys[j] = 0;
+    timestwinned[j] = timestwinned[j] + 1;
+    atomicAdd(count, 1);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static inline int nblk(size_t n, int b = kBlock) { return (int)((n + b - 1) / b); }
+
+template <typename T>
+static hipError_t dalloc(Ctx* c, T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e == hipSuccess) {
+        c->allocs.push_back((void*)*p);
+        e = hipMemsetAsync(*p, 0, n * sizeof(T), c->stream);
+    }
+    return e;
+}
+
+static double reduce_now(Ctx* c, int nblocks, double out[4]) {
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, c->stream, c->red.partial, nblocks, c->red.result);
+    (void)hipMemcpyAsync(c->red.host, c->red.result, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    for (int q = 0; q < 4; ++q) out[q] = c->red.host[q];
+    return out[0];
+}
+
+static float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+static int build_structure(Ctx* c, const cpfem_mesh* m, std::vector<int>& en, std::vector<int>& eg) {
+    const int nel = m->nlmnt, nnod = m->nnod;
+    // node neighbour lists (nodes sharing an element), self first
+    std::vector<std::vector<int>> nb(nnod);
+    for (int e = 0; e < nel; ++e)
+        for (int a = 0; a < 8; ++a)
+            for (int b = 0; b < 8; ++b) nb[en[8 * e + a]].push_back(en[8 * e + b]);
+    int S = 0;
+    size_t nnzb = 0;
+    for (int i = 0; i < nnod; ++i) {
+        auto& v = nb[i];
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        v.erase(std::remove(v.begin(), v.end(), i), v.end());
+        v.insert(v.begin(), i);
+        S = std::max<int>(S, (int)v.size());
+        nnzb += v.size();
+    }
+    // greedy node colouring (nodes sharing an element differ)
+    std::vector<int> col(nnod, -1);
+    int ncol = 0;
+    std::vector<char> used;
+    for (int i = 0; i < nnod; ++i) {
+        used.assign(64, 0);
+        for (int j : nb[i])
+            if (j != i && col[j] >= 0 && col[j] < 64) used[col[j]] = 1;
+        int k = 0;
+        while (k < 64 && used[k]) ++k;
+        if (k == 64) return CPFEM_E_ARG;
+        col[i] = k;
+        ncol = std::max(ncol, k + 1);
+    }
+    std::vector<int> row_node(nnod), node_row(nnod);
+    c->A.cstart.assign(ncol + 1, 0);
+    for (int i = 0; i < nnod; ++i) c->A.cstart[col[i] + 1]++;
+    for (int k = 0; k < ncol; ++k) c->A.cstart[k + 1] += c->A.cstart[k];
+    {
+        std::vector<int> pos(c->A.cstart.begin(), c->A.cstart.end() - 1);
+        for (int i = 0; i < nnod; ++i) {
+            const int r = pos[col[i]]++;
+            row_node[r] = i;
+            node_row[i] = r;
+        }
+    }
+    const int R = nnod;
+    std::vector<int> cols((size_t)S * R);
+    for (int r = 0; r < R; ++r) {
+        const int nd = row_node[r];
+        for (int s = 0; s < S; ++s) cols[(size_t)s * R + r] = s < (int)nb[nd].size() ? nb[nd][s] : nd;
+    }
+    // element scatter map: slot*9*R + row for each node pair
+    std::vector<int> scat((size_t)nel * 64);
+    for (int e = 0; e < nel; ++e)
+        for (int a = 0; a < 8; ++a) {
+            const int ni = en[8 * e + a];
+            const auto& v = nb[ni];
+            for (int b = 0; b < 8; ++b) {
+                const int nj = en[8 * e + b];
+                int slot = 0;
+                if (nj != ni) slot = (int)(std::lower_bound(v.begin() + 1, v.end(), nj) - v.begin());
+                const size_t off = (size_t)slot * 9 * R + node_row[ni];
+                if (off > 0x7fffffffull) return CPFEM_E_ARG;
+                scat[64 * e + 8 * a + b] = (int)off;
+            }
+        }
+    // element colouring (elements sharing a node differ)
+    std::vector<std::vector<int>> node_elems(nnod);
+    for (int e = 0; e < nel; ++e)
+        for (int a = 0; a < 8; ++a) node_elems[en[8 * e + a]].push_back(e);
+    std::vector<int> ecol(nel, -1);
+    int necol = 0;
+    for (int e = 0; e < nel; ++e) {
+        used.assign(256, 0);
+        for (int a = 0; a < 8; ++a)
+            for (int f : node_elems[en[8 * e + a]])
+                if (ecol[f] >= 0) used[ecol[f]] = 1;
+        int k = 0;
+        while (k < 256 && used[k]) ++k;
+        if (k == 256) return CPFEM_E_ARG;
+        ecol[e] = k;
+        necol = std::max(necol, k + 1);
+    }
+    c->ecstart.assign(necol + 1, 0);
+    for (int e = 0; e < nel; ++e) c->ecstart[ecol[e] + 1]++;
+    for (int k = 0; k < necol; ++k) c->ecstart[k + 1] += c->ecstart[k];
+    std::vector<int> ebc(nel);
+    {
+        std::vector<int> pos(c->ecstart.begin(), c->ecstart.end() - 1);
+        for (int e = 0; e < nel; ++e) ebc[pos[ecol[e]]++] = e;
+    }
+    c->A.R = R;
+    c->A.S = S;
+    c->A.ncolor = ncol;
+    c->A.nnzb = nnzb;
+    const size_t vbytes = (c->precision == 1) ? sizeof(float) : sizeof(double);
+    if (dalloc(c, &c->A.row_node, R) || dalloc(c, &c->A.node_row, R) || dalloc(c, &c->A.cols, (size_t)S * R) ||
+        dalloc(c, &c->A.scatter, (size_t)nel * 64) || dalloc(c, &c->elem_by_color, nel))
+        return CPFEM_E_HIP;
+    char* vp = nullptr;
+    if (dalloc(c, &vp, (size_t)S * 9 * R * vbytes)) return CPFEM_E_HIP;
+    c->A.vals = vp;
+    CPFEM_HIP(hipMemcpyAsync(c->A.row_node, row_node.data(), R * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    CPFEM_HIP(hipMemcpyAsync(c->A.node_row, node_row.data(), R * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    CPFEM_HIP(hipMemcpyAsync(c->A.cols, cols.data(), cols.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    CPFEM_HIP(hipMemcpyAsync(c->A.scatter, scat.data(), scat.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    CPFEM_HIP(hipMemcpyAsync(c->elem_by_color, ebc.data(), nel * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    CPFEM_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static void set_point_arrays(Ctx* c, PointArrays& P) {
+    memset(&P, 0, sizeof(P));
+    P.n = c->ngps;
+    P.st = c->st;
+    P.ds = c->ds;
+    P.sv = c->sv;
+    P.dsv = c->dsv;
+    P.t = c->t;
+    P.chi = c->chi;
+    P.deff = c->dep;
+    P.erepmd = c->erepmd;
+    P.srate = c->srate;
+    P.twinsys = c->twinsys;
+    P.timestwinned = c->timestwinned;
+    P.flops = c->flops;
+    P.transc = c->transc;
+    P.err = c->gperr;
+    P.tflag = c->flags;
+    P.errmax = c->flags + 1;
+    P.rng_epoch = c->rng_epoch;
+}
+
+}  // namespace cpfem
+
+using namespace cpfem;
+
+extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int32_t nmat, const cpfem_control* ctl,
+                            cpfem_ctx** out) {
+    if (!m || !mats || nmat < 1 || !ctl || !out) return CPFEM_E_ARG;
+    *out = nullptr;
+    const int nss = mats[0].nss;
+    for (int k = 1; k < nmat; ++k)
+        if (mats[k].nss != nss) return CPFEM_E_ARG;
+    if (material_bytes(nss) == 0) return CPFEM_E_NSS;
+    if (m->nsv != 3 + nss) return CPFEM_E_ARG;
+    if (ctl->precision == 1 && ctl->sor_kind != 1) return CPFEM_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CPFEM_E_NOGPU;
+    // element tables
+    const int nel = m->nlmnt;
+    std::vector<int> en((size_t)nel * 8), eg((size_t)nel * 8, -1), lt(nel), le(nel), np(nel);
+    std::vector<int> gpe(m->ngps, -1), gpl(m->ngps, -1);
+    for (int e = 0; e < nel; ++e) {
+        if (m->nd[e] != 8 || (m->lt[e] != 12 && m->lt[e] != 13)) return CPFEM_E_ARG;
+        if ((m->lt[e] == 12 && m->np[e] != 1) || (m->lt[e] == 13 && m->np[e] != 8)) return CPFEM_E_ARG;
+        if (m->le[e] == 0 || abs(m->le[e]) > 1) return CPFEM_E_ARG;  // one material set
+        lt[e] = m->lt[e];
+        le[e] = m->le[e];
+        np[e] = m->np[e];
+        for (int a = 0; a < 8; ++a) {
+            const int nd = m->ln[20 * e + a] - 1;
+            if (nd < 0 || nd >= m->nnod) return CPFEM_E_ARG;
+            en[8 * e + a] = nd;
+        }
+        for (int g = 0; g < np[e]; ++g) {
+            const int gp = m->lg[8 * e + g] - 1;
+            if (gp < 0 || gp >= m->ngps || gpe[gp] >= 0) return CPFEM_E_ARG;
+            eg[8 * e + g] = gp;
+            gpe[gp] = e;
+            gpl[gp] = g;
+        }
+    }
+    for (int g = 0; g < m->ngps; ++g)
+        if (gpe[g] < 0) return CPFEM_E_ARG;
+    cpfem_ctx* c = new cpfem_ctx();
+    c->nlmnt = nel;
+    c->nnod = m->nnod;
+    c->ngps = m->ngps;
+    c->nsv = m->nsv;
+    c->nss = nss;
+    c->ninc1 = m->ninc1;
+    c->nmat = nmat;
+    c->ctl = *ctl;
+    c->precision = ctl->precision;
+    c->tiem.assign(m->tiem, m->tiem + m->ninc1);
+    memset(&c->stats, 0, sizeof(c->stats));
+    int rc = 0;
+#define TRY(x)                      \
+    do {                            \
+        if ((x) != hipSuccess) {    \
+            rc = CPFEM_E_HIP;       \
+            goto fail;              \
+        }                           \
+    } while (0)
+    {
+        TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (int i = 0; i < 8; ++i) TRY(hipEventCreate(&c->ev[i]));
+        const size_t G = c->ngps, D = 3 * (size_t)c->nnod, nsv = c->nsv;
+        TRY(dalloc(c, &c->elem_nodes, en.size()));
+        TRY(dalloc(c, &c->elem_gps, eg.size()));
+        TRY(dalloc(c, &c->elem_lt, nel));
+        TRY(dalloc(c, &c->elem_le, nel));
+        TRY(dalloc(c, &c->elem_np, nel));
+        TRY(dalloc(c, &c->gp_elem, G));
+        TRY(dalloc(c, &c->gp_local, G));
+        TRY(dalloc(c, &c->ib, c->nnod));
+        TRY(dalloc(c, &c->x_spec, (size_t)m->ninc1 * 99 * 15));
+        TRY(dalloc(c, &c->f_spec, (size_t)m->ninc1 * 99 * 3));
+        TRY(hipMemcpyAsync(c->elem_nodes, en.data(), en.size() * 4, hipMemcpyHostToDevice, c->stream));
+        TRY(hipMemcpyAsync(c->elem_gps, eg.data(), eg.size() * 4, hipMemcpyHostToDevice, c->stream));
+        TRY(hipMemcpyAsync(c->elem_lt, lt.data(), nel * 4, hipMemcpyHostToDevice, c->stream));
+        TRY(hipMemcpyAsync(c->elem_le, le.data(), nel * 4, hipMemcpyHostToDevice, c->stream));
+        TRY(hipMemcpyAsync(c->elem_np, np.data(), nel * 4, hipMemcpyHostToDevice, c->stream));
+        TRY(hipMemcpyAsync(c->gp_elem, gpe.data(), G * 4, hipMemcpyHostToDevice, c->stream));
+        TRY(hipMemcpyAsync(c->gp_local, gpl.data(), G * 4, hipMemcpyHostToDevice, c->stream));
+        TRY(hipMemcpyAsync(c->ib, m->ib, c->nnod * 4, hipMemcpyHostToDevice, c->stream));
+        TRY(hipMemcpyAsync(c->x_spec, m->x_spec, (size_t)m->ninc1 * 99 * 15 * 8, hipMemcpyHostToDevice, c->stream));
+        TRY(hipMemcpyAsync(c->f_spec, m->f_spec, (size_t)m->ninc1 * 99 * 3 * 8, hipMemcpyHostToDevice, c->stream));
+        TRY(dalloc(c, &c->x, D));
+        TRY(dalloc(c, &c->dx, D));
+        TRY(dalloc(c, &c->fc, D));
+        TRY(dalloc(c, &c->rhs, D));
+        TRY(dalloc(c, &c->ddx, D));
+        TRY(dalloc(c, &c->xold, D));
+        if (c->precision == 1) {
+            TRY(dalloc(c, &c->rhs32, D));
+            TRY(dalloc(c, &c->del32, D));
+        }
+        TRY(dalloc(c, &c->eps, G));
+        TRY(dalloc(c, &c->dep, G));
+        TRY(dalloc(c, &c->st, 6 * G));
+        TRY(dalloc(c, &c->ep, 6 * G));
+        TRY(dalloc(c, &c->ds, 6 * G));
+        TRY(dalloc(c, &c->de, 6 * G));
+        TRY(dalloc(c, &c->t, 6 * G));
+        TRY(dalloc(c, &c->rt, 3 * G));
+        TRY(dalloc(c, &c->dr, 3 * G));
+        TRY(dalloc(c, &c->sv, nsv * G));
+        TRY(dalloc(c, &c->dsv, nsv * G));
+        TRY(dalloc(c, &c->srate, (size_t)nss * G));
+        TRY(dalloc(c, &c->cumslip, (size_t)nss * G));
+        TRY(dalloc(c, &c->chi, 36 * G));
+        TRY(dalloc(c, &c->erepmd, G));
+        TRY(dalloc(c, &c->twinsys, G));
+        TRY(dalloc(c, &c->timestwinned, G));
+        TRY(dalloc(c, &c->gperr, G));
+        TRY(dalloc(c, &c->flops, G));
+        TRY(dalloc(c, &c->transc, G));
+        TRY(dalloc(c, &c->flags, 8));
+        // reduction scratch: enough partial slots for the largest launch
+        const int nbr = std::max(nblk(std::max<size_t>(G, D)) + 64, 4096);
+        c->red.nblocks = nbr;
+        TRY(dalloc(c, &c->red.partial, (size_t)nbr * 4));
+        TRY(dalloc(c, &c->red.result, 8));
+        TRY(hipHostMalloc((void**)&c->red.host, 8 * sizeof(double), 0));
+        // material
+        {
+            const size_t mb = material_bytes(nss);
+            std::vector<char> hm(mb);
+            build_material(nss, hm.data(), mats[0], ctl->toler);
+            char* dm = nullptr;
+            TRY(dalloc(c, &dm, mb));
+            c->dmat = dm;
+            TRY(hipMemcpyAsync(dm, hm.data(), mb, hipMemcpyHostToDevice, c->stream));
+            TRY(hipStreamSynchronize(c->stream));
+        }
+        rc = build_structure(c, m, en, eg);
+        if (rc) goto fail;
+    }
+    *out = c;
+    return 0;
+fail:
+    cpfem_destroy(c);
+    return rc ? rc : CPFEM_E_HIP;
+#undef TRY
+}
+
+extern "C" int cpfem_destroy(cpfem_ctx* c) {
+    if (!c) return 0;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->red.host) (void)hipHostFree(c->red.host);
+    for (int i = 0; i < 8; ++i)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+// field-major <-> reference per-point layout
+static void to_field(std::vector<double>& out, const double* in, size_t n, int k, int ld) {
+    out.resize(n * k);
+    for (size_t g = 0; g < n; ++g)
+        for (int i = 0; i < k; ++i) out[(size_t)i * n + g] = in[(size_t)ld * g + i];
+}
+static void from_field(double* out, const std::vector<double>& in, size_t n, int k, int ld) {
+    for (size_t g = 0; g < n; ++g)
+        for (int i = 0; i < k; ++i) out[(size_t)ld * g + i] = in[(size_t)i * n + g];
+}
+
+extern "C" int cpfem_set_state(cpfem_ctx* c, const double* x, const double* dx, const double* eps,
+                               const double* dep, const double* sv, const double* dsv, int32_t ldsv, const double* ep,
+                               const double* de, const double* st, const double* ds, const double* rt,
+                               const double* dr, const int32_t* twinsys, const int32_t* timestwinned) {
+    if (!c || ldsv < c->nsv) return CPFEM_E_ARG;
+    const size_t G = c->ngps, D = 3 * (size_t)c->nnod;
+    hipStream_t s = c->stream;
+    std::vector<double> tmp;
+    if (x) CPFEM_HIP(hipMemcpy(c->x, x, D * 8, hipMemcpyHostToDevice));
+    if (dx) CPFEM_HIP(hipMemcpy(c->dx, dx, D * 8, hipMemcpyHostToDevice));
+    if (eps) CPFEM_HIP(hipMemcpy(c->eps, eps, G * 8, hipMemcpyHostToDevice));
+    if (dep) CPFEM_HIP(hipMemcpy(c->dep, dep, G * 8, hipMemcpyHostToDevice));
+    struct F { const double* h; double* d; int k, ld; } fs[] = {
+        {sv, c->sv, c->nsv, ldsv}, {dsv, c->dsv, c->nsv, ldsv}, {ep, c->ep, 6, 6}, {de, c->de, 6, 6},
+        {st, c->st, 6, 6}, {ds, c->ds, 6, 6}, {rt, c->rt, 3, 3}, {dr, c->dr, 3, 3}};
+    for (auto& f : fs) {
+        if (!f.h) continue;
+        to_field(tmp, f.h, G, f.k, f.ld);
+        CPFEM_HIP(hipMemcpy(f.d, tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice));
+    }
+    if (twinsys) CPFEM_HIP(hipMemcpy(c->twinsys, twinsys, G * 4, hipMemcpyHostToDevice));
+    if (timestwinned) CPFEM_HIP(hipMemcpy(c->timestwinned, timestwinned, G * 4, hipMemcpyHostToDevice));
+    (void)s;
+    return 0;
+}
+
+extern "C" int cpfem_get_state(cpfem_ctx* c, double* x, double* dx, double* fc, double* eps, double* dep,
+                               double* sv, double* dsv, int32_t ldsv, double* ep, double* de, double* st, double* ds,
+                               double* rt, double* dr, double* srate, double* cumslip, int32_t* twinsys,
+                               int32_t* timestwinned) {
+    if (!c || ldsv < c->nsv) return CPFEM_E_ARG;
+    CPFEM_HIP(hipStreamSynchronize(c->stream));
+    const size_t G = c->ngps, D = 3 * (size_t)c->nnod;
+    std::vector<double> tmp;
+    if (x) CPFEM_HIP(hipMemcpy(x, c->x, D * 8, hipMemcpyDeviceToHost));
+    if (dx) CPFEM_HIP(hipMemcpy(dx, c->dx, D * 8, hipMemcpyDeviceToHost));
+    if (fc) CPFEM_HIP(hipMemcpy(fc, c->fc, D * 8, hipMemcpyDeviceToHost));
+    if (eps) CPFEM_HIP(hipMemcpy(eps, c->eps, G * 8, hipMemcpyDeviceToHost));
+    if (dep) CPFEM_HIP(hipMemcpy(dep, c->dep, G * 8, hipMemcpyDeviceToHost));
+    struct F { double* h; const double* d; int k, ld; } fs[] = {
+        {sv, c->sv, c->nsv, ldsv}, {dsv, c->dsv, c->nsv, ldsv}, {ep, c->ep, 6, 6}, {de, c->de, 6, 6},
+        {st, c->st, 6, 6}, {ds, c->ds, 6, 6}, {rt, c->rt, 3, 3}, {dr, c->dr, 3, 3},
+        {srate, c->srate, c->nss, CPFEM_MAXSS}, {cumslip, c->cumslip, c->nss, CPFEM_MAXSS}};
+    for (auto& f : fs) {
+        if (!f.h) continue;
+        tmp.resize(G * f.k);
+        CPFEM_HIP(hipMemcpy(tmp.data(), f.d, tmp.size() * 8, hipMemcpyDeviceToHost));
+        from_field(f.h, tmp, G, f.k, f.ld);
+    }
+    if (twinsys) CPFEM_HIP(hipMemcpy(twinsys, c->twinsys, G * 4, hipMemcpyDeviceToHost));
+    if (timestwinned) CPFEM_HIP(hipMemcpy(timestwinned, c->timestwinned, G * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int cpfem_update_node(cpfem_ctx* c) {
+    if (!c) return CPFEM_E_ARG;
+    const int D = 3 * c->nnod;
+    hipLaunchKernelGGL(k_axpy_nodes, dim3(nblk(D)), dim3(kBlock), 0, c->stream, c->x, (const double*)c->dx, 1.0, D);
+    CPFEM_HIP(hipGetLastError());
+    return 0;
+}
+
+// ELVALS: point kernel, then zero fc/matrix and the colour-ordered assembly.
+static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
+    if (inc < 1 || inc > c->ninc1) return CPFEM_E_ARG;
+    hipStream_t s = c->stream;
+    const double dt = c->tiem[inc - 1];
+    c->rng_epoch += 1;
+    CPFEM_HIP(hipEventRecord(c->ev[0], s));
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, c->flags + 1, (size_t)1, 0);
+    PointMeshArgs PA;
+    PA.ngps = c->ngps;
+    PA.gp_elem = c->gp_elem;
+    PA.gp_local = c->gp_local;
+    PA.elem_nodes = c->elem_nodes;
+    PA.elem_lt = c->elem_lt;
+    PA.elem_le = c->elem_le;
+    PA.x = c->x;
+    PA.dx = c->dx;
+    PA.smale = c->ctl.smale;
+    PA.smalr = c->ctl.smalr;
+    PA.de = c->de;
+    PA.dr = c->dr;
+    PA.dep = c->dep;
+    set_point_arrays(c, PA.P);
+    const DevControl C = make_control(c->ctl);
+    const int gb = nblk(c->ngps, 64);
+    if (c->nss == 12)
+        hipLaunchKernelGGL(k_points_mesh<12>, dim3(gb), dim3(64), 0, s, (const DevMaterial<12>*)c->dmat, C, dt, PA);
+    else if (c->nss == 18)
+        hipLaunchKernelGGL(k_points_mesh<18>, dim3(gb), dim3(64), 0, s, (const DevMaterial<18>*)c->dmat, C, dt, PA);
+    else
+        return CPFEM_E_NSS;
+    CPFEM_HIP(hipGetLastError());
+    CPFEM_HIP(hipEventRecord(c->ev[1], s));
+    // zero fc and the matrix values
+    const size_t D = 3 * (size_t)c->nnod, nv = (size_t)c->A.S * 9 * c->A.R;
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, s, c->fc, D, 0.0);
+    if (c->precision == 1) {
+        hipLaunchKernelGGL(k_fill<float>, dim3(std::min(nblk(nv), 65536)), dim3(kBlock), 0, s, (float*)c->A.vals, nv,
+                           0.0f);
+    } else {
+        hipLaunchKernelGGL(k_fill<double>, dim3(std::min(nblk(nv), 65536)), dim3(kBlock), 0, s, (double*)c->A.vals, nv,
+                           0.0);
+    }
+    AsmArgs AA;
+    AA.R = c->A.R;
+    AA.elem_nodes = c->elem_nodes;
+    AA.elem_gps = c->elem_gps;
+    AA.elem_lt = c->elem_lt;
+    AA.elem_np = c->elem_np;
+    AA.scatter = c->A.scatter;
+    AA.x = c->x;
+    AA.chi = c->chi;
+    AA.t = c->t;
+    AA.ngps = c->ngps;
+    AA.fc = c->fc;
+    for (size_t k = 0; k + 1 < c->ecstart.size(); ++k) {
+        AA.nel_color = c->ecstart[k + 1] - c->ecstart[k];
+        AA.elems = c->elem_by_color + c->ecstart[k];
+        const int nb = (AA.nel_color + 3) / 4;
+        if (c->precision == 1)
+            hipLaunchKernelGGL(k_assemble<float>, dim3(nb), dim3(256), 0, s, AA, (float*)c->A.vals);
+        else
+            hipLaunchKernelGGL(k_assemble<double>, dim3(nb), dim3(256), 0, s, AA, (double*)c->A.vals);
+    }
+    CPFEM_HIP(hipGetLastError());
+    CPFEM_HIP(hipEventRecord(c->ev[2], s));
+    // reductions for the report
+    const int nb = std::min(nblk(c->ngps), c->red.nblocks);
+    hipLaunchKernelGGL(k_point_sums, dim3(nb), dim3(kBlock), 0, s, (const double*)c->erepmd,
+                       (const long long*)c->flops, (const long long*)c->transc, c->ngps, c->red.partial);
+    double r[4];
+    reduce_now(c, nb, r);
+    int fl[2];
+    CPFEM_HIP(hipMemcpy(fl, c->flags, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    c->stats.sumcpiter = r[0];
+    c->stats.flops_const += (int64_t)llround(r[1]);
+    c->stats.transc_const += (int64_t)llround(r[2]);
+    c->stats.gp_updates += c->ngps;
+    c->stats.t_const_ms += elapsed(c->ev[0], c->ev[1]);
+    c->stats.t_asm_ms += elapsed(c->ev[1], c->ev[2]);
+    if (sumcpiter) *sumcpiter = r[0];
+    if (tflag) *tflag = fl[0];
+    if (fl[1] > c->stats.err) c->stats.err = fl[1];
+    return fl[1];
+}
+
+extern "C" int cpfem_elvals(cpfem_ctx* c, int32_t inc, double* sumcpiter, int32_t* tflag) {
+    if (!c) return CPFEM_E_ARG;
+    return elvals_impl(c, inc, sumcpiter, tflag);
+}
+
+static int suvals_impl(Ctx* c, int inc, double* sfc) {
+    if (inc < 1 || inc > c->ninc1) return CPFEM_E_ARG;
+    hipStream_t s = c->stream;
+    CPFEM_HIP(hipEventRecord(c->ev[3], s));
+    BcArgs B;
+    B.nnod = c->nnod;
+    B.R = c->A.R;
+    B.ib = c->ib;
+    B.node_row = c->A.node_row;
+    B.x = c->x;
+    B.dx = c->dx;
+    B.fc = c->fc;
+    B.xs = c->x_spec + (size_t)(inc - 1) * 99 * 15;
+    B.fs = c->f_spec + (size_t)(inc - 1) * 99 * 3;
+    B.efix = c->ctl.efix;
+    B.rhs = c->rhs;
+    const int nb = nblk(c->nnod);
+    if (c->precision == 1)
+        hipLaunchKernelGGL(k_suvals<float>, dim3(nb), dim3(kBlock), 0, s, B, (float*)c->A.vals, (double*)nullptr,
+                           c->red.partial);
+    else
+        hipLaunchKernelGGL(k_suvals<double>, dim3(nb), dim3(kBlock), 0, s, B, (double*)c->A.vals, (double*)nullptr,
+                           c->red.partial);
+    CPFEM_HIP(hipGetLastError());
+    double r[4];
+    reduce_now(c, nb, r);
+    CPFEM_HIP(hipEventRecord(c->ev[4], s));
+    CPFEM_HIP(hipEventSynchronize(c->ev[4]));
+    c->stats.t_suvals_ms += elapsed(c->ev[3], c->ev[4]);
+    c->stats.sfc = sqrt(r[0]);
+    if (sfc) *sfc = c->stats.sfc;
+    return 0;
+}
+
+extern "C" int cpfem_suvals(cpfem_ctx* c, int32_t inc, double* sfc) {
+    if (!c) return CPFEM_E_ARG;
+    return suvals_impl(c, inc, sfc);
+}
+
+// Number of sweeps queued between host checks of the stop flag.
+static const int kSweepBatch = 8;
+
+template <typename T, typename V>
+static void sor_sweep(Ctx* c, const T* vals, const V* b, V* y, double* partial) {
+    SorArgs S;
+    S.R = c->A.R;
+    S.S = c->A.S;
+    S.row_node = c->A.row_node;
+    S.cols = c->A.cols;
+    S.omega = c->ctl.sor_omega;
+    S.stop = c->flags + 2;
+    int slot = 0;
+    for (int k = 0; k < c->A.ncolor; ++k) {
+        S.r0 = c->A.cstart[k];
+        S.r1 = c->A.cstart[k + 1];
+        const int nb = nblk(S.r1 - S.r0);
+        hipLaunchKernelGGL((k_sor_color<T, V>), dim3(nb), dim3(kBlock), 0, c->stream, S, vals, b, y, partial, slot);
+        slot += nb;
+    }
+}
+
+static int sor_partial_blocks(Ctx* c) {
+    int slot = 0;
+    for (int k = 0; k < c->A.ncolor; ++k) slot += nblk(c->A.cstart[k + 1] - c->A.cstart[k]);
+    return slot;
+}
+
+static double residual(Ctx* c, const double* b, const double* y, double* res) {
+    SorArgs S;
+    S.R = c->A.R;
+    S.S = c->A.S;
+    S.row_node = c->A.row_node;
+    S.cols = c->A.cols;
+    S.omega = c->ctl.sor_omega;
+    S.stop = nullptr;
+    const int nb = nblk(c->A.R);
+    if (c->precision == 1)
+        hipLaunchKernelGGL(k_residual<float>, dim3(nb), dim3(kBlock), 0, c->stream, S, (const float*)c->A.vals, b, y,
+                           res, c->red.partial);
+    else
+        hipLaunchKernelGGL(k_residual<double>, dim3(nb), dim3(kBlock), 0, c->stream, S, (const double*)c->A.vals, b, y,
+                           res, c->red.partial);
+    double r[4];
+    reduce_now(c, nb, r);
+    return r[1] > 0.0 ? sqrt(r[0] / r[1]) : 0.0;
+}
+
+// FACSLS: solve A ddx = rhs from a zero start.
+static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out) {
+    hipStream_t s = c->stream;
+    const int D = 3 * c->nnod;
+    const int maxit = c->ctl.sor_maxit;
+    CPFEM_HIP(hipEventRecord(c->ev[5], s));
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, s, c->ddx, (size_t)D, 0.0);
+    int sweeps = 0, code = 0;
+    double err = 0.0;
+    const int npb = sor_partial_blocks(c);
+    if (c->precision == 0 && c->ctl.sor_kind == 0) {
+        // the reference's relative-update test after every sweep, decided on the device
+        const double tol2 = c->ctl.sor_tol * c->ctl.sor_tol;
+        hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, c->flags + 2, (size_t)2, 0);
+        int hf[2] = {0, 0};
+        while (true) {
+            for (int k = 0; k < kSweepBatch; ++k) {
+                sor_sweep<double, double>(c, (const double*)c->A.vals, c->rhs, c->ddx, c->red.partial);
+                hipLaunchKernelGGL(k_sor_check, dim3(1), dim3(kBlock), 0, s, (const double*)c->red.partial, npb, tol2,
+                                   c->flags + 2, c->flags + 3, c->red.result + 4, maxit);
+            }
+            CPFEM_HIP(hipMemcpyAsync(hf, c->flags + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+            CPFEM_HIP(hipStreamSynchronize(s));
+            if (hf[0]) break;
+        }
+        sweeps = hf[1];
+        CPFEM_HIP(hipMemcpy(&err, c->red.result + 4, sizeof(double), hipMemcpyDeviceToHost));
+        if (err > 10.0 * tol2) code = 904;
+    } else {
+        // relative-residual test every kSweepBatch sweeps; in mixed precision the
+        // sweeps run in FP32 on the correction equation A d = b - A ddx and the
+        // residual is formed in FP64 (iterative refinement).
+        hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, c->flags + 2, (size_t)2, 0);
+        double rel = 1.0;
+        if (c->precision == 1) {
+            CPFEM_HIP(hipMemcpyAsync(c->xold, c->rhs, D * sizeof(double), hipMemcpyDeviceToDevice, s));
+        }
+        while (sweeps < maxit) {
+            const int nsw = std::min(kSweepBatch, maxit - sweeps);
+            if (c->precision == 1) {
+                hipLaunchKernelGGL(k_to_float, dim3(nblk(D)), dim3(kBlock), 0, s, (const double*)c->xold, c->rhs32, D);
+                hipLaunchKernelGGL(k_fill<float>, dim3(nblk(D)), dim3(kBlock), 0, s, c->del32, (size_t)D, 0.0f);
+                for (int k = 0; k < nsw; ++k)
+                    sor_sweep<float, float>(c, (const float*)c->A.vals, c->rhs32, c->del32, c->red.partial);
+                hipLaunchKernelGGL(k_add_float, dim3(nblk(D)), dim3(kBlock), 0, s, c->ddx, (const float*)c->del32, D);
+            } else {
+                for (int k = 0; k < nsw; ++k)
+                    sor_sweep<double, double>(c, (const double*)c->A.vals, c->rhs, c->ddx, c->red.partial);
+            }
+            sweeps += nsw;
+            rel = residual(c, c->rhs, c->ddx, c->precision == 1 ? c->xold : nullptr);
+            if (rel < c->ctl.sor_tol) break;
+        }
+        err = rel;
+        if (rel > 10.0 * c->ctl.sor_tol) code = 904;
+    }
+    CPFEM_HIP(hipEventRecord(c->ev[6], s));
+    CPFEM_HIP(hipEventSynchronize(c->ev[6]));
+    c->stats.t_sor_ms += elapsed(c->ev[5], c->ev[6]);
+    c->stats.sor_sweeps = sweeps;
+    c->stats.sor_sweeps_total += sweeps;
+    c->stats.sor_error = err;
+    if (sweeps_out) *sweeps_out = sweeps;
+    if (err_out) *err_out = err;
+    if (code > c->stats.err) c->stats.err = code;
+    return code;
+}
+
+extern "C" int cpfem_facsls(cpfem_ctx* c, int32_t* sweeps, double* error) {
+    if (!c) return CPFEM_E_ARG;
+    return facsls_impl(c, sweeps, error);
+}
+
+static int nr_update_impl(Ctx* c, double* sdx_out, double* alp2_out) {
+    hipStream_t s = c->stream;
+    const int D = 3 * c->nnod;
+    CPFEM_HIP(hipEventRecord(c->ev[5], s));
+    const int nb = std::min(nblk(D), c->red.nblocks);
+    hipLaunchKernelGGL(k_norms2, dim3(nb), dim3(kBlock), 0, s, (const double*)c->dx, (const double*)c->ddx, D,
+                       c->red.partial);
+    double r[4];
+    reduce_now(c, nb, r);
+    const double errdx = sqrt(r[0]), erddx = sqrt(r[1]);
+    const double sdx = (errdx == 0.0) ? c->ctl.toler : erddx / errdx;
+    const double alp2 = 1.0 / (1.0 + c->ctl.alpha * sdx);
+    hipLaunchKernelGGL(k_nr_update, dim3(nblk(D)), dim3(kBlock), 0, s, c->x, c->dx, (const double*)c->ddx, alp2, D);
+    CPFEM_HIP(hipGetLastError());
+    CPFEM_HIP(hipEventRecord(c->ev[6], s));
+    CPFEM_HIP(hipEventSynchronize(c->ev[6]));
+    c->stats.t_update_ms += elapsed(c->ev[5], c->ev[6]);
+    c->stats.sdx = sdx;
+    c->stats.alp2 = alp2;
+    if (sdx_out) *sdx_out = sdx;
+    if (alp2_out) *alp2_out = alp2;
+    return 0;
+}
+
+extern "C" int cpfem_nr_update(cpfem_ctx* c, double* sdx, double* alp2) {
+    if (!c) return CPFEM_E_ARG;
+    return nr_update_impl(c, sdx, alp2);
+}
+
+extern "C" int cpfem_nriter(cpfem_ctx* c, int32_t inc, cpfem_stats* st) {
+    if (!c) return CPFEM_E_ARG;
+    const auto t0 = std::chrono::steady_clock::now();
+    memset(&c->stats, 0, sizeof(c->stats));
+    int nit = 0, code = 0;
+    bool more = true;
+    while (more) {
+        ++nit;
+        double sumcp = 0.0, sfc = 0.0, sdx = 0.0, alp2 = 0.0;
+        int tflag = 0, sweeps = 0;
+        double serr = 0.0;
+        code = elvals_impl(c, inc, &sumcp, &tflag);
+        if (code > 500 || code < 0) break;
+        code = suvals_impl(c, inc, &sfc);
+        if (code) break;
+        code = facsls_impl(c, &sweeps, &serr);
+        if (code > 500 || code < 0) break;
+        code = nr_update_impl(c, &sdx, &alp2);
+        if (code) break;
+        c->stats.sumcpiter = sumcp / (double)c->ngps;
+        more = (sdx >= c->ctl.toler) && (nit < c->ctl.maxit);
+    }
+    c->stats.nit = nit;
+    if (code <= 500 && code >= 0) {
+        hipLaunchKernelGGL(k_cumslip, dim3(nblk(c->ngps)), dim3(kBlock), 0, c->stream, c->ngps, c->nss, c->cumslip,
+                           (const double*)c->srate, c->tiem[inc - 1]);
+        CPFEM_HIP(hipStreamSynchronize(c->stream));
+    }
+    c->stats.t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (code > c->stats.err) c->stats.err = code;
+    if (st) *st = c->stats;
+    return code;
+}
+
+extern "C" int cpfem_update_gptv(cpfem_ctx* c) {
+    if (!c) return CPFEM_E_ARG;
+    hipLaunchKernelGGL(k_commit, dim3(nblk(c->ngps)), dim3(kBlock), 0, c->stream, c->ngps, c->nsv, c->eps,
+                       (const double*)c->dep, c->rt, (const double*)c->dr, c->ep, (const double*)c->de, c->st,
+                       (const double*)c->ds, c->sv, (const double*)c->dsv);
+    CPFEM_HIP(hipGetLastError());
+    return 0;
+}
+
+extern "C" int cpfem_reorient(cpfem_ctx* c, int32_t* n) {
+    if (!c) return CPFEM_E_ARG;
+    int fl = 0;
+    CPFEM_HIP(hipStreamSynchronize(c->stream));
+    CPFEM_HIP(hipMemcpy(&fl, c->flags, sizeof(int), hipMemcpyDeviceToHost));
+    int cnt = 0;
+    if (fl) {
+        // twin tables are only needed here; stage them once per call
+        static_assert(sizeof(TwinTables) == 60 * 8 * 7, "layout");
+        TwinTables* dT = nullptr;
+        int* dcount = nullptr;
+        CPFEM_HIP(hipMalloc((void**)&dT, sizeof(TwinTables)));
+        CPFEM_HIP(hipMalloc((void**)&dcount, sizeof(int)));
+        TwinTables h;
+        memset(&h, 0, sizeof(h));
+        // DevMaterial holds them in the same order: copy back from the device copy
+        {
+            std::vector<char> hm(material_bytes(c->nss));
+            CPFEM_HIP(hipMemcpy(hm.data(), c->dmat, hm.size(), hipMemcpyDeviceToHost));
+            if (c->nss == 12) {
+                auto* M = reinterpret_cast<DevMaterial<12>*>(hm.data());
+                for (int i = 0; i < 12; ++i) {
+                    h.ssang[i] = M->ssang[i];
+                    for (int k = 0; k < 3; ++k) {
+                        h.ab[i][k] = M->ab[i][k];
+                        h.an[i][k] = M->an[i][k];
+                    }
+                }
+            } else {
+                auto* M = reinterpret_cast<DevMaterial<18>*>(hm.data());
+                for (int i = 0; i < 18; ++i) {
+                    h.ssang[i] = M->ssang[i];
+                    for (int k = 0; k < 3; ++k) {
+                        h.ab[i][k] = M->ab[i][k];
+                        h.an[i][k] = M->an[i][k];
+                    }
+                }
+            }
+        }
+        CPFEM_HIP(hipMemcpy(dT, &h, sizeof(h), hipMemcpyHostToDevice));
+        CPFEM_HIP(hipMemset(dcount, 0, sizeof(int)));
+        hipLaunchKernelGGL(k_reorient, dim3(nblk(c->ngps)), dim3(kBlock), 0, c->stream, c->ngps, c->sv, c->twinsys,
+                           c->timestwinned, (const TwinTables*)dT, dcount);
+        CPFEM_HIP(hipStreamSynchronize(c->stream));
+        CPFEM_HIP(hipMemcpy(&cnt, dcount, sizeof(int), hipMemcpyDeviceToHost));
+        (void)hipFree(dT);
+        (void)hipFree(dcount);
+        CPFEM_HIP(hipMemset(c->flags, 0, sizeof(int)));
+    }
+    if (n) *n = cnt;
+    return 0;
+}
+
+extern "C" int cpfem_increment(cpfem_ctx* c, int32_t inc, cpfem_stats* st) {
+    if (!c) return CPFEM_E_ARG;
+    int rc = cpfem_update_node(c);
+    if (rc) return rc;
+    rc = cpfem_nriter(c, inc, st);
+    if (rc > 500 || rc < 0) return rc;
+    int r2 = cpfem_update_gptv(c);
+    if (r2) return r2;
+    r2 = cpfem_reorient(c, nullptr);
+    if (r2) return r2;
+    CPFEM_HIP(hipStreamSynchronize(c->stream));
+    return rc;
+}
+
+extern "C" int cpfem_get_system(cpfem_ctx* c, double* rhs, double* ddx) {
+    if (!c) return CPFEM_E_ARG;
+    CPFEM_HIP(hipStreamSynchronize(c->stream));
+    const size_t D = 3 * (size_t)c->nnod;
+    if (rhs) CPFEM_HIP(hipMemcpy(rhs, c->rhs, D * 8, hipMemcpyDeviceToHost));
+    if (ddx) CPFEM_HIP(hipMemcpy(ddx, c->ddx, D * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int cpfem_get_matrix(cpfem_ctx* c, int64_t* nnz, int32_t* rows, int32_t* cols, double* vals) {
+    if (!c || !nnz) return CPFEM_E_ARG;
+    CPFEM_HIP(hipStreamSynchronize(c->stream));
+    const int R = c->A.R, S = c->A.S;
+    std::vector<int> rn(R), cl((size_t)S * R);
+    CPFEM_HIP(hipMemcpy(rn.data(), c->A.row_node, R * 4, hipMemcpyDeviceToHost));
+    CPFEM_HIP(hipMemcpy(cl.data(), c->A.cols, cl.size() * 4, hipMemcpyDeviceToHost));
+    // count real blocks: slot 0 plus the slots whose column differs from the row's node
+    int64_t nb = 0;
+    for (int r = 0; r < R; ++r)
+        for (int s = 0; s < S; ++s)
+            if (s == 0 || cl[(size_t)s * R + r] != rn[r]) ++nb;
+    if (!rows || !cols || !vals) {
+        *nnz = nb * 9;
+        return 0;
+    }
+    if (*nnz < nb * 9) return CPFEM_E_ARG;
+    const size_t nv = (size_t)S * 9 * R;
+    std::vector<double> v(nv);
+    if (c->precision == 1) {
+        std::vector<float> f(nv);
+        CPFEM_HIP(hipMemcpy(f.data(), c->A.vals, nv * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < nv; ++i) v[i] = f[i];
+    } else {
+        CPFEM_HIP(hipMemcpy(v.data(), c->A.vals, nv * 8, hipMemcpyDeviceToHost));
+    }
+    int64_t k = 0;
+    for (int r = 0; r < R; ++r) {
+        const int nd = rn[r];
+        for (int s = 0; s < S; ++s) {
+            const int cn = (s == 0) ? nd : cl[(size_t)s * R + r];
+            if (s != 0 && cn == nd) continue;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    rows[k] = 3 * nd + a;
+                    cols[k] = 3 * cn + b;
+                    vals[k] = v[((size_t)s * 9 + 3 * a + b) * R + r];
+                    ++k;
+                }
+        }
+    }
+    *nnz = k;
+    return 0;
+}
+
+extern "C" int cpfem_get_stats(cpfem_ctx* c, cpfem_stats* st) {
+    if (!c || !st) return CPFEM_E_ARG;
+    *st = c->stats;
+    return 0;
+}

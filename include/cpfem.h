@@ -85,6 +85,89 @@ int cpfem_dstrss_batch(const cpfem_material *mat, const cpfem_control *ctl, doub
                        int32_t *twinsys, const int32_t *timestwinned,
                        int32_t *tflag, double *sumcpiter, int64_t *flops, int32_t *err);
 
+
+/* ---------------------------------------------------------------------- */
+/* Device-resident problem: the NRITER call sequence (Stif_sol.f90:10-88)  */
+/* ---------------------------------------------------------------------- */
+
+typedef struct cpfem_ctx cpfem_ctx;
+
+/* Mesh, boundary codes and load tables as read from the .fsd file
+ * (Faslt_io_nowin.f90:29-128).  Element types: 12 (8-node brick, one point,
+ * SHAPE_12 Shapef.f90:32) and 13 (8-node brick, 2x2x2 points). */
+typedef struct {
+    int32_t nlmnt, nnod, ngps, nsv;
+    const int32_t *le, *lt, *nd, *np;
+    const int32_t *ln;       /* [nlmnt][20] ln(1:20,el), 1-based */
+    const int32_t *lg;       /* [nlmnt][8]  lg(1:8,el), 1-based */
+    const int32_t *ib;       /* [nnod] boundary codes ccbbaa (Bconds.f90:8-12) */
+    int32_t ninc1, nxs, nfs;
+    const double *tiem;      /* [ninc1] time step of each increment */
+    const double *x_spec;    /* [ninc1][99][15] kinematic tables */
+    const double *f_spec;    /* [ninc1][99][3] load tables */
+} cpfem_mesh;
+
+/* Report values and timings of the last calls. */
+typedef struct {
+    int32_t nit;              /* NR iterations of the last cpfem_nriter */
+    int32_t sor_sweeps;       /* sweeps of the last FACSLS */
+    int32_t sor_sweeps_total; /* sweeps summed over the last cpfem_nriter */
+    int32_t err;              /* largest FAFERR code seen */
+    double sdx, sfc, sumcpiter, alp2, sor_error;
+    double t_const_ms;        /* point kernels (kinematics + constitutive) */
+    double t_asm_ms;          /* element assembly */
+    double t_suvals_ms;       /* boundary conditions */
+    double t_sor_ms;          /* SOR solve */
+    double t_update_ms;       /* norms, damping, update */
+    double t_total_ms;        /* whole call */
+    int64_t flops_const;      /* algorithmic FLOPs of the constitutive updates */
+    int64_t transc_const;     /* their transcendental calls */
+    int64_t gp_updates;       /* points updated (sum over NR iterations) */
+} cpfem_stats;
+
+/* Upload mesh, materials and control; build the block sparsity, colourings
+ * and element scatter map.  Replaces SETPTA (Stif_sol.f90:98), which the
+ * reference calls once after RDDATA (Fasolt3.f90:24). */
+int cpfem_create(const cpfem_mesh *mesh, const cpfem_material *mats, int32_t nmat,
+                 const cpfem_control *ctl, cpfem_ctx **out);
+int cpfem_destroy(cpfem_ctx *ctx);
+
+/* Upload / download the state of the .fsd/.frs records (Faslt_io_nowin.f90:45-65,
+ * 207-240): nodal x, dx (3*nnod) and the point arrays in the reference layout
+ * (eps, dep [ngps]; sv, dsv [ngps][ldsv]; ep, de, st, ds [ngps][6]; rt, dr [ngps][3]). */
+int cpfem_set_state(cpfem_ctx *ctx, const double *x, const double *dx, const double *eps, const double *dep,
+                    const double *sv, const double *dsv, int32_t ldsv, const double *ep, const double *de,
+                    const double *st, const double *ds, const double *rt, const double *dr,
+                    const int32_t *twinsys, const int32_t *timestwinned);
+int cpfem_get_state(cpfem_ctx *ctx, double *x, double *dx, double *fc, double *eps, double *dep, double *sv,
+                    double *dsv, int32_t ldsv, double *ep, double *de, double *st, double *ds, double *rt,
+                    double *dr, double *srate, double *cumslip, int32_t *twinsys, int32_t *timestwinned);
+
+/* UPDATE_NODE (Fasolt3.f90:231): x += dx. */
+int cpfem_update_node(cpfem_ctx *ctx);
+/* fc = 0, ss = 0 and ELVALS (Stif_sol.f90:38-42, Elvals.f90:11): every point's
+ * constitutive update, internal forces and the stiffness matrix. */
+int cpfem_elvals(cpfem_ctx *ctx, int32_t inc, double *sumcpiter, int32_t *tflag);
+/* SUVALS (Bconds.f90:19) and the free-node force norm of Stif_sol.f90:46-51. */
+int cpfem_suvals(cpfem_ctx *ctx, int32_t inc, double *sfc);
+/* FACSLS (Stif_sol.f90:161): solve for ddx. */
+int cpfem_facsls(cpfem_ctx *ctx, int32_t *sweeps, double *error);
+/* Norms, damping and update of Stif_sol.f90:55-74: x -= alp2*ddx, dx -= alp2*ddx. */
+int cpfem_nr_update(cpfem_ctx *ctx, double *sdx, double *alp2);
+/* The whole of NRITER for increment inc (Stif_sol.f90:10-88), cumulative slip included. */
+int cpfem_nriter(cpfem_ctx *ctx, int32_t inc, cpfem_stats *stats);
+/* UPDATE_GPTV (Fasolt3.f90:254): commit the point increments. */
+int cpfem_update_gptv(cpfem_ctx *ctx);
+/* DoReorient (Fasolt3.f90:79) if the twin flag is set; returns the count in *n. */
+int cpfem_reorient(cpfem_ctx *ctx, int32_t *n);
+/* One full increment: UPDATE_NODE, NRITER, UPDATE_GPTV, DoReorient (Fasolt3.f90:38-46). */
+int cpfem_increment(cpfem_ctx *ctx, int32_t inc, cpfem_stats *stats);
+/* Right-hand side and solution of the last solve (3*nnod each, may be NULL). */
+int cpfem_get_system(cpfem_ctx *ctx, double *rhs, double *ddx);
+/* Assembled matrix as COO in DOF numbering (0-based); call with NULL arrays to get *nnz. */
+int cpfem_get_matrix(cpfem_ctx *ctx, int64_t *nnz, int32_t *rows, int32_t *cols, double *vals);
+int cpfem_get_stats(cpfem_ctx *ctx, cpfem_stats *stats);
+
 /* Device name and compute units of the current device (diagnostics). */
 int cpfem_device_info(char *name, int32_t len, int32_t *cus);
 

@@ -36,3 +36,16 @@ def make_gp_batch(material, n, seed=0, elastic_frac=0.0, stress=50.0, strain=1.0
     dsv[:, 3:3 + nss] = rng.random((n, nss)) * 0.5
     dsv = f32(dsv)
     return dict(iel=iel, de=de, r=r, st=st, ds=ds, sv=sv, dsv=dsv)
+
+
+def mesh_state(prob, seed=6, grad_scale=1e-3, stress=30.0):
+    """A plastic-looking mesh state: smooth displacement increments plus stresses."""
+    rng = np.random.default_rng(seed)
+    s = prob.zero_state()
+    x = prob.x.reshape(-1, 3)
+    grad = rng.normal(scale=grad_scale, size=(3, 3))
+    s["x"] = prob.x.copy()
+    s["dx"] = f32((x @ grad.T).reshape(-1) + rng.normal(scale=1e-2 * grad_scale, size=prob.ndof))
+    s["st"] = f32(rng.normal(scale=stress, size=(prob.ngps, 6)))
+    s["ds"] = f32(rng.normal(scale=0.05 * stress, size=(prob.ngps, 6)))
+    return s

@@ -1,0 +1,112 @@
+"""Device-resident NRITER pieces against the FP64 oracle, through the C ABI.
+
+ELVALS (point updates, internal forces, stiffness) and SUVALS are compared
+entry by entry at 1e-10 relative; the solve is compared with a direct sparse
+solve of the same system (the multicolour sweep order changes the iterates,
+not the solution); whole increments are compared with the oracle's natural-
+order run at the solver tolerance.
+"""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from cpfem_code_amd import capi
+from cpfem_code_amd.problem import Control, LT_BRICK_1GP, LT_BRICK_8GP, fcc_plane_strain, fcc_tension
+from gpbatch import mesh_state
+from oracle import Oracle, OracleModel
+
+pytestmark = pytest.mark.gpu
+O64 = Oracle(64)
+
+
+def rel_max(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _pair(prob, state, precision=0):
+    om = OracleModel(O64, prob)
+    for k in ("x", "dx", "st", "ds"):
+        getattr(om, k)[:] = state[k]
+    sv = capi.Solver(prob, precision=precision)
+    sv.set_state(state)
+    return om, sv
+
+
+@pytest.mark.parametrize("lt", [LT_BRICK_8GP, LT_BRICK_1GP])
+def test_elvals_suvals_match_oracle(lt):
+    prob = fcc_tension(3, ninc=3, lt=lt, seed=5)
+    state = mesh_state(prob, seed=6)
+    om, sv = _pair(prob, state)
+    om.set_inc(1)
+    e_o = om.nrstep()
+    code, sumcp, tflag = sv.elvals(1)
+    sfc = sv.suvals(1)
+    assert code == e_o
+    g = sv.get_state()
+    for k in ("ds", "de", "dr", "dep"):
+        assert rel_max(g[k], getattr(om, k)) < 1e-10, k
+    nsv = prob.nsv
+    assert rel_max(g["dsv"][:, :nsv], om.dsv[:, :nsv]) < 1e-10
+    assert rel_max(g["fc"], om.fc) < 1e-10
+    rhs, _ = sv.system()
+    assert rel_max(rhs, om.ddx) < 1e-10
+    A_g = sv.matrix().toarray()
+    A_o = om.csr().toarray()
+    assert np.array_equal(A_g != 0, A_o != 0) or np.max(np.abs(A_g - A_o)) <= 1e-10 * np.max(np.abs(A_o))
+    assert np.max(np.abs(A_g - A_o) / np.maximum(np.abs(A_o), 1e-300 + 1e-12 * np.abs(A_o).max())) < 1e-10
+    assert abs(sumcp - om.m.sumcpiter) <= 1e-10 * max(1.0, abs(om.m.sumcpiter))
+    sv.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_facsls_solves_system(precision):
+    prob = fcc_tension(4, ninc=3, seed=5, control=Control(sor_kind=1, sor_tol=1e-9, sor_omega=1.6, sor_maxit=20000))
+    state = mesh_state(prob, seed=8)
+    sv = capi.Solver(prob, precision=precision)
+    sv.set_state(state)
+    sv.elvals(1)
+    sv.suvals(1)
+    code, sweeps, err = sv.facsls()
+    assert code == 0 and err < 1e-9
+    rhs, ddx = sv.system()
+    A = sv.matrix().tocsc()
+    direct = spla.spsolve(A, rhs)
+    # converged to the solver tolerance of the stored matrix
+    assert np.linalg.norm(A @ ddx - rhs) <= 2e-9 * np.linalg.norm(rhs)
+    assert rel_max(ddx, direct) < 1e-6
+    sv.close()
+
+
+def test_increments_match_oracle():
+    ctl = Control(sor_kind=1, sor_tol=1e-12, sor_omega=1.6, sor_maxit=50000)
+    prob = fcc_tension(3, ninc=3, seed=11, control=ctl)
+    om = OracleModel(O64, prob)
+    sv = capi.Solver(prob)
+    for inc in (1, 2, 3):
+        e_o = om.increment(inc)
+        st = sv.increment(inc)
+        assert st["code"] == e_o
+        assert st["nit"] == om.m.nit
+        g = sv.get_state()
+        assert rel_max(g["x"], om.x) < 1e-9
+        assert rel_max(g["dx"], om.dx) < 1e-6
+        assert rel_max(g["st"], om.st) < 1e-6
+        assert rel_max(g["sv"][:, :prob.nsv], om.sv[:, :prob.nsv]) < 1e-8
+    sv.close()
+
+
+def test_mixed_precision_increment_close_to_fp64():
+    ctl = Control(sor_kind=1, sor_tol=1e-8, sor_omega=1.6, sor_maxit=50000)
+    prob = fcc_plane_strain(4, ninc=2, seed=2, control=ctl)
+    a = capi.Solver(prob, precision=0)
+    b = capi.Solver(prob, precision=1)
+    for inc in (1, 2):
+        sa = a.increment(inc)
+        sb = b.increment(inc)
+        assert sa["code"] == 0 and sb["code"] == 0
+    ga, gb = a.get_state(), b.get_state()
+    # both Newton runs stop on |ddx|/|dx| < 1e-3; their converged fields agree to that order
+    assert rel_max(gb["x"] - prob.x, ga["x"] - prob.x) < 1e-3
+    assert rel_max(gb["st"], ga["st"]) < 1e-2
+    a.close()
+    b.close()
