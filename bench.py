@@ -1,0 +1,200 @@
+"""Benchmark: Gauss-point updates/s and s/increment of FASOLT3's hot path at 64^3 FCC.
+
+Workload (default): FCC aluminium, 64x64x64 8-node bricks with 2x2x2 points
+(262,144 elements, 2,097,152 points, 823,875 DOFs), seeded random texture,
+plane-strain compression -1e-3 per increment, SOR omega 1.6 to a relative
+residual of 1e-8.  One step = one load increment: UPDATE_NODE, the damped
+Newton loop (point updates, assembly, BCs, SOR solve, update) and the commit,
+with all state resident in HBM.  The FP64 path and the mixed path (FP32
+element products and SOR sweeps, FP64 residual refinement) are both timed;
+``value`` is the FP64 path's point-update rate.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 64]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12          # B/s
+FP64_PEAK = 78.6e12        # FLOP/s, vector
+FP32_PEAK = 157.3e12
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def sor_bytes(prob, nnzb, s_a, s_x):
+    """Algorithmic bytes of one SOR sweep: nnz*s_A + 4*nnzb + 4*(nnodes+1) + 3*ndof*s_x."""
+    return 9 * nnzb * s_a + 4 * nnzb + 4 * (prob.nnod + 1) + 3 * prob.ndof * s_x
+
+
+def nnzb_structured(dims):
+    nx, ny, nz = dims
+    def f(n):
+        return 3 * (n + 1) - 2
+    return f(nx) * f(ny) * f(nz)
+
+
+def run_path(prob, precision, warmup, steps, capi):
+    t0 = time.time()
+    sv = capi.Solver(prob, precision=precision)
+    log(f"[bench] precision={precision} setup {time.time() - t0:.1f} s")
+    import torch
+    inc = 0
+    for _ in range(warmup):
+        inc += 1
+        st = sv.increment(inc)
+        log(f"[bench] warmup inc {inc}: nit {st['nit']} sweeps {st['sor_sweeps_total']} "
+            f"{st['t_total_ms']:.1f} ms code {st['code']}")
+    torch.cuda.synchronize()
+    recs = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        inc += 1
+        st = sv.increment(inc)
+        recs.append(st)
+        log(f"[bench] inc {inc}: nit {st['nit']} sweeps {st['sor_sweeps_total']} total {st['t_total_ms']:.1f} ms "
+            f"const {st['t_const_ms']:.1f} asm {st['t_asm_ms']:.1f} sor {st['t_sor_ms']:.1f} code {st['code']}")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    state = sv.get_state()
+    sv.close()
+    return recs, wall, state
+
+
+def cpu_baseline(prob, state, seconds, capi_mod):
+    """Reference DSTRSS (REAL(4), compiled from the reference sources) on a sample of points."""
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import refio
+    if not refio.available():
+        return None
+    rng = np.random.default_rng(0)
+    n = 2000
+    idx = np.sort(rng.choice(prob.ngps, size=min(n, prob.ngps), replace=False))
+    iel = np.ones(len(idx), np.int32)
+    b = dict(de=state["de"][idx], r=state["dr"][idx], st=state["st"][idx] - state["ds"][idx],
+             ds=state["ds"][idx], sv=state["sv"][idx] - state["dsv"][idx], dsv=state["dsv"][idx])
+    mat, ctl = prob.materials[0], prob.control
+    done, t_total = 0, 0.0
+    while t_total < seconds:
+        t0 = time.perf_counter()
+        refio.dstrss([mat], ctl, float(prob.tiem[0]), iel, b["de"], b["r"], b["st"], b["ds"], b["sv"], b["dsv"])
+        t_total += time.perf_counter() - t0
+        done += len(idx)
+    return dict(value=done / t_total, unit="GP-updates/s", cores=1, kind="reference",
+                sample=f"{done} point updates (DSTRSS, REAL(4) reference compiled with flang -O2) on 1 core: "
+                       f"{len(idx)} points drawn from the benchmark state after the timed increments, "
+                       f"{t_total:.1f} s including the harness file I/O")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--paths", default="fp64,mixed")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md, "
+                                   "not launched", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "no GPU"
+    from cpfem_code_amd import capi
+    from cpfem_code_amd.problem import Control, fcc_plane_strain
+    name, cus = capi.device_info()
+    ctl = Control(sor_kind=1, sor_tol=1e-8, sor_omega=1.6, sor_maxit=200000)
+    t0 = time.time()
+    prob = fcc_plane_strain(args.n, ninc=args.warmup + args.steps + 1, seed=2, control=ctl)
+    log(f"[bench] {name} {cus} CUs; problem {args.n}^3: {prob.nlmnt} elements, {prob.ngps} points, "
+        f"{prob.ndof} DOFs (generated in {time.time() - t0:.1f} s)")
+    results = {}
+    last_state = None
+    for path in args.paths.split(","):
+        prec = 0 if path == "fp64" else 1
+        recs, wall, state = run_path(prob, prec, args.warmup, args.steps, capi)
+        results[path] = (recs, wall)
+        if prec == 0:
+            last_state = state
+    recs64, wall64 = results["fp64"]
+    nit = sum(r["nit"] for r in recs64)
+    gp_updates = sum(r["gp_updates"] for r in recs64)
+    t_const = sum(r["t_const_ms"] for r in recs64) * 1e-3
+    t_asm = sum(r["t_asm_ms"] for r in recs64) * 1e-3
+    t_sor = sum(r["t_sor_ms"] for r in recs64) * 1e-3
+    sweeps = sum(r["sor_sweeps_total"] for r in recs64)
+    flops = sum(r["flops_const"] for r in recs64)
+    transc = sum(r["transc_const"] for r in recs64)
+    ms_per_inc = 1e3 * wall64 / args.steps
+    value = gp_updates / wall64
+    nnzb = nnzb_structured(prob.dims)
+    # roofline of each hot kernel, algorithmic work / event-timed duration
+    asm_flops = 75968 * prob.nlmnt * nit
+    sor_b64 = sor_bytes(prob, nnzb, 8, 8) * sweeps
+    roof = {
+        "constitutive": {"bound": "mfma", "achieved": flops / t_const / 1e12, "peak": FP64_PEAK / 1e12,
+                         "unit": "TFLOP/s", "traffic": None},
+        "assembly": {"bound": "mfma", "achieved": asm_flops / t_asm / 1e12, "peak": FP64_PEAK / 1e12,
+                     "unit": "TFLOP/s", "traffic": None},
+        "sor": {"bound": "hbm", "achieved": sor_b64 / t_sor / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "traffic": None},
+    }
+    for v in roof.values():
+        v["frac"] = v["achieved"] / v["peak"]
+    dom = max((("constitutive", t_const), ("assembly", t_asm), ("sor", t_sor)), key=lambda kv: kv[1])[0]
+    extra = {}
+    if "mixed" in results:
+        recs32, wall32 = results["mixed"]
+        extra["s_per_increment_mixed"] = wall32 / args.steps
+        extra["mixed_sor_s_per_increment"] = sum(r["t_sor_ms"] for r in recs32) * 1e-3 / args.steps
+        extra["mixed_sweeps_per_increment"] = sum(r["sor_sweeps_total"] for r in recs32) / args.steps
+        extra["mixed_speedup_vs_fp64"] = wall64 / wall32
+    cpu = cpu_baseline(prob, last_state, args.cpu_seconds, capi) if args.cpu_seconds > 0 else None
+    line = {
+        "metric": "Gauss-pt updates/s + s/increment at 64^3 FCC, FP64 vs mixed-prec SOR",
+        "value": value,
+        "unit": "GP-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_inc,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded mesh, texture and load path; no external data)",
+        "config": {"workload": f"FCC Al plane-strain compression, {args.n}^3 8-node bricks x 8 points, "
+                               f"eps_zz=-1e-3/increment, SOR omega 1.6 to 1e-8 relative residual",
+                   "elements": prob.nlmnt, "points": prob.ngps, "dofs": prob.ndof},
+        "s_per_increment_fp64": wall64 / args.steps,
+        "gp_updates_per_s_constitutive": gp_updates / t_const,
+        "nr_iterations_per_increment": nit / args.steps,
+        "sor_sweeps_per_increment": sweeps / args.steps,
+        "assembly_s_per_increment": t_asm / args.steps,
+        "sor_s_per_increment": t_sor / args.steps,
+        "constitutive_s_per_increment": t_const / args.steps,
+        "constitutive_flops_per_point_update": flops / gp_updates,
+        "constitutive_transcendentals_per_point_update": transc / gp_updates,
+        "roofline": dict(roof[dom], kernel=dom),
+        "roofline_all": roof,
+        "cpu_baseline": cpu,
+        "device": name,
+    }
+    line.update(extra)
+    if cpu:
+        line["gp_rate_vs_cpu_reference"] = value / cpu["value"]
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -50,7 +50,31 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_host()
     return LIB
+
+
+FLANG = "/opt/rocm/lib/llvm/bin/flang"
+HOST_EXE = os.path.join(LIBDIR, "fasolt_gpu")
+
+
+def build_host() -> str:
+    """The Fortran driver (host/fasolt_gpu.f90) linked against libcpfem.so."""
+    if not os.path.exists(FLANG):
+        raise RuntimeError("flang not found: the Fortran host needs /opt/rocm/lib/llvm/bin/flang")
+    hostdir = os.path.join(HERE, "host")
+    tmp = os.path.join(LIBDIR, "_host_build")
+    os.makedirs(tmp, exist_ok=True)
+    objs = []
+    for src in ("cpfem_capi.f90", "fasolt_gpu.f90"):
+        obj = os.path.join(tmp, src.replace(".f90", ".o"))
+        subprocess.check_call([FLANG, "-O2", "-module-dir", tmp, "-c", os.path.join(hostdir, src), "-o", obj])
+        objs.append(obj)
+    subprocess.check_call([FLANG, "-o", HOST_EXE] + objs + ["-L" + LIBDIR, "-lcpfem", "-Wl,-rpath,$ORIGIN"])
+    for f in os.listdir(tmp):
+        os.remove(os.path.join(tmp, f))
+    os.rmdir(tmp)
+    return HOST_EXE
 
 
 if __name__ == "__main__":

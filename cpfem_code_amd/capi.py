@@ -275,10 +275,11 @@ class Solver:
     def update_node(self):
         _check(self.L.cpfem_update_node(self.h), "cpfem_update_node")
 
-    def elvals(self, inc):
+    def elvals(self, inc, raise_on_error=False):
         sp = np.zeros(1)
         tf = np.zeros(1, np.int32)
-        code = _check(self.L.cpfem_elvals(self.h, inc, _ptr(sp), _ptr(tf)), "cpfem_elvals")
+        code = self.L.cpfem_elvals(self.h, inc, _ptr(sp), _ptr(tf))
+        _check(code, "cpfem_elvals", ok_codes=() if raise_on_error else (code,))
         return code, float(sp[0]), int(tf[0])
 
     def suvals(self, inc):

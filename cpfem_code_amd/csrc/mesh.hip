@@ -525,6 +525,12 @@ __global__ void k_norms2(const double* __restrict__ a, const double* __restrict_
     block_partials<4>(v, partial, blockIdx.x);
 }
 
+__global__ void k_dot(const double* __restrict__ a, const double* __restrict__ b, int n, double* partial) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[0] += a[i] * b[i];
+    block_partials<4>(v, partial, blockIdx.x);
+}
+
 // UPDATE_GPTV (Fasolt3.f90:254-284) on field-major arrays.
 __global__ void k_commit(int n, int nsv, double* eps, const double* dep, double* rt, const double* dr, double* ep,
                          const double* de, double* st, const double* ds, double* sv, const double* dsv) {
@@ -1321,6 +1327,26 @@ extern "C" int cpfem_nriter(cpfem_ctx* c, int32_t inc, cpfem_stats* st) {
     if (code > c->stats.err) c->stats.err = code;
     if (st) *st = c->stats;
     return code;
+}
+
+extern "C" int cpfem_cumslip(cpfem_ctx* c, int32_t inc) {
+    if (!c || inc < 1 || inc > c->ninc1) return CPFEM_E_ARG;
+    hipLaunchKernelGGL(k_cumslip, dim3(nblk(c->ngps)), dim3(kBlock), 0, c->stream, c->ngps, c->nss, c->cumslip,
+                       (const double*)c->srate, c->tiem[inc - 1]);
+    CPFEM_HIP(hipGetLastError());
+    return 0;
+}
+
+extern "C" int cpfem_power(cpfem_ctx* c, double* fdx) {
+    if (!c || !fdx) return CPFEM_E_ARG;
+    const int D = 3 * c->nnod;
+    const int nb = std::min(nblk(D), c->red.nblocks);
+    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kBlock), 0, c->stream, (const double*)c->fc, (const double*)c->dx, D,
+                       c->red.partial);
+    double r[4];
+    reduce_now(c, nb, r);
+    *fdx = r[0];
+    return 0;
 }
 
 extern "C" int cpfem_update_gptv(cpfem_ctx* c) {

@@ -156,6 +156,10 @@ int cpfem_facsls(cpfem_ctx *ctx, int32_t *sweeps, double *error);
 int cpfem_nr_update(cpfem_ctx *ctx, double *sdx, double *alp2);
 /* The whole of NRITER for increment inc (Stif_sol.f90:10-88), cumulative slip included. */
 int cpfem_nriter(cpfem_ctx *ctx, int32_t inc, cpfem_stats *stats);
+/* Cumulative slip at the end of NRITER (Stif_sol.f90:80-84): cumslip += srate*tiem(inc). */
+int cpfem_cumslip(cpfem_ctx *ctx, int32_t inc);
+/* Power of the POWER report (Faslt_io_nowin.f90:353-366): sum(fc*dx). */
+int cpfem_power(cpfem_ctx *ctx, double *fdx);
 /* UPDATE_GPTV (Fasolt3.f90:254): commit the point increments. */
 int cpfem_update_gptv(cpfem_ctx *ctx);
 /* DoReorient (Fasolt3.f90:79) if the twin flag is set; returns the count in *n. */

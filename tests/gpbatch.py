@@ -38,7 +38,7 @@ def make_gp_batch(material, n, seed=0, elastic_frac=0.0, stress=50.0, strain=1.0
     return dict(iel=iel, de=de, r=r, st=st, ds=ds, sv=sv, dsv=dsv)
 
 
-def mesh_state(prob, seed=6, grad_scale=1e-3, stress=30.0):
+def mesh_state(prob, seed=6, grad_scale=2e-4, stress=30.0):
     """A plastic-looking mesh state: smooth displacement increments plus stresses."""
     rng = np.random.default_rng(seed)
     s = prob.zero_state()
