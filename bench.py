@@ -188,6 +188,7 @@ def main():
         "roofline": dict(roof[dom], kernel=dom),
         "roofline_all": roof,
         "cpu_baseline": cpu,
+        "faferr_codes_per_increment": [r["code"] for r in recs64],
         "device": name,
     }
     line.update(extra)

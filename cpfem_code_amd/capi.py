@@ -183,6 +183,12 @@ def _bind_mesh_api(L):
     L.cpfem_get_stats.argtypes = [_CTX, C.POINTER(CpfemStats)]
 
 
+# Codes after which the device Newton loop goes on (the caller decides, as the
+# reference's FAFERR would): 987 slip Newton residual, 910 singular slip
+# matrix, 904 SOR not converged.
+LOCAL_CODES = (987, 910, 904)
+
+
 def _check(code, where, ok_codes=()):
     if code < 0:
         raise RuntimeError(f"{where} failed ({code})")
@@ -301,7 +307,7 @@ class Solver:
 
     def nriter(self, inc):
         st = CpfemStats()
-        code = _check(self.L.cpfem_nriter(self.h, inc, C.byref(st)), "cpfem_nriter", ok_codes=(904,))
+        code = _check(self.L.cpfem_nriter(self.h, inc, C.byref(st)), "cpfem_nriter", ok_codes=LOCAL_CODES)
         d = st.as_dict()
         d["code"] = code
         return d
@@ -316,7 +322,7 @@ class Solver:
 
     def increment(self, inc):
         st = CpfemStats()
-        code = _check(self.L.cpfem_increment(self.h, inc, C.byref(st)), "cpfem_increment", ok_codes=(904,))
+        code = _check(self.L.cpfem_increment(self.h, inc, C.byref(st)), "cpfem_increment", ok_codes=LOCAL_CODES)
         d = st.as_dict()
         d["code"] = code
         return d

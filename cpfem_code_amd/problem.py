@@ -72,7 +72,7 @@ class Control:
     sor_omega: float = 1.2       # Stif_sol.f90:173
     sor_maxit: int = 1000        # Stif_sol.f90:173
     sor_tol: float = 1.0e-3      # reference: toler
-    sor_kind: int = 0            # 0: reference |dx|^2/|x|^2 < tol^2, 1: |b-Ax|/|b| < tol
+    sor_kind: int = 0            # 0: reference |dx|^2/|x|^2 < tol^2, 1: |D^-1(b-Ax)|/|D^-1 b| < tol, D = diag(A)
 
 
 @dataclasses.dataclass

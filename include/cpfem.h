@@ -52,7 +52,7 @@ typedef struct {
     double smald, smale, smalr;
     double sor_omega;     /* Stif_sol.f90:173 alphal */
     double sor_tol;       /* tolerance of the SOR stopping test */
-    int32_t sor_kind;     /* 0: |dx|^2/|x|^2 < tol^2 (Stif_sol.f90:208-210); 1: |b-Ax|/|b| < tol */
+    int32_t sor_kind;     /* 0: |dx|^2/|x|^2 < tol^2 (Stif_sol.f90:208-210); 1: |D^-1(b-Ax)|/|D^-1 b| < tol, D = diag(A) */
     int32_t precision;    /* 0: FP64 throughout; 1: FP32 element products and SOR sweeps, FP64 residuals */
 } cpfem_control;
 

@@ -29,6 +29,9 @@
 #include <stdlib.h>
 #include <string.h>
 #include <tgmath.h>
+#ifdef ORC_TRACE
+#include <stdio.h>
+#endif
 
 #ifndef ORC_REAL
 #define ORC_REAL double
@@ -429,6 +432,7 @@ static void slpini(cpwork *w, real dt, real *am) {
             w->isa[nsa++] = i;
     }
     w->nsa = nsa;
+    if (w->cnt && nsa > w->cnt->max_nsa) w->cnt->max_nsa = nsa;
     (void)nspow;
     count(w->cnt, 5 * (int64_t)ns, 2 * (int64_t)ns);
 }
@@ -605,6 +609,9 @@ static real cpstrs(cpwork *w, real dt, real el_comp[6][6], const real *del, cons
         for (int i = 0; i < 6; ++i) erreps[i] = scale * erreps[i];
         for (int i = 0; i < 6; ++i) s[i] = s[i] + erreps[i];
         l_iter = (iter < maxcpsit) && (erepmd > tolercp);
+#ifdef ORC_TRACE
+        fprintf(stderr, "outer %d nsa %d erepmd %.3e scale %.3e\n", iter, nsa, (double)erepmd, (double)scale);
+#endif
         count(w->cnt,
               12 * (int64_t)ns + 12 * (int64_t)ns + 90 + 72 * (int64_t)nsa + 36 + 44 + factor_flops(nsa) +
                   6 * solver_flops(nsa) + factor_flops(6) + solver_flops(6) + 28,
@@ -1109,9 +1116,14 @@ int orc_facsls(orc_model *m, int ndof) {
     int err = 0;
     for (int i = 0; i < ndof; ++i)
         if (fabs(R(SS(0, i))) == R(0.0)) err = 903;
+    /* kind 1: relative Jacobi-scaled residual |D^-1 (b - A x)| / |D^-1 b| (build's own
+     * criterion; the reference only has the relative-update test) */
     double bnorm = 0.0;
     if (m->ctl->sor_kind == 1)
-        for (int i = 0; i < ndof; ++i) bnorm += (double)y[i] * (double)y[i];
+        for (int i = 0; i < ndof; ++i) {
+            double yi = (double)y[i] / (double)R(SS(0, i));
+            bnorm += yi * yi;
+        }
     int iter = 0, conv = 0;
     real error = R(0.0);
     double rel = 0.0;
@@ -1134,12 +1146,12 @@ int orc_facsls(orc_model *m, int ndof) {
             error = num / den;
             conv = (error < tolerl);
         } else {
-            /* relative residual |b - A x| / |b|, accumulated in double */
+            /* relative Jacobi-scaled residual, accumulated in double */
             double rr = 0.0;
             for (int i = 0; i < ndof; ++i) {
                 double sum = (double)R(SS(0, i)) * (double)xv[i];
                 for (int j = 1; j <= IS(0, i); ++j) sum += (double)R(SS(j, i)) * (double)xv[IS(j, i) - 1];
-                double ri = (double)y[i] - sum;
+                double ri = ((double)y[i] - sum) / (double)R(SS(0, i));
                 rr += ri * ri;
             }
             rel = bnorm > 0.0 ? sqrt(rr / bnorm) : 0.0;

