@@ -62,7 +62,8 @@ def run_path(prob, precision, warmup, steps, capi):
         st = sv.increment(inc)
         recs.append(st)
         log(f"[bench] inc {inc}: nit {st['nit']} sweeps {st['sor_sweeps_total']} total {st['t_total_ms']:.1f} ms "
-            f"const {st['t_const_ms']:.1f} asm {st['t_asm_ms']:.1f} sor {st['t_sor_ms']:.1f} code {st['code']}")
+            f"const {st['t_const_ms']:.1f} asm {st['t_asm_ms']:.1f} sor {st['t_sor_ms']:.1f} code {st['code']} "
+            f"pts {st['gp_updates']} continued {st['pts_continued']} wide {st['pts_wide']}")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     state = sv.get_state()

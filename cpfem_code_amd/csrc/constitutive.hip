@@ -1,4 +1,4 @@
-// Constitutive kernels and the batch entry point of the C ABI.
+// Constitutive kernels, their multi-pass schedule and the batch entry point.
 #include <stdlib.h>
 #include <string.h>
 
@@ -7,68 +7,575 @@
 #include "common.hpp"
 #include "constitutive.hpp"
 #include "constitutive_launch.hpp"
+#include "kernels.hpp"
 
 namespace cpfem {
 
-// One lane per point; SoA arrays with stride n (field-major).
 template <int N>
-__global__ __launch_bounds__(64) void k_dstrss_points(const DevMaterial<N>* __restrict__ Mp, const DevControl C,
-                                                      const double dt, const PointArrays A) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= A.n) return;
-    const DevMaterial<N>& M = *Mp;
-    const size_t n = (size_t)A.n;
-    PointIO io;
-    io.iel = A.iel[g];
-    io.err = 0;
-    io.tflag = 0;
-    io.deff = 0.0;
+__device__ __forceinline__ double rate_m(const DevMaterial<N>& M, int i) {
+    return M.am[M.uniform_par ? 0 : i];
+}
+
+template <int N>
+__device__ __forceinline__ double latent(const DevMaterial<N>& M, int i, int j) {
+    if (M.two_valued) return (i == j) ? M.hd : M.hq;
+    return M.smatrx[i][j];
+}
+
+// Compacted slip state of one point (module FAS_CPL scratch of the reference).
+template <int AMAX>
+struct Compact {
+    int nsa;
+    int isa[AMAX];
+    double gam[AMAX], tau[AMAX], s0[AMAX], s1[AMAX];
+    double ah[AMAX][AMAX];
+};
+
+// PLASTC on the active set: theta at s0, s1 and the log-hardening matrix.
+// Slots >= nsa of ah are identity rows / columns.  th[] receives theta.
+template <int N, int AMAX>
+__device__ __forceinline__ void plastc_c(const DevMaterial<N>& M, Compact<AMAX>& S, double dt, double th[AMAX],
+                                         int64_t& flops, int64_t& transc) {
+    const int nsa = S.nsa;
+    double sumss = 0.0;
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        io.de[i] = A.de[i * n + g];
-        io.st[i] = A.st[i * n + g];
-        io.ds[i] = A.ds[i * n + g];
+    for (int k = 0; k < AMAX; ++k)
+        if (k < nsa) sumss = sumss + fabs(S.gam[k]);
+    const double rate = sumss / dt;
+#pragma unroll
+    for (int k = 0; k < AMAX; ++k) {
+        if (k < nsa) {
+            const int j = M.uniform_par ? 0 : S.isa[k];
+            const double sst = M.taus[j] * pow_fast(rate, M.nsat[j]);
+            double arg = (sst != 0.0) ? 1.0 - (S.s0[k] / sst) : 0.0;
+            if (arg < 0.0) arg = 0.0;
+            th[k] = M.th4[j] + M.th0[j] * copysign(pow_fast(fabs(arg), M.alp[j]), arg);
+        } else {
+            th[k] = 0.0;
+        }
     }
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        io.r[i] = A.r[i * n + g];
-        io.sv0[i] = A.sv[i * n + g];
-        io.dsv0[i] = A.dsv[i * n + g];
-    }
-    io.twinsys = A.twinsys[g];
-    io.timestwinned = A.timestwinned[g];
-    io.rng_key = (A.rng_epoch << 32) ^ (uint64_t)(g + A.gp_offset);
-    double s0[N], dsv[3 + N], srate[N];
+    for (int k = 0; k < AMAX; ++k) {
+        if (k < nsa) {
+            double s = S.s0[k];
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-        s0[i] = A.sv[(3 + i) * n + g];
-        srate[i] = A.srate[i * n + g];
+            for (int l = 0; l < AMAX; ++l)
+                if (l < nsa) s = s + (latent(M, S.isa[k], S.isa[l]) * th[l]) * fabs(S.gam[l]);
+            S.s1[k] = s;
+        }
     }
 #pragma unroll
-    for (int i = 0; i < 3 + N; ++i) dsv[i] = A.dsv[i * n + g];
-    Count cnt{0, 0, 0, 0};
-    dstrss<N>(M, C, dt, io, s0, dsv, srate, cnt);
+    for (int k = 0; k < AMAX; ++k) {
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        A.t[i * n + g] = io.t[i];
-        A.ds[i * n + g] = io.ds[i];
+        for (int l = 0; l < AMAX; ++l) {
+            double v = (k == l) ? 1.0 : 0.0;
+            if (k < nsa && l < nsa) {
+                const double ag = fabs(S.gam[l]);
+                v = (ag > 1.e-14) ? ((S.gam[l] / ag) * (latent(M, S.isa[k], S.isa[l]) * th[l])) / S.s1[k] : 0.0;
+            }
+            S.ah[k][l] = v;
+        }
     }
+    flops += 3 * (int64_t)N * N + 6 * N + 3 * (int64_t)nsa * nsa;
+    transc += 4 * N;
+}
+
+// SCALE_IT on the active set; amax starts at am of system 1 (Cpstrs.f90:200).
+template <int N, int AMAX>
+__device__ __forceinline__ void scale_it_c(const DevMaterial<N>& M, Compact<AMAX>& S, double s[6], double gamcap,
+                                           double afactor, int64_t& flops, int64_t& transc) {
+    double amax = M.am[0];
+    double gammax = fabs(S.gam[0]);
 #pragma unroll
-    for (int j = 0; j < 6; ++j)
+    for (int k = 1; k < AMAX; ++k) {
+        if (k < S.nsa && fabs(S.gam[k]) > gammax) {
+            gammax = fabs(S.gam[k]);
+            amax = rate_m(M, S.isa[k]);
+        }
+    }
+    if (gammax > gamcap) {
+        const double scale = gamcap / gammax;
+        double smult = spower(0.9 * scale, afactor * amax);
+        if (smult == 0.0) smult = 0.004 / amax;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) A.chi[(i + 6 * j) * n + g] = io.chi[i][j];
-    A.deff[g] = io.deff;
-    A.erepmd[g] = io.erepmd;
+        for (int i = 0; i < 6; ++i) s[i] = smult * s[i];
 #pragma unroll
-    for (int i = 0; i < 3 + N; ++i) A.dsv[i * n + g] = dsv[i];
+        for (int k = 0; k < AMAX; ++k) {
+            S.gam[k] = scale * S.gam[k];
+            S.tau[k] = smult * S.tau[k];
+        }
+        flops += S.nsa + 2 * N + 6;
+        transc += 2;
+    } else {
+        flops += S.nsa;
+    }
+}
+
+// SLPFTA on the active set; leaves ah in X form and gpre = slips of the last PLASTC.
+template <int N, int AMAX>
+__device__ __forceinline__ void slpfta_c(const DevMaterial<N>& M, const DevControl& C, Compact<AMAX>& S, double dt,
+                                         double* gpre, int& err, int64_t& flops, int64_t& transc) {
+    const int nsa = S.nsa;
+    double th[AMAX];
+    int it = 0;
+    bool more = true;
+    double tauerr = 0.0;
+    while (more) {
+        ++it;
+        plastc_c<N, AMAX>(M, S, dt, th, flops, transc);
 #pragma unroll
-    for (int i = 0; i < N; ++i) A.srate[i * n + g] = srate[i];
-    A.twinsys[g] = io.twinsys;
-    if (A.flops) A.flops[g] = cnt.flops;
-    if (A.transc) A.transc[g] = cnt.transc;
-    if (A.err) A.err[g] = io.err;
-    if (io.err) atomicMax(A.errmax, io.err);
-    if (io.tflag) atomicOr(A.tflag, 1);
+        for (int i = 0; i < AMAX; ++i) {
+            if (i < nsa) {
+                const double gamn = S.gam[i];
+#pragma unroll
+                for (int j = 0; j < AMAX; ++j)
+                    if (j < nsa) S.ah[j][i] = gamn * S.ah[j][i];
+                S.ah[i][i] = S.ah[i][i] + rate_m(M, S.isa[i]);
+            }
+        }
+        if (factor<AMAX>(S.ah)) err = err > 910 ? err : 910;
+        double e[AMAX];
+#pragma unroll
+        for (int k = 0; k < AMAX; ++k) {
+            double v = 0.0;
+            if (k < nsa) {
+                const double taufg = S.s1[k] * spower((S.gam[k] / dt), rate_m(M, S.isa[k]));
+                if ((S.tau[k] * taufg) > 0.0) v = log(S.tau[k] / taufg);
+            }
+            e[k] = v;
+        }
+        tauerr = 0.0;
+#pragma unroll
+        for (int k = 0; k < AMAX; ++k)
+            if (k < nsa) tauerr = tauerr + e[k] * e[k];
+        tauerr = sqrt(tauerr);
+        solver<AMAX>(S.ah, e);
+        double errmax = 0.0;
+#pragma unroll
+        for (int k = 0; k < AMAX; ++k)
+            if (k < nsa && fabs(e[k]) > errmax) errmax = fabs(e[k]);
+        const double scale = (errmax > 5.0) ? 5.0 / errmax : 1.0;
+#pragma unroll
+        for (int k = 0; k < AMAX; ++k)
+            if (k < nsa) S.gam[k] = S.gam[k] * exp(scale * e[k]);
+        more = (it < C.maxit) && (tauerr > M.conv);
+        flops += (int64_t)nsa * nsa + 10 * nsa + 1 + factor_flops(nsa) + solver_flops(nsa);
+        transc += 4 * nsa;
+    }
+    if (tauerr > 0.5) err = err > 987 ? err : 987;
+    plastc_c<N, AMAX>(M, S, dt, th, flops, transc);
+#pragma unroll
+    for (int k = 0; k < AMAX; ++k) gpre[k * 64] = S.gam[k];
+#pragma unroll
+    for (int i = 0; i < AMAX; ++i) {
+        if (i < nsa) {
+#pragma unroll
+            for (int j = 0; j < AMAX; ++j)
+                if (j < nsa) S.ah[j][i] = S.tau[j] * S.ah[j][i];
+            const double ag = fabs(S.gam[i]);
+            const double am = rate_m(M, S.isa[i]);
+            S.ah[i][i] = S.ah[i][i] + ((ag > 1.e-12) ? am * S.tau[i] / S.gam[i] : am / 1.e-12);
+        }
+    }
+    flops += (int64_t)nsa * nsa + 3 * nsa;
+}
+
+// The per-point kernel.  One lane per point, 64 lanes per block.
+template <int N, int AMAX>
+__global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__ Mp, const DevControl C,
+                                              const double dt, const ConstArgs A) {
+    __shared__ double sh_el[21][64];
+    __shared__ double sh_b[6 * AMAX][64];
+    __shared__ double sh_t[6][64];
+    __shared__ double sh_gp[AMAX][64];
+    __shared__ int sh_isa[AMAX][64];
+    const int lane = threadIdx.x;
+    const int idx = blockIdx.x * 64 + lane;
+    const bool first = (A.in_list == nullptr);
+    bool live;
+    int g = 0;
+    if (first) {
+        live = idx < A.n;
+        g = idx;
+    } else {
+        live = idx < *A.in_count;
+        if (live) g = A.in_list[idx];
+    }
+    bool want_cont = false, want_ovf = false;
+    if (live) {
+        const DevMaterial<N>& M = *Mp;
+        const size_t n = (size_t)A.n;
+        double* el = &sh_el[0][lane];
+        double* bc = &sh_b[0][lane];
+        double* tl = &sh_t[0][lane];
+        double* gpre = &sh_gp[0][lane];
+        double de[6], r[3], st[6], t[6], s[6];
+        int kerr = 0, iel;
+        double deff_kin = 0.0;
+        if (A.mesh) {
+            const int e = A.gp_elem[g];
+            iel = A.elem_le[e];
+            if (first) {
+                double xl[8][3], dxl[8][3], sfd[8][3];
+#pragma unroll
+                for (int a = 0; a < 8; ++a) {
+                    const int nd = A.elem_nodes[8 * e + a];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        xl[a][k] = A.x[3 * nd + k];
+                        dxl[a][k] = A.dx[3 * nd + k];
+                    }
+                }
+                const double w = brick_shape(A.elem_lt[e], A.gp_local[g], xl, sfd);
+                if (w < 1.e-7) kerr = 920;
+                const int e2 = dstrns(sfd, dxl, A.smale, A.smalr, de, r, deff_kin);
+                if (e2 > kerr) kerr = e2;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) A.de[i * n + g] = de[i];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) A.r[i * n + g] = r[i];
+            }
+        } else {
+            iel = A.iel[g];
+        }
+        if (!first || !A.mesh) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) de[i] = A.de[i * n + g];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) r[i] = A.r[i * n + g];
+        }
+        if (!first) kerr = A.ck_kerr[g];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) st[i] = A.st[i * n + g];
+        double ph[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) ph[i] = A.sv[i * n + g] + A.dsv[i * n + g];
+        elastc<N>(M, ph[0], ph[1], ph[2], el, 64);
+        tens_rot(r, st, t);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) tl[i * 64] = t[i];
+        int64_t flops = 0, transc = 0;
+        int iter = 0;
+        bool elastic = false;
+        if (first) {
+            flops = 1575 + 12 + 120 + 28;
+            transc = 6;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) s[i] = t[i] + A.ds[i * n + g];
+            const double sinit = fj2vom(true, s);
+            const double einit = fj2vom(false, de);
+            elastic = !(iel > 0 && sinit > 0.0 && einit > 0.0);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) s[i] = A.ck_s[i * n + g];
+            iter = A.ck_iter[g];
+            flops = A.flops[g];
+            transc = A.transc[g];
+        }
+        if (elastic) {
+            // Sig_eps.f90:130-145: chi = S^-1, t = chi de (the start stress is not used)
+            double lu[6][6], chi[6][6];
+#pragma unroll
+            for (int i = 0; i < 6; ++i)
+#pragma unroll
+                for (int j = 0; j < 6; ++j) lu[i][j] = el[sym21(i, j) * 64];
+            (void)factor<6>(lu);
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                double c[6];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) c[k] = (k == i) ? 1.0 : 0.0;
+                solver<6>(lu, c);
+#pragma unroll
+                for (int j = 0; j < 6; ++j) chi[j][i] = c[j];
+            }
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                double sum = 0.0;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) sum = sum + chi[i][k] * de[k];
+                A.t[i * n + g] = sum;
+                A.ds[i * n + g] = sum - st[i];
+            }
+#pragma unroll
+            for (int j = 0; j < 6; ++j)
+#pragma unroll
+                for (int i = 0; i < 6; ++i) A.chi[(i + 6 * j) * n + g] = chi[i][j];
+            A.dep[g] = 0.0;
+            A.erepmd[g] = 0.0;
+            flops += factor_flops(6) + 6 * solver_flops(6) + 72 + 12;
+            A.flops[g] = flops;
+            A.transc[g] = transc;
+            A.err[g] = kerr;
+            if (kerr) atomicMax(A.errmax, kerr);
+        } else {
+            // ---------------- CPSTRS (Cpstrs.f90:34-172) ----------------
+            double a[3][3], aug[3];
+            ctmatrx(ph[0], ph[1], ph[2], a, aug);
+            const double gamcap = 6.0 * fj2vom(false, de);
+            if (first) {
+                flops += 14 + 1 + 12 + 60 * (int64_t)N;
+                transc += 6;
+            }
+            int twinsys = A.twinsys[g];
+            const int timestwinned = A.timestwinned[g];
+            const uint64_t rng_key = (A.rng_epoch << 32) ^ (uint64_t)g;
+            int err = kerr, tflag = 0;
+            Compact<AMAX> S;
+            double djac[6][6], epl[6];
+            double erepmd = 0.0;
+            bool done = false, ovf = false;
+            while (!done && iter < A.iter_limit) {
+                // ---- SLPINI (crys-cons.f90:143-197) over all systems
+                double gfull[N], tfull[N];
+                double gammax = 0.0;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    double sdr[3], spn[3], b[6];
+                    slip_vectors(M, a, i, sdr, spn);
+                    schmid(sdr, spn, b);
+                    double sum = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) sum = sum + b[k] * s[k];
+                    tfull[i] = sum;
+                    const double s0i = A.sv[(3 + i) * n + g];
+                    double gi;
+                    if (M.any_twin && fabs(M.ssang[i]) > 1.5) {
+                        gi = 0.0;
+                        if ((sum > s0i) && (timestwinned != M.twinlim) && (twinsys == 0)) {
+                            if (twin_draw(rng_key) < M.twinprob) {
+                                tflag = 1;
+                                twinsys = i + 1;
+                            }
+                        }
+                    } else if (M.any_unidir && M.ssang[i] * sum / s0i < 0.0) {
+                        gi = 0.0;
+                    } else {
+                        gi = dt * spower((sum / s0i), (1.0 / M.am[i]));
+                    }
+                    gfull[i] = gi;
+                    if (fabs(gi) > gammax) gammax = fabs(gi);
+                }
+                const double cutoff = 1.e-6 * gammax;
+                int nsa = 0;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    if (!(fabs(gfull[i]) < cutoff)) {
+                        if (nsa < AMAX) {
+                            // stage in the b slice, read back at fixed slots below
+                            bc[(4 * nsa + 0) * 64] = gfull[i];
+                            bc[(4 * nsa + 1) * 64] = tfull[i];
+                            sh_isa[nsa][lane] = i;
+                        }
+                        ++nsa;
+                    }
+                }
+                if (nsa > AMAX) {
+                    ovf = true;
+                    break;
+                }
+                ++iter;
+                S.nsa = nsa;
+#pragma unroll
+                for (int k = 0; k < AMAX; ++k) {
+                    if (k < nsa) {
+                        const int i = sh_isa[k][lane];
+                        S.isa[k] = i;
+                        S.gam[k] = bc[(4 * k + 0) * 64];
+                        S.tau[k] = bc[(4 * k + 1) * 64];
+                        S.s0[k] = A.sv[(3 + i) * n + g];
+                    } else {
+                        S.isa[k] = 0;
+                        S.gam[k] = 0.0;
+                        S.tau[k] = 0.0;
+                        S.s0[k] = 1.0;
+                    }
+                }
+                flops += 5 * (int64_t)N;
+                transc += 2 * (int64_t)N;
+                // Schmid columns of the active systems (BTMATRX, crys_math.f90:75-115)
+#pragma unroll
+                for (int k = 0; k < AMAX; ++k) {
+                    if (k < nsa) {
+                        double sdr[3], spn[3], b[6];
+                        slip_vectors(M, a, S.isa[k], sdr, spn);
+                        schmid(sdr, spn, b);
+#pragma unroll
+                        for (int q = 0; q < 6; ++q) bc[(6 * k + q) * 64] = b[q];
+                    }
+                }
+                scale_it_c<N, AMAX>(M, S, s, gamcap, C.afactor, flops, transc);
+                slpfta_c<N, AMAX>(M, C, S, dt, gpre, err, flops, transc);
+                scale_it_c<N, AMAX>(M, S, s, gamcap, C.afactor, flops, transc);
+                // plastic strain increment and strain-increment error (Cpstrs.f90:96-98)
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    double sum = 0.0;
+#pragma unroll
+                    for (int k = 0; k < AMAX; ++k)
+                        if (k < nsa) sum = sum + bc[(6 * k + i) * 64] * S.gam[k];
+                    epl[i] = sum;
+                }
+                double erreps[6];
+                {
+                    double sm[6];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) sm[k] = s[k] - tl[k * 64];
+#pragma unroll
+                    for (int i = 0; i < 6; ++i) {
+                        double sum = 0.0;
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) sum = sum + el[sym21(i, k) * 64] * sm[k];
+                        erreps[i] = A.de[i * n + g] - epl[i] - sum;
+                    }
+                }
+                if (factor<AMAX>(S.ah)) err = err > 910 ? err : 910;
+                // djac = B_a X^-1 B_a^T + S: one solve per column (Cpstrs.f90:101-119)
+#pragma unroll
+                for (int j = 0; j < 6; ++j) {
+                    double w[AMAX];
+#pragma unroll
+                    for (int k = 0; k < AMAX; ++k) w[k] = (k < nsa) ? bc[(6 * k + j) * 64] : 0.0;
+                    solver<AMAX>(S.ah, w);
+#pragma unroll
+                    for (int i = 0; i < 6; ++i) {
+                        double sum = 0.0;
+#pragma unroll
+                        for (int k = 0; k < AMAX; ++k)
+                            if (k < nsa) sum = sum + bc[(6 * k + i) * 64] * w[k];
+                        djac[i][j] = sum + el[sym21(i, j) * 64];
+                    }
+                }
+                if (factor<6>(djac)) err = err > 910 ? err : 910;
+                solver<6>(djac, erreps);
+                erepmd = fj2vom(true, erreps) / fj2vom(true, s);
+                const double scale = 1.0 / (1.0 + C.alpha * erepmd);
+#pragma unroll
+                for (int i = 0; i < 6; ++i) erreps[i] = scale * erreps[i];
+#pragma unroll
+                for (int i = 0; i < 6; ++i) s[i] = s[i] + erreps[i];
+                done = !((iter < C.maxit) && (erepmd > C.toler));
+                flops += 24 * (int64_t)N + 90 + 72 * (int64_t)nsa + 36 + 44 + factor_flops(nsa) +
+                         6 * solver_flops(nsa) + factor_flops(6) + solver_flops(6) + 28;
+            }
+            if (tflag) atomicOr(A.tflag, 1);
+            if (done) {
+                // ---------------- finish CPSTRS and DSTRSS ----------------
+                const int nsa = S.nsa;
+#pragma unroll
+                for (int i = 0; i < N; ++i) A.srate[i * n + g] = 0.0;
+#pragma unroll
+                for (int k = 0; k < AMAX; ++k)
+                    if (k < nsa) A.srate[S.isa[k] * n + g] = fabs(S.gam[k] / dt);
+                // ECHROT (crys_math.f90:137-167)
+                double rtn[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+                for (int k = 0; k < AMAX; ++k) {
+                    if (k < nsa) {
+                        double sdr[3], spn[3];
+                        slip_vectors(M, a, S.isa[k], sdr, spn);
+                        rtn[0] = rtn[0] + ((sdr[1] * spn[2] - sdr[2] * spn[1]) / 2.0) * S.gam[k];
+                        rtn[1] = rtn[1] + ((sdr[2] * spn[0] - sdr[0] * spn[2]) / 2.0) * S.gam[k];
+                        rtn[2] = rtn[2] + ((sdr[0] * spn[1] - sdr[1] * spn[0]) / 2.0) * S.gam[k];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 3; ++q) rtn[q] = rtn[q] - A.r[q * n + g];
+                double rx[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    double sum = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) sum = sum + a[i][k] * rtn[k];
+                    rx[i] = sum;
+                }
+                const double dp1 = (fabs(aug[2]) < 1.e-5) ? rtn[2] : (aug[0] * rx[0] + aug[1] * rx[1]) / aug[2];
+                const double dph = aug[1] * rx[0] - aug[0] * rx[1];
+                const double dp2 = rx[2] - dp1 * a[2][2];
+                A.dsv[0 * n + g] = dp1;
+                A.dsv[1 * n + g] = dph;
+                A.dsv[2 * n + g] = dp2;
+                // dsv(3+i) = s1(i) - s0(i), s1 of the last PLASTC over every system
+                {
+                    double th[AMAX];
+                    double sumss = 0.0;
+#pragma unroll
+                    for (int k = 0; k < AMAX; ++k)
+                        if (k < nsa) sumss = sumss + fabs(gpre[k * 64]);
+                    const double rate = sumss / dt;
+#pragma unroll
+                    for (int k = 0; k < AMAX; ++k) {
+                        th[k] = 0.0;
+                        if (k < nsa) {
+                            const int j = M.uniform_par ? 0 : S.isa[k];
+                            const double sst = M.taus[j] * pow_fast(rate, M.nsat[j]);
+                            double arg = (sst != 0.0) ? 1.0 - (S.s0[k] / sst) : 0.0;
+                            if (arg < 0.0) arg = 0.0;
+                            th[k] = M.th4[j] + M.th0[j] * copysign(pow_fast(fabs(arg), M.alp[j]), arg);
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < N; ++i) {
+                        const double s0i = A.sv[(3 + i) * n + g];
+                        double s1 = s0i;
+#pragma unroll
+                        for (int l = 0; l < AMAX; ++l)
+                            if (l < nsa) s1 = s1 + (latent(M, i, S.isa[l]) * th[l]) * fabs(gpre[l * 64]);
+                        A.dsv[(3 + i) * n + g] = s1 - s0i;
+                    }
+                }
+                // chi = djac^-1 (Cpstrs.f90:154-161), symmetrised (Sig_eps.f90:122-128)
+                double chi[6][6];
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    double c[6];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) c[k] = (k == i) ? 1.0 : 0.0;
+                    solver<6>(djac, c);
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) chi[j][i] = c[j];
+                }
+#pragma unroll
+                for (int i = 0; i < 6; ++i)
+#pragma unroll
+                    for (int j = i; j < 6; ++j) {
+                        const double d = (chi[i][j] + chi[j][i]) / 2.0;
+                        chi[i][j] = d;
+                        chi[j][i] = d;
+                    }
+#pragma unroll
+                for (int j = 0; j < 6; ++j)
+#pragma unroll
+                    for (int i = 0; i < 6; ++i) A.chi[(i + 6 * j) * n + g] = chi[i][j];
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+                    A.t[i * n + g] = s[i];
+                    A.ds[i * n + g] = s[i] - A.st[i * n + g];
+                }
+                A.dep[g] = fj2vom(false, epl);
+                A.erepmd[g] = erepmd;
+                flops += 6 * (int64_t)N + 25 + 6 * solver_flops(6) + 82;
+                A.flops[g] = flops;
+                A.transc[g] = transc;
+                A.err[g] = err;
+                A.twinsys[g] = twinsys;
+                if (err) atomicMax(A.errmax, err);
+            } else {
+                // checkpoint: stress at the next outer iteration and the work so far
+#pragma unroll
+                for (int i = 0; i < 6; ++i) A.ck_s[i * n + g] = s[i];
+                A.ck_iter[g] = iter;
+                A.ck_kerr[g] = err;
+                A.flops[g] = flops;
+                A.transc[g] = transc;
+                A.twinsys[g] = twinsys;
+                want_ovf = ovf;
+                want_cont = !ovf;
+            }
+        }
+    }
+    list_append(want_cont, g, A.cont_list, A.cont_count);
+    list_append(want_ovf, g, A.ovf_list, A.ovf_count);
 }
 
 template <int N>
@@ -79,8 +586,14 @@ void fill_material(DevMaterial<N>& d, const cpfem_material& m, double toler) {
     d.twinprob = m.twinprob;
     for (int i = 0; i < 9; ++i) d.e[i] = m.e[i];
     double amx = m.ampar[0][0];
+    d.uniform_par = 1;
+    d.two_valued = 1;
+    d.hd = m.smatrx[0][0];
+    d.hq = N > 1 ? m.smatrx[0][1] : 0.0;
     for (int i = 0; i < N; ++i) {
         d.ssang[i] = m.ssang[i];
+        if (fabs(m.ssang[i]) > 1.5) d.any_twin = 1;
+        if (m.ssang[i] != 0.0) d.any_unidir = 1;
         for (int k = 0; k < 3; ++k) {
             d.ab[i][k] = m.ab[i][k];
             d.an[i][k] = m.an[i][k];
@@ -91,14 +604,17 @@ void fill_material(DevMaterial<N>& d, const cpfem_material& m, double toler) {
         d.alp[i] = m.ampar[i][3];
         d.taus[i] = m.ampar[i][4];
         d.nsat[i] = m.ampar[i][5];
-        for (int k = 0; k < N; ++k) d.smatrx[i][k] = m.smatrx[k][i];  // Fortran smatrx(i,k)
+        for (int q = 0; q < 6; ++q)
+            if (m.ampar[i][q] != m.ampar[0][q]) d.uniform_par = 0;
+        for (int k = 0; k < N; ++k) {
+            d.smatrx[i][k] = m.smatrx[k][i];  // Fortran smatrx(i,k)
+            const double want = (i == k) ? d.hd : d.hq;
+            if (d.smatrx[i][k] != want) d.two_valued = 0;
+        }
         if (m.ampar[i][0] > amx) amx = m.ampar[i][0];
     }
     d.conv = toler * amx;
 }
-
-template void fill_material<12>(DevMaterial<12>&, const cpfem_material&, double);
-template void fill_material<18>(DevMaterial<18>&, const cpfem_material&, double);
 
 DevControl make_control(const cpfem_control& c) {
     DevControl d;
@@ -108,26 +624,6 @@ DevControl make_control(const cpfem_control& c) {
     d.maxit = c.maxit;
     d.pad_ = 0;
     return d;
-}
-
-template <int N>
-hipError_t launch_dstrss_points(const void* dmat, const DevControl& C, double dt, const PointArrays& A,
-                                hipStream_t stream) {
-    if (A.n <= 0) return hipSuccess;
-    const int block = 64;
-    const int grid = (A.n + block - 1) / block;
-    hipLaunchKernelGGL(k_dstrss_points<N>, dim3(grid), dim3(block), 0, stream,
-                       reinterpret_cast<const DevMaterial<N>*>(dmat), C, dt, A);
-    return hipGetLastError();
-}
-
-hipError_t launch_dstrss(int nss, const void* dmat, const DevControl& C, double dt, const PointArrays& A,
-                         hipStream_t stream) {
-    switch (nss) {
-        case 12: return launch_dstrss_points<12>(dmat, C, dt, A, stream);
-        case 18: return launch_dstrss_points<18>(dmat, C, dt, A, stream);
-        default: return hipErrorInvalidValue;
-    }
 }
 
 size_t material_bytes(int nss) {
@@ -141,6 +637,136 @@ size_t material_bytes(int nss) {
 void build_material(int nss, void* host, const cpfem_material& m, double toler) {
     if (nss == 12) fill_material<12>(*reinterpret_cast<DevMaterial<12>*>(host), m, toler);
     if (nss == 18) fill_material<18>(*reinterpret_cast<DevMaterial<18>*>(host), m, toler);
+}
+
+// Width of the first kernel (active systems held in registers) and the
+// outer iterations every point gets before the unfinished ones are compacted.
+#ifndef CPFEM_AMAX
+#define CPFEM_AMAX 6
+#endif
+constexpr int kAmax = CPFEM_AMAX;
+constexpr int kAmaxMid = 8;
+constexpr int kAmaxWide = 12;
+constexpr int kFirstPassIters = 8;
+
+__global__ void k_zero4(int* p) {
+    if (threadIdx.x < 4) p[threadIdx.x] = 0;
+}
+
+__global__ void k_set1(int* p, int v) {
+    if (threadIdx.x == 0) *p = v;
+}
+
+template <int N, int AMAX>
+static void launch_one(const void* dmat, const DevControl& C, double dt, const ConstArgs& A, int nthreads,
+                       hipStream_t stream) {
+    if (nthreads <= 0) return;
+    hipLaunchKernelGGL((k_const<N, AMAX>), dim3((nthreads + 63) / 64), dim3(64), 0, stream,
+                       reinterpret_cast<const DevMaterial<N>*>(dmat), C, dt, A);
+}
+
+template <int N>
+static hipError_t read_counts(ConstSchedule& S, hipStream_t stream) {
+    hipError_t e = hipMemcpyAsync(S.host_counts, S.counts, 4 * sizeof(int), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    return e;
+}
+
+// Tiers of active-set width: kAmax for every point, then kAmaxMid and
+// kAmaxWide for the points whose active set outgrew the previous tier.
+template <int N>
+static hipError_t schedule(const void* dmat, const DevControl& C, double dt, ConstArgs A, ConstSchedule& S,
+                           hipStream_t stream, int counts[4]) {
+    // device counters: [0] list A, [1] list B, [2] overflow list, [3] overflow of the overflow
+    hipLaunchKernelGGL(k_zero4, dim3(1), dim3(64), 0, stream, S.counts);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    A.ck_s = S.ck_s;
+    A.ck_iter = S.ck_iter;
+    A.ck_kerr = S.ck_kerr;
+    // tier 1, pass 1: every point, at most kFirstPassIters outer iterations
+    A.in_list = nullptr;
+    A.in_count = nullptr;
+    A.cont_list = S.list_a;
+    A.cont_count = S.counts + 0;
+    A.ovf_list = S.list_ovf;
+    A.ovf_count = S.counts + 2;
+    A.iter_limit = kFirstPassIters < C.maxit ? kFirstPassIters : C.maxit;
+    launch_one<N, kAmax>(dmat, C, dt, A, A.n, stream);
+    if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
+    counts[0] = S.host_counts[0];
+    // tier 1, pass 2: the unfinished points, compacted, up to maxit
+    if (S.host_counts[0] > 0) {
+        A.in_list = S.list_a;
+        A.in_count = S.counts + 0;
+        A.cont_list = S.list_b;
+        A.cont_count = S.counts + 1;
+        A.iter_limit = C.maxit;
+        launch_one<N, kAmax>(dmat, C, dt, A, S.host_counts[0], stream);
+        if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
+    }
+    counts[1] = S.host_counts[2];
+    // tier 2: overflow of tier 1 (list_ovf) -> overflow into list_a
+    if (S.host_counts[2] > 0) {
+        hipLaunchKernelGGL(k_zero4, dim3(1), dim3(64), 0, stream, S.counts);  // list_a reused below
+        // the overflow count was read on the host; keep it on the device in slot 3
+        hipLaunchKernelGGL(k_set1, dim3(1), dim3(64), 0, stream, S.counts + 3, S.host_counts[2]);
+        A.in_list = S.list_ovf;
+        A.in_count = S.counts + 3;
+        A.cont_list = S.list_b;
+        A.cont_count = S.counts + 1;
+        A.ovf_list = S.list_a;
+        A.ovf_count = S.counts + 0;
+        A.iter_limit = C.maxit;
+        launch_one<N, kAmaxMid>(dmat, C, dt, A, S.host_counts[2], stream);
+        if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
+        counts[2] = S.host_counts[0];
+        // tier 3: overflow of tier 2 (list_a)
+        if (S.host_counts[0] > 0) {
+            A.in_list = S.list_a;
+            A.in_count = S.counts + 0;
+            A.cont_list = S.list_b;
+            A.cont_count = S.counts + 1;
+            A.ovf_list = S.list_ovf;
+            A.ovf_count = S.counts + 2;
+            hipLaunchKernelGGL(k_set1, dim3(1), dim3(64), 0, stream, S.counts + 2, 0);
+            launch_one<N, kAmaxWide>(dmat, C, dt, A, S.host_counts[0], stream);
+            if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
+            // more than kAmaxWide active systems at one point: not compiled in
+            if (S.host_counts[2] > 0) return hipErrorNotSupported;
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t run_constitutive(int nss, const void* dmat, const DevControl& C, double dt, const ConstArgs& A,
+                            ConstSchedule& S, hipStream_t stream, int counts[4]) {
+    switch (nss) {
+        case 12: return schedule<12>(dmat, C, dt, A, S, stream, counts);
+        case 18: return schedule<18>(dmat, C, dt, A, S, stream, counts);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t alloc_schedule(ConstSchedule& S, int n) {
+    hipError_t e;
+    if ((e = hipMalloc((void**)&S.ck_s, sizeof(double) * 6 * (size_t)n)) != hipSuccess) return e;
+    if ((e = hipMalloc((void**)&S.ck_iter, sizeof(int) * (size_t)n)) != hipSuccess) return e;
+    if ((e = hipMalloc((void**)&S.ck_kerr, sizeof(int) * (size_t)n)) != hipSuccess) return e;
+    if ((e = hipMalloc((void**)&S.list_a, sizeof(int) * (size_t)n)) != hipSuccess) return e;
+    if ((e = hipMalloc((void**)&S.list_b, sizeof(int) * (size_t)n)) != hipSuccess) return e;
+    if ((e = hipMalloc((void**)&S.list_ovf, sizeof(int) * (size_t)n)) != hipSuccess) return e;
+    if ((e = hipMalloc((void**)&S.counts, sizeof(int) * 4)) != hipSuccess) return e;
+    if ((e = hipHostMalloc((void**)&S.host_counts, sizeof(int) * 4, 0)) != hipSuccess) return e;
+    return hipSuccess;
+}
+
+void free_schedule(ConstSchedule& S) {
+    void* p[] = {S.ck_s, S.ck_iter, S.ck_kerr, S.list_a, S.list_b, S.list_ovf, S.counts};
+    for (void* q : p)
+        if (q) (void)hipFree(q);
+    if (S.host_counts) (void)hipHostFree(S.host_counts);
+    memset(&S, 0, sizeof(S));
 }
 
 }  // namespace cpfem
@@ -206,23 +832,23 @@ extern "C" int cpfem_dstrss_batch(const cpfem_material* mat, const cpfem_control
     }
     hi[4 * N] = *tflag;
     CPFEM_HIP(hipMemcpy(di.p, hi.data(), hi.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    PointArrays A;
+    ConstArgs A;
     memset(&A, 0, sizeof(A));
-    const ptrdiff_t off = 0;
     A.n = n;
+    A.mesh = 0;
     A.iel = di.p;
     A.twinsys = di.p + N;
     A.timestwinned = di.p + 2 * N;
     A.err = di.p + 3 * N;
     A.tflag = di.p + 4 * N;
     A.errmax = di.p + 4 * N + 1;
-    A.de = dd.p + (hde - h.data()) + off;
+    A.de = dd.p + (hde - h.data());
     A.r = dd.p + (hr - h.data());
     A.st = dd.p + (hst - h.data());
     A.ds = dd.p + (hds - h.data());
     A.t = dd.p + (ht - h.data());
     A.chi = dd.p + (hchi - h.data());
-    A.deff = dd.p + (hdeff - h.data());
+    A.dep = dd.p + (hdeff - h.data());
     A.erepmd = dd.p + (hrep - h.data());
     A.sv = dd.p + (hsv - h.data());
     A.dsv = dd.p + (hdsv - h.data());
@@ -230,10 +856,19 @@ extern "C" int cpfem_dstrss_batch(const cpfem_material* mat, const cpfem_control
     A.flops = reinterpret_cast<long long*>(dl.p);
     A.transc = reinterpret_cast<long long*>(dl.p + N);
     A.rng_epoch = 0;
-    A.gp_offset = 0;
+    ConstSchedule S;
+    memset(&S, 0, sizeof(S));
+    hipError_t he = alloc_schedule(S, n);
+    if (he != hipSuccess) {
+        free_schedule(S);
+        return CPFEM_E_HIP;
+    }
     DevControl C = make_control(*ctl);
-    CPFEM_HIP(launch_dstrss(nss, dm.p, C, dt, A, 0));
-    CPFEM_HIP(hipDeviceSynchronize());
+    int counts[4];
+    he = run_constitutive(nss, dm.p, C, dt, A, S, 0, counts);
+    if (he == hipSuccess) he = hipDeviceSynchronize();
+    free_schedule(S);
+    if (he != hipSuccess) return CPFEM_E_HIP;
     CPFEM_HIP(hipMemcpy(h.data(), dd.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
     CPFEM_HIP(hipMemcpy(hi.data(), di.p, hi.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
     std::vector<int64_t> hl(2 * N);

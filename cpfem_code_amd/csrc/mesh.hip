@@ -19,6 +19,14 @@ struct cpfem_ctx : cpfem::Ctx {};
 namespace cpfem {
 
 constexpr int kBlock = 256;
+constexpr int kXcd = 8;   // MI355X: 8 XCDs, blocks dispatched to them round-robin
+
+// Logical block of a launch of nb_pad = kXcd * per_xcd blocks: consecutive
+// logical blocks (neighbouring rows, i.e. neighbouring mesh nodes) stay on one
+// XCD, so each XCD's L2 holds the vector slice its rows gather from.
+__device__ __forceinline__ int xcd_block(int per_xcd) {
+    return (int)(blockIdx.x % kXcd) * per_xcd + (int)(blockIdx.x / kXcd);
+}
 
 // ---------------------------------------------------------------------------
 // deterministic reductions: per-block partials, then one block in fixed order
@@ -77,110 +85,6 @@ __global__ void k_fill(T* __restrict__ p, size_t n, T v) {
 __global__ void k_axpy_nodes(double* __restrict__ y, const double* __restrict__ x, double a, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = y[i] + a * x[i];
-}
-
-// ---------------------------------------------------------------------------
-// point kernel: kinematics (SHAPE + DSTRNS) and DSTRSS for every point
-// ---------------------------------------------------------------------------
-struct PointMeshArgs {
-    int ngps;
-    const int* gp_elem;
-    const int* gp_local;
-    const int* elem_nodes;
-    const int* elem_lt;
-    const int* elem_le;
-    const double* x;
-    const double* dx;
-    double smale, smalr;
-    double* de;      // [6][n]
-    double* dr;      // [3][n]
-    double* dep;     // [n]
-    PointArrays P;
-};
-
-template <int N>
-__global__ __launch_bounds__(64) void k_points_mesh(const DevMaterial<N>* __restrict__ Mp, const DevControl C,
-                                                    const double dt, const PointMeshArgs A) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= A.ngps) return;
-    const DevMaterial<N>& M = *Mp;
-    const size_t n = (size_t)A.ngps;
-    const int e = A.gp_elem[g], ig = A.gp_local[g];
-    const int lt = A.elem_lt[e];
-    PointIO io;
-    io.err = 0;
-    io.tflag = 0;
-    io.deff = 0.0;
-    {
-        double xl[8][3], dxl[8][3], sfd[8][3];
-#pragma unroll
-        for (int a = 0; a < 8; ++a) {
-            const int nd = A.elem_nodes[8 * e + a];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                xl[a][k] = A.x[3 * nd + k];
-                dxl[a][k] = A.dx[3 * nd + k];
-            }
-        }
-        const double w = brick_shape(lt, ig, xl, sfd);
-        if (w < 1.e-7) io.err = 920;
-        double deff;
-        const int e2 = dstrns(sfd, dxl, A.smale, A.smalr, io.de, io.r, deff);
-        if (e2 > io.err) io.err = e2;
-        io.deff = deff;
-    }
-    io.iel = A.elem_le[e];
-    const PointArrays& P = A.P;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        io.st[i] = P.st[i * n + g];
-        io.ds[i] = P.ds[i * n + g];
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        io.sv0[i] = P.sv[i * n + g];
-        io.dsv0[i] = P.dsv[i * n + g];
-    }
-    io.twinsys = P.twinsys[g];
-    io.timestwinned = P.timestwinned[g];
-    io.rng_key = (P.rng_epoch << 32) ^ (uint64_t)g;
-    double s0[N], dsv[3 + N], srate[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        s0[i] = P.sv[(3 + i) * n + g];
-        srate[i] = P.srate[i * n + g];
-    }
-#pragma unroll
-    for (int i = 0; i < 3 + N; ++i) dsv[i] = P.dsv[i * n + g];
-    Count cnt{0, 0, 0, 0};
-    const int kin_err = io.err;
-    io.err = 0;
-    dstrss<N>(M, C, dt, io, s0, dsv, srate, cnt);
-    if (kin_err > io.err) io.err = kin_err;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        A.de[i * n + g] = io.de[i];
-        P.t[i * n + g] = io.t[i];
-        P.ds[i * n + g] = io.ds[i];
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) A.dr[i * n + g] = io.r[i];
-    A.dep[g] = io.deff;
-#pragma unroll
-    for (int j = 0; j < 6; ++j)
-#pragma unroll
-        for (int i = 0; i < 6; ++i) P.chi[(i + 6 * j) * n + g] = io.chi[i][j];
-    P.erepmd[g] = io.erepmd;
-#pragma unroll
-    for (int i = 0; i < 3 + N; ++i) P.dsv[i * n + g] = dsv[i];
-#pragma unroll
-    for (int i = 0; i < N; ++i) P.srate[i * n + g] = srate[i];
-    P.twinsys[g] = io.twinsys;
-    P.flops[g] = cnt.flops;
-    P.transc[g] = cnt.transc;
-    P.err[g] = io.err;
-    if (io.err) atomicMax(P.errmax, io.err);
-    if (io.tflag) atomicOr(P.tflag, 1);
 }
 
 // Sums of erepmd, flops and transcendentals over the points (report values).
@@ -387,72 +291,122 @@ __global__ void k_suvals(const BcArgs A, T* __restrict__ vals, double* __restric
 // ---------------------------------------------------------------------------
 struct SorArgs {
     int R, S, r0, r1;
+    int per_xcd;         // logical blocks per XCD of this launch
     const int* row_node;
     const int* cols;     // [S][R]
     double omega;
     const int* stop;     // device flag: nonzero -> skip the sweep
 };
 
-// One colour of one sweep on A y = b (y, b nodal vectors in DOF order).  The
-// three DOFs of a node are relaxed in x, y, z order with the fresh values of
-// the earlier ones, as the reference's natural-order sweep does within a node.
-// Accumulates |y_new - y_old|^2 and |y_new|^2 for the relative-update test.
-template <typename T, typename V>
+// One colour of one sweep on A y = b (y, b nodal vectors in DOF order).  LPR
+// lanes share a row: each sums every LPR-th off-diagonal block and a shuffle
+// tree combines them; lane 0 of the group then relaxes the node's three DOFs
+// in x, y, z order with the fresh values of the earlier ones, as the
+// reference's natural-order sweep does within a node.  SC > 0 fixes the slot
+// count at compile time (the 27-block rows of a structured brick mesh) so that
+// every load of a lane is issued before the first one is consumed.  With ACC
+// the kernel also accumulates |y_new - y_old|^2 and |y_new|^2 for the
+// relative-update test.
+template <typename T, typename V, bool ACC, int SC, int LPR>
 __global__ __launch_bounds__(256) void k_sor_color(const SorArgs A, const T* __restrict__ vals,
                                                    const V* __restrict__ b, V* __restrict__ y, double* partial,
                                                    int pslot0) {
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    const int r = A.r0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (*A.stop == 0 && r < A.r1) {
-        const size_t R = (size_t)A.R;
-        T sum[3] = {(T)0, (T)0, (T)0};
-        for (int sl = 1; sl < A.S; ++sl) {
-            const int c = A.cols[(size_t)sl * R + r];
-            const T y0 = (T)y[3 * c], y1 = (T)y[3 * c + 1], y2 = (T)y[3 * c + 2];
-            const T* vb = vals + (size_t)sl * 9 * R + r;
+    const int tid = xcd_block(A.per_xcd) * blockDim.x + threadIdx.x;
+    const int r = A.r0 + tid / LPR, q = tid % LPR;
+    const bool live = (r < A.r1) && (*A.stop == 0);
+    const size_t R = (size_t)A.R;
+    T sum[3] = {(T)0, (T)0, (T)0};
+    if (live) {
+        if (SC > 0) {
+            constexpr int PER = SC > 0 ? (SC - 1 + LPR - 1) / LPR : 1;
+            int c[PER];
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
-                sum[a] = sum[a] + vb[(3 * a + 0) * R] * y0 + vb[(3 * a + 1) * R] * y1 + vb[(3 * a + 2) * R] * y2;
+            for (int k = 0; k < PER; ++k) {
+                const int sl = 1 + q + k * LPR;
+                c[k] = (sl < SC) ? A.cols[(size_t)sl * R + r] : 0;
+            }
+            T yv[PER][3], m[PER][9];
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int sl = 1 + q + k * LPR;
+                if (sl < SC) {
+#pragma unroll
+                    for (int e = 0; e < 9; ++e) m[k][e] = vals[((size_t)sl * 9 + e) * R + r];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) yv[k][a] = (T)y[3 * c[k] + a];
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 9; ++e) m[k][e] = (T)0;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) yv[k][a] = (T)0;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < PER; ++k)
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    sum[a] = sum[a] + m[k][3 * a + 0] * yv[k][0] + m[k][3 * a + 1] * yv[k][1] +
+                             m[k][3 * a + 2] * yv[k][2];
+        } else {
+            for (int sl = 1 + q; sl < A.S; sl += LPR) {
+                const int c = A.cols[(size_t)sl * R + r];
+                const T y0 = (T)y[3 * c], y1 = (T)y[3 * c + 1], y2 = (T)y[3 * c + 2];
+                const T* vb = vals + (size_t)sl * 9 * R + r;
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    sum[a] = sum[a] + vb[(3 * a + 0) * R] * y0 + vb[(3 * a + 1) * R] * y1 + vb[(3 * a + 2) * R] * y2;
+            }
         }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int off = 1; off < LPR; off <<= 1) sum[a] = sum[a] + __shfl_xor(sum[a], off, 64);
+    if (live && q == 0) {
         const int nd = A.row_node[r];
         T d[9];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) d[q] = vals[(size_t)q * R + r];
+        for (int e = 0; e < 9; ++e) d[e] = vals[(size_t)e * R + r];
         T yn[3] = {(T)y[3 * nd], (T)y[3 * nd + 1], (T)y[3 * nd + 2]};
         const T om = (T)A.omega;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             T s = sum[a];
 #pragma unroll
-            for (int q = 0; q < 3; ++q)
-                if (q != a) s = s + d[3 * a + q] * yn[q];
+            for (int k = 0; k < 3; ++k)
+                if (k != a) s = s + d[3 * a + k] * yn[k];
             const T old = yn[a];
             const T nw = old + om * (((T)b[3 * nd + a] - s) / d[3 * a + a] - old);
             yn[a] = nw;
-            const double dd = (double)nw - (double)old;
-            v[0] += dd * dd;
-            v[1] += (double)nw * (double)nw;
+            if (ACC) {
+                const double dd = (double)nw - (double)old;
+                v[0] += dd * dd;
+                v[1] += (double)nw * (double)nw;
+            }
         }
 #pragma unroll
         for (int a = 0; a < 3; ++a) y[3 * nd + a] = (V)yn[a];
     }
-    block_partials<4>(v, partial, pslot0 + blockIdx.x);
+    if (ACC) block_partials<4>(v, partial, pslot0 + blockIdx.x);
 }
 
 // Residual r = b - A y in FP64 over all rows (A read in its stored precision).
 // Partial sums: |D^-1 r|^2 and |D^-1 b|^2 with D the scalar diagonal, so that
 // penalty rows (efix ~ 1e20) and stiffness rows are measured in the same
 // displacement units; the unscaled |r|^2 and |b|^2 are kept in slots 2, 3.
-template <typename T>
+template <typename T, int LPR>
 __global__ __launch_bounds__(256) void k_residual(const SorArgs A, const T* __restrict__ vals,
                                                   const double* __restrict__ b, const double* __restrict__ y,
                                                   double* __restrict__ res, double* partial) {
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < A.R) {
-        const size_t R = (size_t)A.R;
-        double sum[3] = {0.0, 0.0, 0.0};
-        for (int sl = 0; sl < A.S; ++sl) {
+    const int tid = xcd_block(A.per_xcd) * blockDim.x + threadIdx.x;
+    const int r = tid / LPR, q = tid % LPR;
+    const bool live = r < A.R;
+    const size_t R = (size_t)A.R;
+    double sum[3] = {0.0, 0.0, 0.0};
+    if (live) {
+        for (int sl = q; sl < A.S; sl += LPR) {
             const int c = (sl == 0) ? A.row_node[r] : A.cols[(size_t)sl * R + r];
             const double y0 = y[3 * c], y1 = y[3 * c + 1], y2 = y[3 * c + 2];
             const T* vb = vals + (size_t)sl * 9 * R + r;
@@ -461,6 +415,12 @@ __global__ __launch_bounds__(256) void k_residual(const SorArgs A, const T* __re
                 sum[a] += (double)vb[(3 * a + 0) * R] * y0 + (double)vb[(3 * a + 1) * R] * y1 +
                           (double)vb[(3 * a + 2) * R] * y2;
         }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int off = 1; off < LPR; off <<= 1) sum[a] = sum[a] + __shfl_xor(sum[a], off, 64);
+    if (live && q == 0) {
         const int nd = A.row_node[r];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -769,27 +729,6 @@ static int build_structure(Ctx* c, const cpfem_mesh* m, std::vector<int>& en, st
     return 0;
 }
 
-static void set_point_arrays(Ctx* c, PointArrays& P) {
-    memset(&P, 0, sizeof(P));
-    P.n = c->ngps;
-    P.st = c->st;
-    P.ds = c->ds;
-    P.sv = c->sv;
-    P.dsv = c->dsv;
-    P.t = c->t;
-    P.chi = c->chi;
-    P.deff = c->dep;
-    P.erepmd = c->erepmd;
-    P.srate = c->srate;
-    P.twinsys = c->twinsys;
-    P.timestwinned = c->timestwinned;
-    P.flops = c->flops;
-    P.transc = c->transc;
-    P.err = c->gperr;
-    P.tflag = c->flags;
-    P.errmax = c->flags + 1;
-    P.rng_epoch = c->rng_epoch;
-}
 
 }  // namespace cpfem
 
@@ -909,7 +848,7 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
         TRY(dalloc(c, &c->transc, G));
         TRY(dalloc(c, &c->flags, 8));
         // reduction scratch: enough partial slots for the largest launch
-        const int nbr = std::max(nblk(std::max<size_t>(G, D)) + 64, 4096);
+        const int nbr = std::max(nblk(std::max<size_t>(std::max<size_t>(G, D), 8 * (size_t)c->nnod)) + 64, 4096);
         c->red.nblocks = nbr;
         TRY(dalloc(c, &c->red.partial, (size_t)nbr * 4));
         TRY(dalloc(c, &c->red.result, 8));
@@ -925,6 +864,7 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
             TRY(hipMemcpyAsync(dm, hm.data(), mb, hipMemcpyHostToDevice, c->stream));
             TRY(hipStreamSynchronize(c->stream));
         }
+        TRY(alloc_schedule(c->sched, c->ngps));
         rc = build_structure(c, m, en, eg);
         if (rc) goto fail;
     }
@@ -939,6 +879,9 @@ fail:
 extern "C" int cpfem_destroy(cpfem_ctx* c) {
     if (!c) return 0;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_schedule(c->sched);
+    for (int i = 0; i < 3; ++i)
+        if (c->graphs[i]) (void)hipGraphExecDestroy(c->graphs[i]);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->red.host) (void)hipHostFree(c->red.host);
     for (int i = 0; i < 8; ++i)
@@ -1029,30 +972,43 @@ static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
     c->rng_epoch += 1;
     CPFEM_HIP(hipEventRecord(c->ev[0], s));
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, c->flags + 1, (size_t)1, 0);
-    PointMeshArgs PA;
-    PA.ngps = c->ngps;
-    PA.gp_elem = c->gp_elem;
-    PA.gp_local = c->gp_local;
-    PA.elem_nodes = c->elem_nodes;
-    PA.elem_lt = c->elem_lt;
-    PA.elem_le = c->elem_le;
-    PA.x = c->x;
-    PA.dx = c->dx;
-    PA.smale = c->ctl.smale;
-    PA.smalr = c->ctl.smalr;
-    PA.de = c->de;
-    PA.dr = c->dr;
-    PA.dep = c->dep;
-    set_point_arrays(c, PA.P);
+    ConstArgs A;
+    memset(&A, 0, sizeof(A));
+    A.n = c->ngps;
+    A.mesh = 1;
+    A.gp_elem = c->gp_elem;
+    A.gp_local = c->gp_local;
+    A.elem_nodes = c->elem_nodes;
+    A.elem_lt = c->elem_lt;
+    A.elem_le = c->elem_le;
+    A.x = c->x;
+    A.dx = c->dx;
+    A.smale = c->ctl.smale;
+    A.smalr = c->ctl.smalr;
+    A.de = c->de;
+    A.r = c->dr;
+    A.dep = c->dep;
+    A.st = c->st;
+    A.ds = c->ds;
+    A.sv = c->sv;
+    A.dsv = c->dsv;
+    A.t = c->t;
+    A.chi = c->chi;
+    A.erepmd = c->erepmd;
+    A.srate = c->srate;
+    A.twinsys = c->twinsys;
+    A.timestwinned = c->timestwinned;
+    A.flops = c->flops;
+    A.transc = c->transc;
+    A.err = c->gperr;
+    A.tflag = c->flags;
+    A.errmax = c->flags + 1;
+    A.rng_epoch = c->rng_epoch;
     const DevControl C = make_control(c->ctl);
-    const int gb = nblk(c->ngps, 64);
-    if (c->nss == 12)
-        hipLaunchKernelGGL(k_points_mesh<12>, dim3(gb), dim3(64), 0, s, (const DevMaterial<12>*)c->dmat, C, dt, PA);
-    else if (c->nss == 18)
-        hipLaunchKernelGGL(k_points_mesh<18>, dim3(gb), dim3(64), 0, s, (const DevMaterial<18>*)c->dmat, C, dt, PA);
-    else
-        return CPFEM_E_NSS;
-    CPFEM_HIP(hipGetLastError());
+    int counts[4] = {0, 0, 0, 0};
+    CPFEM_HIP(run_constitutive(c->nss, c->dmat, C, dt, A, c->sched, s, counts));
+    c->stats.pts_continued += counts[0];
+    c->stats.pts_wide += counts[1];
     CPFEM_HIP(hipEventRecord(c->ev[1], s));
     // zero fc and the matrix values
     const size_t D = 3 * (size_t)c->nnod, nv = (size_t)c->A.S * 9 * c->A.R;
@@ -1151,10 +1107,14 @@ extern "C" int cpfem_suvals(cpfem_ctx* c, int32_t inc, double* sfc) {
     return suvals_impl(c, inc, sfc);
 }
 
-// Number of sweeps queued between host checks of the stop flag.
-static const int kSweepBatch = 8;
+// Number of sweeps queued between host checks (one graph replay).
+static const int kSweepBatch = 16;
 
-template <typename T, typename V>
+// lanes per matrix row in the sweep and residual kernels
+static const int kLpr = 8;
+static const int kLprRes = 8;
+
+template <typename T, typename V, bool ACC>
 static void sor_sweep(Ctx* c, const T* vals, const V* b, V* y, double* partial) {
     SorArgs S;
     S.R = c->A.R;
@@ -1167,16 +1127,64 @@ static void sor_sweep(Ctx* c, const T* vals, const V* b, V* y, double* partial) 
     for (int k = 0; k < c->A.ncolor; ++k) {
         S.r0 = c->A.cstart[k];
         S.r1 = c->A.cstart[k + 1];
-        const int nb = nblk(S.r1 - S.r0);
-        hipLaunchKernelGGL((k_sor_color<T, V>), dim3(nb), dim3(kBlock), 0, c->stream, S, vals, b, y, partial, slot);
+        S.per_xcd = (nblk((size_t)kLpr * (S.r1 - S.r0)) + kXcd - 1) / kXcd;
+        const int nb = S.per_xcd * kXcd;
+        if (S.S == 27)
+            hipLaunchKernelGGL((k_sor_color<T, V, ACC, 27, kLpr>), dim3(nb), dim3(kBlock), 0, c->stream, S, vals, b,
+                               y, partial, slot);
+        else
+            hipLaunchKernelGGL((k_sor_color<T, V, ACC, 0, kLpr>), dim3(nb), dim3(kBlock), 0, c->stream, S, vals, b,
+                               y, partial, slot);
         slot += nb;
     }
 }
 
 static int sor_partial_blocks(Ctx* c) {
     int slot = 0;
-    for (int k = 0; k < c->A.ncolor; ++k) slot += nblk(c->A.cstart[k + 1] - c->A.cstart[k]);
+    for (int k = 0; k < c->A.ncolor; ++k)
+        slot += kXcd * ((nblk((size_t)kLpr * (c->A.cstart[k + 1] - c->A.cstart[k])) + kXcd - 1) / kXcd);
     return slot;
+}
+
+// A batch of sweeps captured once as a graph and replayed (kernel nodes only).
+static hipError_t sweep_graph(Ctx* c, int which, hipGraphExec_t* exec_out) {
+    if (c->graphs[which]) {
+        *exec_out = c->graphs[which];
+        return hipSuccess;
+    }
+    hipStream_t s = c->stream;
+    const int D = 3 * c->nnod;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    if (which == 0) {
+        // FP64 sweeps with the relative-update test after each one
+        const double tol2 = c->ctl.sor_tol * c->ctl.sor_tol;
+        const int npb = sor_partial_blocks(c);
+        for (int k = 0; k < kSweepBatch; ++k) {
+            sor_sweep<double, double, true>(c, (const double*)c->A.vals, c->rhs, c->ddx, c->red.partial);
+            hipLaunchKernelGGL(k_sor_check, dim3(1), dim3(kBlock), 0, s, (const double*)c->red.partial, npb, tol2,
+                               c->flags + 2, c->flags + 3, c->red.result + 4, c->ctl.sor_maxit);
+        }
+    } else if (which == 1) {
+        // FP64 sweeps, residual test by the caller
+        for (int k = 0; k < kSweepBatch; ++k)
+            sor_sweep<double, double, false>(c, (const double*)c->A.vals, c->rhs, c->ddx, c->red.partial);
+    } else {
+        // FP32 sweeps on the correction equation A d = r, r = b - A ddx in xold
+        hipLaunchKernelGGL(k_to_float, dim3(nblk(D)), dim3(kBlock), 0, s, (const double*)c->xold, c->rhs32, D);
+        hipLaunchKernelGGL(k_fill<float>, dim3(nblk(D)), dim3(kBlock), 0, s, c->del32, (size_t)D, 0.0f);
+        for (int k = 0; k < kSweepBatch; ++k)
+            sor_sweep<float, float, false>(c, (const float*)c->A.vals, c->rhs32, c->del32, c->red.partial);
+        hipLaunchKernelGGL(k_add_float, dim3(nblk(D)), dim3(kBlock), 0, s, c->ddx, (const float*)c->del32, D);
+    }
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess) return e;
+    e = hipGraphInstantiate(&c->graphs[which], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return e;
+    *exec_out = c->graphs[which];
+    return hipSuccess;
 }
 
 static double residual(Ctx* c, const double* b, const double* y, double* res) {
@@ -1187,13 +1195,16 @@ static double residual(Ctx* c, const double* b, const double* y, double* res) {
     S.cols = c->A.cols;
     S.omega = c->ctl.sor_omega;
     S.stop = nullptr;
-    const int nb = nblk(c->A.R);
+    S.r0 = 0;
+    S.r1 = c->A.R;
+    S.per_xcd = (nblk((size_t)kLprRes * c->A.R) + kXcd - 1) / kXcd;
+    const int nb = S.per_xcd * kXcd;
     if (c->precision == 1)
-        hipLaunchKernelGGL(k_residual<float>, dim3(nb), dim3(kBlock), 0, c->stream, S, (const float*)c->A.vals, b, y,
-                           res, c->red.partial);
+        hipLaunchKernelGGL((k_residual<float, kLprRes>), dim3(nb), dim3(kBlock), 0, c->stream, S,
+                           (const float*)c->A.vals, b, y, res, c->red.partial);
     else
-        hipLaunchKernelGGL(k_residual<double>, dim3(nb), dim3(kBlock), 0, c->stream, S, (const double*)c->A.vals, b, y,
-                           res, c->red.partial);
+        hipLaunchKernelGGL((k_residual<double, kLprRes>), dim3(nb), dim3(kBlock), 0, c->stream, S,
+                           (const double*)c->A.vals, b, y, res, c->red.partial);
     double r[4];
     reduce_now(c, nb, r);
     return r[1] > 0.0 ? sqrt(r[0] / r[1]) : 0.0;
@@ -1206,49 +1217,37 @@ static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out) {
     const int maxit = c->ctl.sor_maxit;
     CPFEM_HIP(hipEventRecord(c->ev[5], s));
     hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, s, c->ddx, (size_t)D, 0.0);
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, c->flags + 2, (size_t)2, 0);
     int sweeps = 0, code = 0;
     double err = 0.0;
-    const int npb = sor_partial_blocks(c);
     if (c->precision == 0 && c->ctl.sor_kind == 0) {
         // the reference's relative-update test after every sweep, decided on the device
         const double tol2 = c->ctl.sor_tol * c->ctl.sor_tol;
-        hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, c->flags + 2, (size_t)2, 0);
+        hipGraphExec_t gx;
+        CPFEM_HIP(sweep_graph(c, 0, &gx));
         int hf[2] = {0, 0};
         while (true) {
-            for (int k = 0; k < kSweepBatch; ++k) {
-                sor_sweep<double, double>(c, (const double*)c->A.vals, c->rhs, c->ddx, c->red.partial);
-                hipLaunchKernelGGL(k_sor_check, dim3(1), dim3(kBlock), 0, s, (const double*)c->red.partial, npb, tol2,
-                                   c->flags + 2, c->flags + 3, c->red.result + 4, maxit);
-            }
-            CPFEM_HIP(hipMemcpyAsync(hf, c->flags + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+            CPFEM_HIP(hipGraphLaunch(gx, s));
+            CPFEM_HIP(hipMemcpyAsync(c->red.host + 4, c->flags + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
             CPFEM_HIP(hipStreamSynchronize(s));
+            memcpy(hf, c->red.host + 4, 2 * sizeof(int));
             if (hf[0]) break;
         }
         sweeps = hf[1];
         CPFEM_HIP(hipMemcpy(&err, c->red.result + 4, sizeof(double), hipMemcpyDeviceToHost));
         if (err > 10.0 * tol2) code = 904;
     } else {
-        // relative-residual test every kSweepBatch sweeps; in mixed precision the
-        // sweeps run in FP32 on the correction equation A d = b - A ddx and the
-        // residual is formed in FP64 (iterative refinement).
-        hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, c->flags + 2, (size_t)2, 0);
+        // relative Jacobi-scaled residual every kSweepBatch sweeps; in mixed precision
+        // the sweeps run in FP32 on the correction equation and the residual is
+        // formed in FP64 (iterative refinement)
         double rel = 1.0;
-        if (c->precision == 1) {
+        if (c->precision == 1)
             CPFEM_HIP(hipMemcpyAsync(c->xold, c->rhs, D * sizeof(double), hipMemcpyDeviceToDevice, s));
-        }
+        hipGraphExec_t gx;
+        CPFEM_HIP(sweep_graph(c, c->precision == 1 ? 2 : 1, &gx));
         while (sweeps < maxit) {
-            const int nsw = std::min(kSweepBatch, maxit - sweeps);
-            if (c->precision == 1) {
-                hipLaunchKernelGGL(k_to_float, dim3(nblk(D)), dim3(kBlock), 0, s, (const double*)c->xold, c->rhs32, D);
-                hipLaunchKernelGGL(k_fill<float>, dim3(nblk(D)), dim3(kBlock), 0, s, c->del32, (size_t)D, 0.0f);
-                for (int k = 0; k < nsw; ++k)
-                    sor_sweep<float, float>(c, (const float*)c->A.vals, c->rhs32, c->del32, c->red.partial);
-                hipLaunchKernelGGL(k_add_float, dim3(nblk(D)), dim3(kBlock), 0, s, c->ddx, (const float*)c->del32, D);
-            } else {
-                for (int k = 0; k < nsw; ++k)
-                    sor_sweep<double, double>(c, (const double*)c->A.vals, c->rhs, c->ddx, c->red.partial);
-            }
-            sweeps += nsw;
+            CPFEM_HIP(hipGraphLaunch(gx, s));
+            sweeps += kSweepBatch;
             rel = residual(c, c->rhs, c->ddx, c->precision == 1 ? c->xold : nullptr);
             if (rel < c->ctl.sor_tol) break;
         }

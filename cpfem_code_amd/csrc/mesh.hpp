@@ -79,6 +79,8 @@ struct Ctx {
     uint64_t rng_epoch = 0;
     Matrix A;
     Reducer red;
+    ConstSchedule sched;
+    hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};  // sweep batches: FP64 update test, FP64, FP32
     // timing and counters of the last calls
     hipEvent_t ev[8];
     cpfem_stats stats;

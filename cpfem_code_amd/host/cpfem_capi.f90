@@ -42,7 +42,7 @@ MODULE CPFEM_CAPI
         INTEGER(C_INT32_T) :: nit, sor_sweeps, sor_sweeps_total, err
         REAL(C_DOUBLE) :: sdx, sfc, sumcpiter, alp2, sor_error
         REAL(C_DOUBLE) :: t_const_ms, t_asm_ms, t_suvals_ms, t_sor_ms, t_update_ms, t_total_ms
-        INTEGER(C_INT64_T) :: flops_const, transc_const, gp_updates
+        INTEGER(C_INT64_T) :: flops_const, transc_const, gp_updates, pts_continued, pts_wide
     END TYPE cpfem_stats
 
     INTERFACE

@@ -123,6 +123,8 @@ typedef struct {
     int64_t flops_const;      /* algorithmic FLOPs of the constitutive updates */
     int64_t transc_const;     /* their transcendental calls */
     int64_t gp_updates;       /* points updated (sum over NR iterations) */
+    int64_t pts_continued;    /* point updates that needed more than the first pass's outer iterations */
+    int64_t pts_wide;         /* point updates whose active set exceeded the register-resident width */
 } cpfem_stats;
 
 /* Upload mesh, materials and control; build the block sparsity, colourings
