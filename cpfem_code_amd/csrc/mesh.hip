@@ -103,14 +103,13 @@ __global__ void k_point_sums(const double* __restrict__ erepmd, const long long*
 // element assembly: one 64-lane wave per element, lane = node pair (i, j)
 // ---------------------------------------------------------------------------
 struct AsmArgs {
-    int R;                       // matrix rows
     int nel_color;
     const int* elems;            // elements of this colour
     const int* elem_nodes;
     const int* elem_gps;
     const int* elem_lt;
     const int* elem_np;
-    const int* scatter;          // [nel][64]
+    const int* scatter;          // [nel][64] entry 0 of each node pair's block; entry e at +64*e
     const double* x;
     const double* chi;           // [36][ngps]
     const double* t;             // [6][ngps]
@@ -180,13 +179,12 @@ __global__ __launch_bounds__(256) void k_assemble(const AsmArgs A, T* __restrict
 #pragma unroll
             for (int q = 0; q < 3; ++q) acc[p][q] = acc[p][q] + w * gm[p][q];
     }
-    const size_t R = (size_t)A.R;
     const int off = A.scatter[64 * e + lane];
 #pragma unroll
     for (int p = 0; p < 3; ++p)
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            T* dst = vals + off + (size_t)(3 * p + q) * R;
+            T* dst = vals + off + (3 * p + q) * 64;
             *dst = *dst + acc[p][q];
         }
     // internal forces of node i (Elvals.f90:122-126), lanes with j == 0
@@ -217,9 +215,9 @@ __global__ __launch_bounds__(256) void k_assemble(const AsmArgs A, T* __restrict
 // SUVALS: rhs = fc + penalty terms; penalty stiffness on the diagonal blocks
 // ---------------------------------------------------------------------------
 struct BcArgs {
-    int nnod, R;
+    int nnod;
     const int* ib;
-    const int* node_row;
+    const int* diag_off;  // [nnod] entry 0 of the node's diagonal block; entry e at +64*e
     const double* x;
     const double* dx;
     const double* fc;
@@ -230,7 +228,7 @@ struct BcArgs {
 };
 
 template <typename T>
-__global__ void k_suvals(const BcArgs A, T* __restrict__ vals, double* __restrict__ vals64, double* partial) {
+__global__ void k_suvals(const BcArgs A, T* __restrict__ vals, double* partial) {
     const int nd = blockIdx.x * blockDim.x + threadIdx.x;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     if (nd < A.nnod) {
@@ -256,15 +254,14 @@ __global__ void k_suvals(const BcArgs A, T* __restrict__ vals, double* __restric
                     }
                 }
                 const int point[3][3] = {{0, 1, 3}, {1, 2, 4}, {3, 4, 5}};
-                const size_t row = (size_t)A.node_row[nd], R = (size_t)A.R;
+                T* dg = vals + A.diag_off[nd];
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
 #pragma unroll
                     for (int j = 0; j < 3; ++j) {
                         const double add = A.efix * xs[point[i][j]];
-                        T* dst = vals + (size_t)(3 * i + j) * R + row;
+                        T* dst = dg + (3 * i + j) * 64;
                         *dst = *dst + (T)add;
-                        if (vals64) vals64[(size_t)(3 * i + j) * R + row] += add;
                     }
                     double sum = 0.0;
 #pragma unroll
@@ -290,84 +287,110 @@ __global__ void k_suvals(const BcArgs A, T* __restrict__ vals, double* __restric
 // multicolour block SOR
 // ---------------------------------------------------------------------------
 struct SorArgs {
-    int R, S, r0, r1;
-    int per_xcd;         // logical blocks per XCD of this launch
-    const int* row_node;
-    const int* cols;     // [S][R]
+    int nrows;           // rows of the launch (one colour)
+    int gbase;           // first row group of the launch
+    int per_xcd;         // logical blocks per XCD of the launch
+    int P;               // slots per lane
+    const int* cols;     // [group][P][64]
     double omega;
     const int* stop;     // device flag: nonzero -> skip the sweep
 };
 
-// One colour of one sweep on A y = b (y, b nodal vectors in DOF order).  LPR
-// lanes share a row: each sums every LPR-th off-diagonal block and a shuffle
-// tree combines them; lane 0 of the group then relaxes the node's three DOFs
-// in x, y, z order with the fresh values of the earlier ones, as the
-// reference's natural-order sweep does within a node.  SC > 0 fixes the slot
-// count at compile time (the 27-block rows of a structured brick mesh) so that
-// every load of a lane is issued before the first one is consumed.  With ACC
-// the kernel also accumulates |y_new - y_old|^2 and |y_new|^2 for the
-// relative-update test.
-template <typename T, typename V, bool ACC, int SC, int LPR>
-__global__ __launch_bounds__(256) void k_sor_color(const SorArgs A, const T* __restrict__ vals,
-                                                   const V* __restrict__ b, V* __restrict__ y, double* partial,
-                                                   int pslot0) {
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
-    const int tid = xcd_block(A.per_xcd) * blockDim.x + threadIdx.x;
-    const int r = A.r0 + tid / LPR, q = tid % LPR;
-    const bool live = (r < A.r1) && (*A.stop == 0);
-    const size_t R = (size_t)A.R;
-    T sum[3] = {(T)0, (T)0, (T)0};
-    if (live) {
-        if (SC > 0) {
-            constexpr int PER = SC > 0 ? (SC - 1 + LPR - 1) / LPR : 1;
-            int c[PER];
+constexpr int kWavesPerBlock = kBlock / 64;
+
+// Row sums of one group: lane (l, i) multiplies its slots p*kRowLanes + l of row
+// i with the gathered y and the kRowLanes partial sums of a row are combined on
+// its l == 0 lane.  Matrix values and columns are streamed with nontemporal
+// loads (each is read once per sweep); y stays cached.  With DIAG the slot 0
+// block is part of the sum (residual); without it lane l == 0 keeps the
+// diagonal block in d and the row's node in nd (relaxation).
+template <typename T, typename V, int PC, bool DIAG>
+__device__ __forceinline__ void group_row_sums(const T* __restrict__ vg, const int* __restrict__ cg, int Prt, int l,
+                                               const V* __restrict__ y, T sum[3], T d[9], int& nd) {
+    const int P = PC > 0 ? PC : Prt;
+    if (PC > 0) {
+        int c[PC > 0 ? PC : 1];
+        T m[PC > 0 ? PC : 1][9];
+        T yv[PC > 0 ? PC : 1][3];
 #pragma unroll
-            for (int k = 0; k < PER; ++k) {
-                const int sl = 1 + q + k * LPR;
-                c[k] = (sl < SC) ? A.cols[(size_t)sl * R + r] : 0;
-            }
-            T yv[PER][3], m[PER][9];
+        for (int p = 0; p < PC; ++p) c[p] = __builtin_nontemporal_load(cg + p * 64);
 #pragma unroll
-            for (int k = 0; k < PER; ++k) {
-                const int sl = 1 + q + k * LPR;
-                if (sl < SC) {
+        for (int p = 0; p < PC; ++p)
 #pragma unroll
-                    for (int e = 0; e < 9; ++e) m[k][e] = vals[((size_t)sl * 9 + e) * R + r];
+            for (int e = 0; e < 9; ++e) m[p][e] = __builtin_nontemporal_load(vg + (p * 9 + e) * 64);
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) yv[k][a] = (T)y[3 * c[k] + a];
-                } else {
+        for (int p = 0; p < PC; ++p)
 #pragma unroll
-                    for (int e = 0; e < 9; ++e) m[k][e] = (T)0;
+            for (int a = 0; a < 3; ++a) yv[p][a] = (T)y[3 * c[p] + a];
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) yv[k][a] = (T)0;
-                }
+        for (int p = 0; p < PC; ++p) {
+            if (p == 0 && l == 0) {
+#pragma unroll
+                for (int e = 0; e < 9; ++e) d[e] = m[0][e];
+                nd = c[0];
+                if (!DIAG) continue;
             }
 #pragma unroll
-            for (int k = 0; k < PER; ++k)
+            for (int a = 0; a < 3; ++a)
+                sum[a] = sum[a] + m[p][3 * a + 0] * yv[p][0] + m[p][3 * a + 1] * yv[p][1] + m[p][3 * a + 2] * yv[p][2];
+        }
+    } else {
+        for (int p = 0; p < P; ++p) {
+            const int c = __builtin_nontemporal_load(cg + p * 64);
+            T m[9];
 #pragma unroll
-                for (int a = 0; a < 3; ++a)
-                    sum[a] = sum[a] + m[k][3 * a + 0] * yv[k][0] + m[k][3 * a + 1] * yv[k][1] +
-                             m[k][3 * a + 2] * yv[k][2];
-        } else {
-            for (int sl = 1 + q; sl < A.S; sl += LPR) {
-                const int c = A.cols[(size_t)sl * R + r];
-                const T y0 = (T)y[3 * c], y1 = (T)y[3 * c + 1], y2 = (T)y[3 * c + 2];
-                const T* vb = vals + (size_t)sl * 9 * R + r;
+            for (int e = 0; e < 9; ++e) m[e] = __builtin_nontemporal_load(vg + (p * 9 + e) * 64);
+            const T y0 = (T)y[3 * c], y1 = (T)y[3 * c + 1], y2 = (T)y[3 * c + 2];
+            if (p == 0 && l == 0) {
 #pragma unroll
-                for (int a = 0; a < 3; ++a)
-                    sum[a] = sum[a] + vb[(3 * a + 0) * R] * y0 + vb[(3 * a + 1) * R] * y1 + vb[(3 * a + 2) * R] * y2;
+                for (int e = 0; e < 9; ++e) d[e] = m[e];
+                nd = c;
+                if (!DIAG) continue;
             }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) sum[a] = sum[a] + m[3 * a + 0] * y0 + m[3 * a + 1] * y1 + m[3 * a + 2] * y2;
         }
     }
+}
+
+// Sum of the kRowLanes partial sums of row i onto its lane l == 0 (lane i).
+template <typename T>
+__device__ __forceinline__ void row_combine(T sum[3], int lane) {
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < 3; ++a) {
+        T acc = sum[a];
 #pragma unroll
-        for (int off = 1; off < LPR; off <<= 1) sum[a] = sum[a] + __shfl_xor(sum[a], off, 64);
-    if (live && q == 0) {
-        const int nd = A.row_node[r];
-        T d[9];
-#pragma unroll
-        for (int e = 0; e < 9; ++e) d[e] = vals[(size_t)e * R + r];
+        for (int q = 1; q < kRowLanes; ++q) acc = acc + __shfl(sum[a], (lane + q * kGroupRows) & 63, 64);
+        sum[a] = acc;
+    }
+}
+
+// One colour of one sweep on A y = b (y, b nodal vectors in DOF order), one
+// row group per wave.  After the row sums, lane l == 0 of each row relaxes the
+// node's three DOFs in x, y, z order with the fresh values of the earlier ones,
+// as the reference's natural-order sweep does within a node.  PC > 0 fixes the
+// slots per lane at compile time (3 for the 27-block rows of a brick mesh) so
+// that every load of a lane is issued before the first one is consumed.  With
+// ACC the kernel also accumulates |y_new - y_old|^2 and |y_new|^2 for the
+// relative-update test.
+template <typename T, typename V, bool ACC, int PC>
+__global__ __launch_bounds__(kBlock) void k_sor_color(const SorArgs A, const T* __restrict__ vals,
+                                                      const V* __restrict__ b, V* __restrict__ y, double* partial,
+                                                      int pslot0) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    const int lane = threadIdx.x & 63;
+    const int w = xcd_block(A.per_xcd) * kWavesPerBlock + (threadIdx.x >> 6);
+    const int l = lane / kGroupRows, i = lane - l * kGroupRows;
+    const bool live = (l < kRowLanes) && (w * kGroupRows + i < A.nrows) && (*A.stop == 0);
+    const size_t g = (size_t)A.gbase + w;
+    T sum[3] = {(T)0, (T)0, (T)0};
+    T d[9];
+    int nd = 0;
+    if (live)
+        group_row_sums<T, V, PC, false>(vals + g * (size_t)A.P * 576 + lane, A.cols + g * (size_t)A.P * 64 + lane,
+                                        A.P, l, y, sum, d, nd);
+    row_combine<T>(sum, lane);
+    if (live && l == 0) {
         T yn[3] = {(T)y[3 * nd], (T)y[3 * nd + 1], (T)y[3 * nd + 2]};
         const T om = (T)A.omega;
 #pragma unroll
@@ -391,44 +414,53 @@ __global__ __launch_bounds__(256) void k_sor_color(const SorArgs A, const T* __r
     if (ACC) block_partials<4>(v, partial, pslot0 + blockIdx.x);
 }
 
-// Residual r = b - A y in FP64 over all rows (A read in its stored precision).
-// Partial sums: |D^-1 r|^2 and |D^-1 b|^2 with D the scalar diagonal, so that
-// penalty rows (efix ~ 1e20) and stiffness rows are measured in the same
-// displacement units; the unscaled |r|^2 and |b|^2 are kept in slots 2, 3.
-template <typename T, int LPR>
-__global__ __launch_bounds__(256) void k_residual(const SorArgs A, const T* __restrict__ vals,
-                                                  const double* __restrict__ b, const double* __restrict__ y,
-                                                  double* __restrict__ res, double* partial) {
+// Residual r = b - A y in FP64 over all row groups (A read in its stored
+// precision, products accumulated in FP64).  Partial sums: |D^-1 r|^2 and
+// |D^-1 b|^2 with D the scalar diagonal, so that penalty rows (efix ~ 1e20)
+// and stiffness rows are measured in the same displacement units; the
+// unscaled |r|^2 and |b|^2 are kept in slots 2, 3.  Padded rows carry column
+// -1 and are skipped.
+template <typename T, int PC>
+__global__ __launch_bounds__(kBlock) void k_residual(const SorArgs A, const T* __restrict__ vals,
+                                                     const double* __restrict__ b, const double* __restrict__ y,
+                                                     double* __restrict__ res, double* partial) {
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    const int tid = xcd_block(A.per_xcd) * blockDim.x + threadIdx.x;
-    const int r = tid / LPR, q = tid % LPR;
-    const bool live = r < A.R;
-    const size_t R = (size_t)A.R;
+    const int lane = threadIdx.x & 63;
+    const int w = xcd_block(A.per_xcd) * kWavesPerBlock + (threadIdx.x >> 6);
+    const int l = lane / kGroupRows;
+    const size_t g = (size_t)w;
+    bool live = (l < kRowLanes) && (w < A.nrows);   // nrows = number of groups here
+    const int* cg = A.cols + g * (size_t)A.P * 64 + lane;
+    if (live) live = cg[0] >= 0;
     double sum[3] = {0.0, 0.0, 0.0};
+    double d[9];
+    int nd = 0;
     if (live) {
-        for (int sl = q; sl < A.S; sl += LPR) {
-            const int c = (sl == 0) ? A.row_node[r] : A.cols[(size_t)sl * R + r];
-            const double y0 = y[3 * c], y1 = y[3 * c + 1], y2 = y[3 * c + 2];
-            const T* vb = vals + (size_t)sl * 9 * R + r;
+        const T* vg = vals + g * (size_t)A.P * 576 + lane;
+        for (int p = 0; p < (PC > 0 ? PC : A.P); ++p) {
+            const int c = __builtin_nontemporal_load(cg + p * 64);
+            double m[9];
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
-                sum[a] += (double)vb[(3 * a + 0) * R] * y0 + (double)vb[(3 * a + 1) * R] * y1 +
-                          (double)vb[(3 * a + 2) * R] * y2;
+            for (int e = 0; e < 9; ++e) m[e] = (double)__builtin_nontemporal_load(vg + (p * 9 + e) * 64);
+            if (p == 0 && l == 0) {
+#pragma unroll
+                for (int e = 0; e < 9; ++e) d[e] = m[e];
+                nd = c;
+            }
+            const double y0 = y[3 * c], y1 = y[3 * c + 1], y2 = y[3 * c + 2];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) sum[a] += m[3 * a + 0] * y0 + m[3 * a + 1] * y1 + m[3 * a + 2] * y2;
         }
     }
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int off = 1; off < LPR; off <<= 1) sum[a] = sum[a] + __shfl_xor(sum[a], off, 64);
-    if (live && q == 0) {
-        const int nd = A.row_node[r];
+    row_combine<double>(sum, lane);
+    if (live && l == 0) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const double ra = b[3 * nd + a] - sum[a];
-            const double d = (double)vals[(size_t)(4 * a) * R + r];
+            const double dd = d[4 * a];
             if (res) res[3 * nd + a] = ra;
-            v[0] += (ra / d) * (ra / d);
-            v[1] += (b[3 * nd + a] / d) * (b[3 * nd + a] / d);
+            v[0] += (ra / dd) * (ra / dd);
+            v[1] += (b[3 * nd + a] / dd) * (b[3 * nd + a] / dd);
             v[2] += ra * ra;
             v[3] += b[3 * nd + a] * b[3 * nd + a];
         }
@@ -664,12 +696,25 @@ static int build_structure(Ctx* c, const cpfem_mesh* m, std::vector<int>& en, st
         }
     }
     const int R = nnod;
-    std::vector<int> cols((size_t)S * R);
+    // row groups: each colour's rows cut into groups of kGroupRows
+    const int P = (S + kRowLanes - 1) / kRowLanes;
+    c->A.gstart.assign(ncol + 1, 0);
+    for (int k = 0; k < ncol; ++k)
+        c->A.gstart[k + 1] = c->A.gstart[k] + (c->A.cstart[k + 1] - c->A.cstart[k] + kGroupRows - 1) / kGroupRows;
+    const int ngroups = c->A.gstart[ncol];
+    if ((size_t)ngroups * P * 576 > 0x7fffffffull) return CPFEM_E_ARG;
+    std::vector<int> gpos(R);   // group position q of each row
+    for (int k = 0; k < ncol; ++k)
+        for (int r = c->A.cstart[k]; r < c->A.cstart[k + 1]; ++r)
+            gpos[r] = c->A.gstart[k] * kGroupRows + (r - c->A.cstart[k]);
+    std::vector<int> cols((size_t)ngroups * P * 64, -1), diag_off(nnod);
     for (int r = 0; r < R; ++r) {
         const int nd = row_node[r];
-        for (int s = 0; s < S; ++s) cols[(size_t)s * R + r] = s < (int)nb[nd].size() ? nb[nd][s] : nd;
+        for (int s = 0; s < P * kRowLanes; ++s)
+            cols[col_index(gpos[r], P, s)] = s < (int)nb[nd].size() ? nb[nd][s] : nd;
+        diag_off[nd] = (int)val_index(gpos[r], P, 0, 0);
     }
-    // element scatter map: slot*9*R + row for each node pair
+    // element scatter map: entry 0 of the block of each node pair
     std::vector<int> scat((size_t)nel * 64);
     for (int e = 0; e < nel; ++e)
         for (int a = 0; a < 8; ++a) {
@@ -679,9 +724,7 @@ static int build_structure(Ctx* c, const cpfem_mesh* m, std::vector<int>& en, st
                 const int nj = en[8 * e + b];
                 int slot = 0;
                 if (nj != ni) slot = (int)(std::lower_bound(v.begin() + 1, v.end(), nj) - v.begin());
-                const size_t off = (size_t)slot * 9 * R + node_row[ni];
-                if (off > 0x7fffffffull) return CPFEM_E_ARG;
-                scat[64 * e + 8 * a + b] = (int)off;
+                scat[64 * e + 8 * a + b] = (int)val_index(gpos[node_row[ni]], P, slot, 0);
             }
         }
     // element colouring (elements sharing a node differ)
@@ -711,17 +754,18 @@ static int build_structure(Ctx* c, const cpfem_mesh* m, std::vector<int>& en, st
     }
     c->A.R = R;
     c->A.S = S;
+    c->A.P = P;
     c->A.ncolor = ncol;
+    c->A.ngroups = ngroups;
     c->A.nnzb = nnzb;
     const size_t vbytes = (c->precision == 1) ? sizeof(float) : sizeof(double);
-    if (dalloc(c, &c->A.row_node, R) || dalloc(c, &c->A.node_row, R) || dalloc(c, &c->A.cols, (size_t)S * R) ||
+    if (dalloc(c, &c->A.diag_off, nnod) || dalloc(c, &c->A.cols, cols.size()) ||
         dalloc(c, &c->A.scatter, (size_t)nel * 64) || dalloc(c, &c->elem_by_color, nel))
         return CPFEM_E_HIP;
     char* vp = nullptr;
-    if (dalloc(c, &vp, (size_t)S * 9 * R * vbytes)) return CPFEM_E_HIP;
+    if (dalloc(c, &vp, c->A.nvals() * vbytes)) return CPFEM_E_HIP;
     c->A.vals = vp;
-    CPFEM_HIP(hipMemcpyAsync(c->A.row_node, row_node.data(), R * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    CPFEM_HIP(hipMemcpyAsync(c->A.node_row, node_row.data(), R * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    CPFEM_HIP(hipMemcpyAsync(c->A.diag_off, diag_off.data(), nnod * sizeof(int), hipMemcpyHostToDevice, c->stream));
     CPFEM_HIP(hipMemcpyAsync(c->A.cols, cols.data(), cols.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     CPFEM_HIP(hipMemcpyAsync(c->A.scatter, scat.data(), scat.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     CPFEM_HIP(hipMemcpyAsync(c->elem_by_color, ebc.data(), nel * sizeof(int), hipMemcpyHostToDevice, c->stream));
@@ -848,7 +892,9 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
         TRY(dalloc(c, &c->transc, G));
         TRY(dalloc(c, &c->flags, 8));
         // reduction scratch: enough partial slots for the largest launch
-        const int nbr = std::max(nblk(std::max<size_t>(std::max<size_t>(G, D), 8 * (size_t)c->nnod)) + 64, 4096);
+        // (row-group launches use at most nnod / (4 * kGroupRows) + 8 blocks per colour)
+        const int nbr = std::max(nblk(std::max<size_t>(std::max<size_t>(G, D), 8 * (size_t)c->nnod)) + 64,
+                                 (int)((size_t)c->nnod / (kWavesPerBlock * kGroupRows)) + 16 * 64 + 64);
         c->red.nblocks = nbr;
         TRY(dalloc(c, &c->red.partial, (size_t)nbr * 4));
         TRY(dalloc(c, &c->red.result, 8));
@@ -1011,7 +1057,7 @@ static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
     c->stats.pts_wide += counts[1];
     CPFEM_HIP(hipEventRecord(c->ev[1], s));
     // zero fc and the matrix values
-    const size_t D = 3 * (size_t)c->nnod, nv = (size_t)c->A.S * 9 * c->A.R;
+    const size_t D = 3 * (size_t)c->nnod, nv = c->A.nvals();
     hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, s, c->fc, D, 0.0);
     if (c->precision == 1) {
         hipLaunchKernelGGL(k_fill<float>, dim3(std::min(nblk(nv), 65536)), dim3(kBlock), 0, s, (float*)c->A.vals, nv,
@@ -1021,7 +1067,6 @@ static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
                            0.0);
     }
     AsmArgs AA;
-    AA.R = c->A.R;
     AA.elem_nodes = c->elem_nodes;
     AA.elem_gps = c->elem_gps;
     AA.elem_lt = c->elem_lt;
@@ -1074,9 +1119,8 @@ static int suvals_impl(Ctx* c, int inc, double* sfc) {
     CPFEM_HIP(hipEventRecord(c->ev[3], s));
     BcArgs B;
     B.nnod = c->nnod;
-    B.R = c->A.R;
     B.ib = c->ib;
-    B.node_row = c->A.node_row;
+    B.diag_off = c->A.diag_off;
     B.x = c->x;
     B.dx = c->dx;
     B.fc = c->fc;
@@ -1086,11 +1130,9 @@ static int suvals_impl(Ctx* c, int inc, double* sfc) {
     B.rhs = c->rhs;
     const int nb = nblk(c->nnod);
     if (c->precision == 1)
-        hipLaunchKernelGGL(k_suvals<float>, dim3(nb), dim3(kBlock), 0, s, B, (float*)c->A.vals, (double*)nullptr,
-                           c->red.partial);
+        hipLaunchKernelGGL(k_suvals<float>, dim3(nb), dim3(kBlock), 0, s, B, (float*)c->A.vals, c->red.partial);
     else
-        hipLaunchKernelGGL(k_suvals<double>, dim3(nb), dim3(kBlock), 0, s, B, (double*)c->A.vals, (double*)nullptr,
-                           c->red.partial);
+        hipLaunchKernelGGL(k_suvals<double>, dim3(nb), dim3(kBlock), 0, s, B, (double*)c->A.vals, c->red.partial);
     CPFEM_HIP(hipGetLastError());
     double r[4];
     reduce_now(c, nb, r);
@@ -1110,39 +1152,35 @@ extern "C" int cpfem_suvals(cpfem_ctx* c, int32_t inc, double* sfc) {
 // Number of sweeps queued between host checks (one graph replay).
 static const int kSweepBatch = 16;
 
-// lanes per matrix row in the sweep and residual kernels
-static const int kLpr = 8;
-static const int kLprRes = 8;
+// Blocks per XCD of a launch over ng row groups.
+static int group_per_xcd(int ng) { return ((ng + kWavesPerBlock - 1) / kWavesPerBlock + kXcd - 1) / kXcd; }
 
 template <typename T, typename V, bool ACC>
 static void sor_sweep(Ctx* c, const T* vals, const V* b, V* y, double* partial) {
     SorArgs S;
-    S.R = c->A.R;
-    S.S = c->A.S;
-    S.row_node = c->A.row_node;
+    S.P = c->A.P;
     S.cols = c->A.cols;
     S.omega = c->ctl.sor_omega;
     S.stop = c->flags + 2;
     int slot = 0;
     for (int k = 0; k < c->A.ncolor; ++k) {
-        S.r0 = c->A.cstart[k];
-        S.r1 = c->A.cstart[k + 1];
-        S.per_xcd = (nblk((size_t)kLpr * (S.r1 - S.r0)) + kXcd - 1) / kXcd;
+        S.nrows = c->A.cstart[k + 1] - c->A.cstart[k];
+        S.gbase = c->A.gstart[k];
+        S.per_xcd = group_per_xcd(c->A.gstart[k + 1] - c->A.gstart[k]);
         const int nb = S.per_xcd * kXcd;
-        if (S.S == 27)
-            hipLaunchKernelGGL((k_sor_color<T, V, ACC, 27, kLpr>), dim3(nb), dim3(kBlock), 0, c->stream, S, vals, b,
-                               y, partial, slot);
+        if (S.P == 3)
+            hipLaunchKernelGGL((k_sor_color<T, V, ACC, 3>), dim3(nb), dim3(kBlock), 0, c->stream, S, vals, b, y,
+                               partial, slot);
         else
-            hipLaunchKernelGGL((k_sor_color<T, V, ACC, 0, kLpr>), dim3(nb), dim3(kBlock), 0, c->stream, S, vals, b,
-                               y, partial, slot);
+            hipLaunchKernelGGL((k_sor_color<T, V, ACC, 0>), dim3(nb), dim3(kBlock), 0, c->stream, S, vals, b, y,
+                               partial, slot);
         slot += nb;
     }
 }
 
 static int sor_partial_blocks(Ctx* c) {
     int slot = 0;
-    for (int k = 0; k < c->A.ncolor; ++k)
-        slot += kXcd * ((nblk((size_t)kLpr * (c->A.cstart[k + 1] - c->A.cstart[k])) + kXcd - 1) / kXcd);
+    for (int k = 0; k < c->A.ncolor; ++k) slot += kXcd * group_per_xcd(c->A.gstart[k + 1] - c->A.gstart[k]);
     return slot;
 }
 
@@ -1189,22 +1227,29 @@ static hipError_t sweep_graph(Ctx* c, int which, hipGraphExec_t* exec_out) {
 
 static double residual(Ctx* c, const double* b, const double* y, double* res) {
     SorArgs S;
-    S.R = c->A.R;
-    S.S = c->A.S;
-    S.row_node = c->A.row_node;
+    S.nrows = c->A.ngroups;
+    S.gbase = 0;
+    S.P = c->A.P;
     S.cols = c->A.cols;
     S.omega = c->ctl.sor_omega;
     S.stop = nullptr;
-    S.r0 = 0;
-    S.r1 = c->A.R;
-    S.per_xcd = (nblk((size_t)kLprRes * c->A.R) + kXcd - 1) / kXcd;
+    S.per_xcd = group_per_xcd(c->A.ngroups);
     const int nb = S.per_xcd * kXcd;
-    if (c->precision == 1)
-        hipLaunchKernelGGL((k_residual<float, kLprRes>), dim3(nb), dim3(kBlock), 0, c->stream, S,
-                           (const float*)c->A.vals, b, y, res, c->red.partial);
-    else
-        hipLaunchKernelGGL((k_residual<double, kLprRes>), dim3(nb), dim3(kBlock), 0, c->stream, S,
-                           (const double*)c->A.vals, b, y, res, c->red.partial);
+    if (c->precision == 1) {
+        if (S.P == 3)
+            hipLaunchKernelGGL((k_residual<float, 3>), dim3(nb), dim3(kBlock), 0, c->stream, S,
+                               (const float*)c->A.vals, b, y, res, c->red.partial);
+        else
+            hipLaunchKernelGGL((k_residual<float, 0>), dim3(nb), dim3(kBlock), 0, c->stream, S,
+                               (const float*)c->A.vals, b, y, res, c->red.partial);
+    } else {
+        if (S.P == 3)
+            hipLaunchKernelGGL((k_residual<double, 3>), dim3(nb), dim3(kBlock), 0, c->stream, S,
+                               (const double*)c->A.vals, b, y, res, c->red.partial);
+        else
+            hipLaunchKernelGGL((k_residual<double, 0>), dim3(nb), dim3(kBlock), 0, c->stream, S,
+                               (const double*)c->A.vals, b, y, res, c->red.partial);
+    }
     double r[4];
     reduce_now(c, nb, r);
     return r[1] > 0.0 ? sqrt(r[0] / r[1]) : 0.0;
@@ -1446,21 +1491,24 @@ extern "C" int cpfem_get_system(cpfem_ctx* c, double* rhs, double* ddx) {
 extern "C" int cpfem_get_matrix(cpfem_ctx* c, int64_t* nnz, int32_t* rows, int32_t* cols, double* vals) {
     if (!c || !nnz) return CPFEM_E_ARG;
     CPFEM_HIP(hipStreamSynchronize(c->stream));
-    const int R = c->A.R, S = c->A.S;
-    std::vector<int> rn(R), cl((size_t)S * R);
-    CPFEM_HIP(hipMemcpy(rn.data(), c->A.row_node, R * 4, hipMemcpyDeviceToHost));
+    const int P = c->A.P, ng = c->A.ngroups;
+    const int nslot = P * kRowLanes;
+    std::vector<int> cl((size_t)ng * P * 64);
     CPFEM_HIP(hipMemcpy(cl.data(), c->A.cols, cl.size() * 4, hipMemcpyDeviceToHost));
-    // count real blocks: slot 0 plus the slots whose column differs from the row's node
+    // real blocks: slot 0 plus the slots whose column differs from the row's node
     int64_t nb = 0;
-    for (int r = 0; r < R; ++r)
-        for (int s = 0; s < S; ++s)
-            if (s == 0 || cl[(size_t)s * R + r] != rn[r]) ++nb;
+    for (int q = 0; q < ng * kGroupRows; ++q) {
+        const int nd = cl[col_index(q, P, 0)];
+        if (nd < 0) continue;
+        for (int s = 0; s < nslot; ++s)
+            if (s == 0 || cl[col_index(q, P, s)] != nd) ++nb;
+    }
     if (!rows || !cols || !vals) {
         *nnz = nb * 9;
         return 0;
     }
     if (*nnz < nb * 9) return CPFEM_E_ARG;
-    const size_t nv = (size_t)S * 9 * R;
+    const size_t nv = c->A.nvals();
     std::vector<double> v(nv);
     if (c->precision == 1) {
         std::vector<float> f(nv);
@@ -1470,16 +1518,17 @@ extern "C" int cpfem_get_matrix(cpfem_ctx* c, int64_t* nnz, int32_t* rows, int32
         CPFEM_HIP(hipMemcpy(v.data(), c->A.vals, nv * 8, hipMemcpyDeviceToHost));
     }
     int64_t k = 0;
-    for (int r = 0; r < R; ++r) {
-        const int nd = rn[r];
-        for (int s = 0; s < S; ++s) {
-            const int cn = (s == 0) ? nd : cl[(size_t)s * R + r];
+    for (int q = 0; q < ng * kGroupRows; ++q) {
+        const int nd = cl[col_index(q, P, 0)];
+        if (nd < 0) continue;
+        for (int s = 0; s < nslot; ++s) {
+            const int cn = cl[col_index(q, P, s)];
             if (s != 0 && cn == nd) continue;
             for (int a = 0; a < 3; ++a)
                 for (int b = 0; b < 3; ++b) {
                     rows[k] = 3 * nd + a;
                     cols[k] = 3 * cn + b;
-                    vals[k] = v[((size_t)s * 9 + 3 * a + b) * R + r];
+                    vals[k] = v[val_index(q, P, s, 3 * a + b)];
                     ++k;
                 }
         }

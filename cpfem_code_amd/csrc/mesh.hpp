@@ -8,11 +8,18 @@
 //   sv, dsv                        [3+nss][ngps]
 //   srate, cumslip                 [nss][ngps]
 //   chi                            [36][ngps] tangent of the last pass
-//   matrix (block ELL over nodes)  vals[slot][9][R] (FP64 or FP32), cols[slot][R]
-// Matrix rows are the nodes permuted colour-major: rows of one colour are
-// contiguous, so a colour sweep reads vals/cols fully coalesced.  Slot 0 of a
-// row is its diagonal 3x3 block; padded slots point at the row's own node with
-// zero values.
+//   matrix (block ELL over nodes, row groups; see below)
+// Matrix rows are the nodes permuted colour-major.  Each colour's rows are cut
+// into groups of kGroupRows rows, one group per 64-lane wave, kRowLanes lanes
+// per row (lane = l*kGroupRows + i for row i of the group, lane 63 idle).
+// Slot s of a row (slot 0 = the diagonal 3x3 block, then the neighbour nodes in
+// ascending order, padded with the row's own node and zero values) belongs to
+// lane l = s % kRowLanes at depth p = s / kRowLanes, and a group stores
+//   vals[group][p][e][lane]  (e = 3*a + b of the 3x3 block)
+//   cols[group][p][lane]
+// so one load instruction of a wave reads 64 consecutive values: a sweep
+// streams each group's P*9*64 values as whole contiguous segments.  Padded rows
+// of a colour's last group have column -1 and zero values.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -31,18 +38,34 @@ struct Reducer {
     int nblocks = 0;
 };
 
+constexpr int kRowLanes = 9;                     // lanes per matrix row
+constexpr int kGroupRows = 64 / kRowLanes;       // rows per group (= per wave)
+
+// Index of entry e of slot s of the row at group position q (q = group *
+// kGroupRows + row within the group), for P slots per lane.
+__host__ __device__ inline size_t val_index(int q, int P, int s, int e) {
+    return (size_t)(q / kGroupRows) * P * 576 + (size_t)((s / kRowLanes) * 9 + e) * 64 +
+           (s % kRowLanes) * kGroupRows + q % kGroupRows;
+}
+__host__ __device__ inline size_t col_index(int q, int P, int s) {
+    return (size_t)(q / kGroupRows) * P * 64 + (size_t)(s / kRowLanes) * 64 + (s % kRowLanes) * kGroupRows +
+           q % kGroupRows;
+}
+
 struct Matrix {
     int R = 0;            // rows (= nodes)
     int S = 0;            // slots per row (max neighbours incl. self)
+    int P = 0;            // slots per lane, ceil(S / kRowLanes)
     int ncolor = 0;
+    int ngroups = 0;
     std::vector<int> cstart;   // [ncolor+1] row range of each colour
-    int* row_node = nullptr;   // [R] permuted row -> node (0-based)
-    int* node_row = nullptr;   // [R] node -> row
-    int* cols = nullptr;       // [S][R] neighbour node of each slot
-    void* vals = nullptr;      // [S][9][R] double or float
-    double* vals64 = nullptr;  // FP64 copy for residuals in mixed mode (null in FP64 mode)
-    int* scatter = nullptr;    // [nel][64] slot*9*R + row of each element node pair
+    std::vector<int> gstart;   // [ncolor+1] group range of each colour
+    int* cols = nullptr;       // [ngroups][P][64] neighbour node of each slot
+    void* vals = nullptr;      // [ngroups][P][9][64] double or float
+    int* diag_off = nullptr;   // [nnod] index of entry 0 of each node's diagonal block
+    int* scatter = nullptr;    // [nel][64] index of entry 0 of the block of each element node pair
     size_t nnzb = 0;           // true (unpadded) blocks
+    size_t nvals() const { return (size_t)ngroups * P * 576; }
 };
 
 struct Ctx {
