@@ -9,6 +9,15 @@ with all state resident in HBM.  The FP64 path and the mixed path (FP32
 element products and SOR sweeps, FP64 residual refinement) are both timed;
 ``value`` is the FP64 path's point-update rate.
 
+``roofline`` describes the dominant kernel, the FP64 colour-sweep kernel:
+algorithmic bytes per launch (matrix values, columns, row pointers and the
+vectors of one sweep, divided over the colour launches) over its average
+launch time, taken from HIP events recorded on the library's stream around
+every replay of the sweep graphs inside the timed increments.  ``traffic`` is
+the HBM bytes per launch that the FETCH_SIZE and WRITE_SIZE counters measured
+for the same kernel (profiles/pmc_traffic.json, written from rocprofv3 runs of
+cpfem_code_amd/sweep_probe.py), or null when that file is absent.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 64]
 """
 import argparse
@@ -43,6 +52,16 @@ def nnzb_structured(dims):
     return f(nx) * f(ny) * f(nz)
 
 
+def load_traffic():
+    """HBM bytes per launch of the sweep kernels from the committed counter summary."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return {k: v["hbm_bytes_per_launch"] for k, v in d.get("kernels", {}).items()}
+
+
 def run_path(prob, precision, warmup, steps, capi):
     t0 = time.time()
     sv = capi.Solver(prob, precision=precision)
@@ -63,7 +82,8 @@ def run_path(prob, precision, warmup, steps, capi):
         recs.append(st)
         log(f"[bench] inc {inc}: nit {st['nit']} sweeps {st['sor_sweeps_total']} total {st['t_total_ms']:.1f} ms "
             f"const {st['t_const_ms']:.1f} asm {st['t_asm_ms']:.1f} sor {st['t_sor_ms']:.1f} code {st['code']} "
-            f"pts {st['gp_updates']} continued {st['pts_continued']} wide {st['pts_wide']}")
+            f"pts {st['gp_updates']} continued {st['pts_continued']} wide {st['pts_wide']} "
+            f"sweep launches {st['sweep_launches']} {st['t_sweep_ms']:.1f} ms")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     state = sv.get_state()
@@ -99,7 +119,7 @@ def cpu_baseline(prob, state, seconds, capi_mod):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--paths", default="fp64,mixed")
@@ -139,27 +159,40 @@ def main():
     ms_per_inc = 1e3 * wall64 / args.steps
     value = gp_updates / wall64
     nnzb = nnzb_structured(prob.dims)
-    # roofline of each hot kernel, algorithmic work / event-timed duration
+    traffic = load_traffic()
+    # roofline of each hot kernel: algorithmic work per launch / event-timed launch duration
     asm_flops = 75968 * prob.nlmnt * nit
-    sor_b64 = sor_bytes(prob, nnzb, 8, 8) * sweeps
+
+    def sweep_roof(recs, s_a, s_x, key):
+        launches = sum(r["sweep_launches"] for r in recs)
+        t = sum(r["t_sweep_ms"] for r in recs) * 1e-3
+        nsw = sum(r["sor_sweeps_total"] for r in recs)
+        per_launch = sor_bytes(prob, nnzb, s_a, s_x) * nsw / launches
+        dur = t / launches
+        tr = traffic.get(key) if traffic else None
+        return {"bound": "hbm", "achieved": per_launch / dur / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "traffic": tr, "frac": per_launch / dur / HBM_PEAK, "bytes_per_launch": per_launch,
+                "launch_us": dur * 1e6, "launches": launches}
+
     roof = {
+        "sor_fp64": sweep_roof(recs64, 8, 8, "k_sor_color<double>"),
         "constitutive": {"bound": "mfma", "achieved": flops / t_const / 1e12, "peak": FP64_PEAK / 1e12,
-                         "unit": "TFLOP/s", "traffic": None},
+                         "unit": "TFLOP/s", "traffic": None, "frac": flops / t_const / FP64_PEAK},
         "assembly": {"bound": "mfma", "achieved": asm_flops / t_asm / 1e12, "peak": FP64_PEAK / 1e12,
-                     "unit": "TFLOP/s", "traffic": None},
-        "sor": {"bound": "hbm", "achieved": sor_b64 / t_sor / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "traffic": None},
+                     "unit": "TFLOP/s", "traffic": None, "frac": asm_flops / t_asm / FP64_PEAK},
     }
-    for v in roof.values():
-        v["frac"] = v["achieved"] / v["peak"]
-    dom = max((("constitutive", t_const), ("assembly", t_asm), ("sor", t_sor)), key=lambda kv: kv[1])[0]
     extra = {}
     if "mixed" in results:
         recs32, wall32 = results["mixed"]
+        roof["sor_fp32"] = sweep_roof(recs32, 4, 4, "k_sor_color<float>")
         extra["s_per_increment_mixed"] = wall32 / args.steps
         extra["mixed_sor_s_per_increment"] = sum(r["t_sor_ms"] for r in recs32) * 1e-3 / args.steps
         extra["mixed_sweeps_per_increment"] = sum(r["sor_sweeps_total"] for r in recs32) / args.steps
+        extra["mixed_nr_iterations_per_increment"] = sum(r["nit"] for r in recs32) / args.steps
         extra["mixed_speedup_vs_fp64"] = wall64 / wall32
+        extra["faferr_codes_per_increment_mixed"] = [r["code"] for r in recs32]
+    shares = {"sor_fp64": t_sor, "constitutive": t_const, "assembly": t_asm}
+    dom = max(shares, key=shares.get)
     cpu = cpu_baseline(prob, last_state, args.cpu_seconds, capi) if args.cpu_seconds > 0 else None
     line = {
         "metric": "Gauss-pt updates/s + s/increment at 64^3 FCC, FP64 vs mixed-prec SOR",
@@ -186,7 +219,8 @@ def main():
         "constitutive_s_per_increment": t_const / args.steps,
         "constitutive_flops_per_point_update": flops / gp_updates,
         "constitutive_transcendentals_per_point_update": transc / gp_updates,
-        "roofline": dict(roof[dom], kernel=dom),
+        "roofline": dict(roof[dom], kernel={"sor_fp64": "k_sor_color<double> (one colour of one SOR sweep)",
+                                            "constitutive": "k_const", "assembly": "k_assemble<double>"}[dom]),
         "roofline_all": roof,
         "cpu_baseline": cpu,
         "faferr_codes_per_increment": [r["code"] for r in recs64],

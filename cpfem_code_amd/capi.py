@@ -154,7 +154,8 @@ class CpfemStats(C.Structure):
                 ("sor_error", C.c_double), ("t_const_ms", C.c_double), ("t_asm_ms", C.c_double),
                 ("t_suvals_ms", C.c_double), ("t_sor_ms", C.c_double), ("t_update_ms", C.c_double),
                 ("t_total_ms", C.c_double), ("flops_const", C.c_int64), ("transc_const", C.c_int64),
-                ("gp_updates", C.c_int64), ("pts_continued", C.c_int64), ("pts_wide", C.c_int64)]
+                ("gp_updates", C.c_int64), ("pts_continued", C.c_int64), ("pts_wide", C.c_int64),
+                ("t_sweep_ms", C.c_double), ("sweep_launches", C.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -326,6 +327,12 @@ class Solver:
         d = st.as_dict()
         d["code"] = code
         return d
+
+    def stats(self):
+        """Counters and timers accumulated since the last NRITER (or since creation)."""
+        st = CpfemStats()
+        _check(self.L.cpfem_get_stats(self.h, C.byref(st)), "cpfem_get_stats")
+        return st.as_dict()
 
     def system(self):
         d = self.prob.ndof

@@ -838,7 +838,7 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
     } while (0)
     {
         TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        for (int i = 0; i < 8; ++i) TRY(hipEventCreate(&c->ev[i]));
+        for (int i = 0; i < Ctx::kEvents; ++i) TRY(hipEventCreate(&c->ev[i]));
         const size_t G = c->ngps, D = 3 * (size_t)c->nnod, nsv = c->nsv;
         TRY(dalloc(c, &c->elem_nodes, en.size()));
         TRY(dalloc(c, &c->elem_gps, eg.size()));
@@ -930,7 +930,7 @@ extern "C" int cpfem_destroy(cpfem_ctx* c) {
         if (c->graphs[i]) (void)hipGraphExecDestroy(c->graphs[i]);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->red.host) (void)hipHostFree(c->red.host);
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < Ctx::kEvents; ++i)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1272,9 +1272,13 @@ static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out) {
         CPFEM_HIP(sweep_graph(c, 0, &gx));
         int hf[2] = {0, 0};
         while (true) {
+            CPFEM_HIP(hipEventRecord(c->ev[7], s));
             CPFEM_HIP(hipGraphLaunch(gx, s));
+            CPFEM_HIP(hipEventRecord(c->ev[8], s));
             CPFEM_HIP(hipMemcpyAsync(c->red.host + 4, c->flags + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
             CPFEM_HIP(hipStreamSynchronize(s));
+            c->stats.t_sweep_ms += elapsed(c->ev[7], c->ev[8]);
+            c->stats.sweep_launches += (int64_t)kSweepBatch * c->A.ncolor;
             memcpy(hf, c->red.host + 4, 2 * sizeof(int));
             if (hf[0]) break;
         }
@@ -1291,9 +1295,13 @@ static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out) {
         hipGraphExec_t gx;
         CPFEM_HIP(sweep_graph(c, c->precision == 1 ? 2 : 1, &gx));
         while (sweeps < maxit) {
+            CPFEM_HIP(hipEventRecord(c->ev[7], s));
             CPFEM_HIP(hipGraphLaunch(gx, s));
+            CPFEM_HIP(hipEventRecord(c->ev[8], s));
             sweeps += kSweepBatch;
             rel = residual(c, c->rhs, c->ddx, c->precision == 1 ? c->xold : nullptr);
+            c->stats.t_sweep_ms += elapsed(c->ev[7], c->ev[8]);
+            c->stats.sweep_launches += (int64_t)kSweepBatch * c->A.ncolor;
             if (rel < c->ctl.sor_tol) break;
         }
         err = rel;

@@ -105,7 +105,8 @@ struct Ctx {
     ConstSchedule sched;
     hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};  // sweep batches: FP64 update test, FP64, FP32
     // timing and counters of the last calls
-    hipEvent_t ev[8];
+    static constexpr int kEvents = 10;   // [7], [8]: around each sweep batch
+    hipEvent_t ev[kEvents] = {};
     cpfem_stats stats;
     std::vector<void*> allocs;
 };

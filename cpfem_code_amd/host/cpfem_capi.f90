@@ -43,6 +43,8 @@ MODULE CPFEM_CAPI
         REAL(C_DOUBLE) :: sdx, sfc, sumcpiter, alp2, sor_error
         REAL(C_DOUBLE) :: t_const_ms, t_asm_ms, t_suvals_ms, t_sor_ms, t_update_ms, t_total_ms
         INTEGER(C_INT64_T) :: flops_const, transc_const, gp_updates, pts_continued, pts_wide
+        REAL(C_DOUBLE) :: t_sweep_ms
+        INTEGER(C_INT64_T) :: sweep_launches
     END TYPE cpfem_stats
 
     INTERFACE

@@ -125,6 +125,8 @@ typedef struct {
     int64_t gp_updates;       /* points updated (sum over NR iterations) */
     int64_t pts_continued;    /* point updates that needed more than the first pass's outer iterations */
     int64_t pts_wide;         /* point updates whose active set exceeded the register-resident width */
+    double t_sweep_ms;        /* device time of the sweep-kernel batches alone (events around each batch) */
+    int64_t sweep_launches;   /* colour-sweep kernel launches in those batches */
 } cpfem_stats;
 
 /* Upload mesh, materials and control; build the block sparsity, colourings
