@@ -1178,8 +1178,11 @@ int orc_facsls(orc_model *m, int ndof) {
 int orc_nriter(orc_model *m) {
     const real toler = R(m->ctl->toler), alpha = R(m->ctl->alpha);
     const int ndof = 3 * m->nnod;
-    int nit = 0, l_conv = 1, err = 0;
+    int nit = 0, l_conv = 1, err = 0, worst = 0;
     real sdx = R(0.0), sfc, alp2 = R(1.0);
+    /* codes that end the loop: all fatal ones, or with keep_going those other
+     * than 987 (slip Newton), 910 (singular slip matrix) and 904 (SOR) */
+#define ORC_STOPS(e) ((e) > 500 && !(m->ctl->keep_going && ((e) == 987 || (e) == 910 || (e) == 904)))
     while (l_conv) {
         ++nit;
         m->sumcpiter = 0.0;
@@ -1187,7 +1190,8 @@ int orc_nriter(orc_model *m) {
         for (size_t i = 0; i < (size_t)ndof * m->maxofd; ++i) m->ss[i] = 0.0;
         m->rng_calls += 1;
         err = orc_elvals(m);
-        if (err > 500) break;
+        if (err > worst) worst = err;
+        if (ORC_STOPS(err)) break;
         orc_suvals(m);
         sfc = R(0.0);
         for (int i = 0; i < m->nnod; ++i)
@@ -1197,7 +1201,8 @@ int orc_nriter(orc_model *m) {
             }
         sfc = sqrt(sfc);
         err = orc_facsls(m, ndof);
-        if (err > 500) break;
+        if (err > worst) worst = err;
+        if (ORC_STOPS(err)) break;
         real dd = R(0.0), ee = R(0.0);
         for (int i = 0; i < ndof; ++i) dd = dd + R(m->dx[i]) * R(m->dx[i]);
         for (int i = 0; i < ndof; ++i) ee = ee + R(m->ddx[i]) * R(m->ddx[i]);
@@ -1216,7 +1221,10 @@ int orc_nriter(orc_model *m) {
     m->nit = nit;
     m->sdx = sdx;
     m->alp2 = alp2;
-    if (err <= 500) {
+    const int stopped = ORC_STOPS(err);
+#undef ORC_STOPS
+    err = worst;
+    if (!stopped) {
         /* cumulative slip, Stif_sol.f90:80-84 */
         int ns = m->mat[(m->mat_code > 0 ? m->mat_code : 1) - 1].nss;
         real dt = R(m->tiem[m->inc - 1]);

@@ -46,7 +46,9 @@ typedef struct {
     double sor_omega;      /* Stif_sol.f90:173 alphal (1.2) */
     double sor_tol;        /* tolerance of the stopping test; reference: toler */
     int32_t sor_kind;      /* 0: reference |dx|^2/|x|^2 < tol^2; 1: |D^-1(b-Ax)|/|D^-1 b| < tol, D = diag(A) */
-    int32_t pad_;
+    int32_t keep_going;    /* 0: NRITER stops on a fatal code, as the reference's FAFERR does;
+                              1: it continues on 987, 910 and 904 and returns the largest code
+                              seen, the policy of the device library */
 } orc_control;
 
 /* Algorithmic operation counters of one constitutive call. */

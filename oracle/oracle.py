@@ -30,7 +30,7 @@ class OrcControl(C.Structure):
                 ("efix", C.c_double), ("smalp", C.c_double), ("smalf", C.c_double),
                 ("smald", C.c_double), ("smale", C.c_double), ("smalr", C.c_double),
                 ("sor_omega", C.c_double), ("sor_tol", C.c_double),
-                ("sor_kind", C.c_int32), ("pad_", C.c_int32)]
+                ("sor_kind", C.c_int32), ("keep_going", C.c_int32)]
 
 
 class OrcCounters(C.Structure):
@@ -85,6 +85,7 @@ def control_struct(c) -> OrcControl:
     s.maxit = int(c.maxit)
     s.sor_maxit = int(c.sor_maxit)
     s.sor_kind = int(c.sor_kind)
+    s.keep_going = int(getattr(c, "keep_going", 0))
     return s
 
 
@@ -294,7 +295,7 @@ class OracleModel:
         self.m.inc = inc
         self.o.lib.orc_update_node(C.byref(self.m))
         err = self.o.lib.orc_nriter(C.byref(self.m))
-        if err > 500:
+        if err > 500 and not (self.ctl.keep_going and err in (987, 910, 904)):
             return err
         self.o.lib.orc_update_gptv(C.byref(self.m))
         if self.m.tflag != 0:

@@ -55,12 +55,64 @@ def test_fcc_fixture_batch():
     compare(fcc_aluminium(), b)
 
 
-@pytest.mark.skip(reason="18-system kernel returns a wrong tangent entry on elastic points under its current "
-                         "register-spill load; skipped until the register layout is reworked")
+def sensitivity(mat, b, dt, delta=1e-13, seed=0):
+    """Per-point relative change of the oracle's outputs under a relative input
+    perturbation delta of de and st (same iteration path required).  Points whose
+    change exceeds 1e4 * delta amplify rounding by more than 1e4."""
+    rng = np.random.default_rng(seed)
+    o = O64.dstrss_batch([mat], Control(), dt, **b)
+    p = dict(b)
+    p["de"] = b["de"] * (1 + delta * rng.standard_normal(b["de"].shape))
+    p["st"] = b["st"] * (1 + delta * rng.standard_normal(b["st"].shape))
+    q = O64.dstrss_batch([mat], Control(), dt, **p)
+    s = np.zeros(len(b["iel"]))
+    for k in ("t", "chi", "srate", "dsv"):
+        s = np.maximum(s, per_point_rel(q[k], o[k]))
+    s[q["flops"] != o["flops"]] = np.inf
+    return s
+
+
+def compare_conditioned(mat, b, dt=1e-3, min_good=0.98):
+    """1e-10 on every point with rounding amplification below 1e4; the rest within
+    100x their own measured sensitivity."""
+    ctl = Control()
+    g = capi.dstrss_batch(mat, ctl, dt, b["iel"], b["de"], b["r"], b["st"], b["ds"], b["sv"], b["dsv"])
+    o = O64.dstrss_batch([mat], ctl, dt, **b)
+    sens = sensitivity(mat, b, dt)
+    nss = mat.nss
+    same_path = g["flops"] == o["flops"]
+    assert same_path.mean() >= 0.99, same_path.mean()
+    assert np.array_equal(g["err"][same_path], o["err"][same_path])
+    well = same_path & (o["err"] == 0) & (sens <= 1e4 * 1e-13)
+    ill = same_path & (o["err"] == 0) & ~well & np.isfinite(sens)
+    assert well.mean() >= min_good, well.mean()
+    for k in ("t", "ds", "chi", "srate", "dsv"):
+        x, y = g[k], o[k]
+        if k == "srate":
+            x, y = x[:, :nss], y[:, :nss]
+        if k == "dsv":
+            x, y = x[:, :3 + nss], y[:, :3 + nss]
+        e = per_point_rel(x, y)
+        assert e[well].max() <= TOL, (k, e[well].max())
+        if ill.any():
+            assert np.all(e[ill] <= np.maximum(100 * sens[ill], TOL)), (k, e[ill], sens[ill])
+    assert np.array_equal(g["twinsys"], o["twinsys"])
+    assert g["tflag"] == o["tflag"]
+    return g, o
+
+
 def test_hcp_fixture_batch():
     z = golden("dstrss_hcp.npz")
     b = {k[3:]: z[k] for k in z.files if k.startswith("in_")}
-    compare(hcp_magnesium(), b, ok_frac=0.95)
+    compare_conditioned(hcp_magnesium(), b, dt=float(z["dt"]))
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_hcp_fresh_batches_with_twins(seed):
+    mat = hcp_magnesium()
+    b = make_gp_batch(mat, 500, seed=300 + seed, elastic_frac=0.05, stress=30.0)
+    g, o = compare_conditioned(mat, b)
+    assert g["tflag"] == 1 and (g["twinsys"] > 0).any()
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3])

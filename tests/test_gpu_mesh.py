@@ -11,7 +11,7 @@ import pytest
 import scipy.sparse.linalg as spla
 
 from cpfem_code_amd import capi
-from cpfem_code_amd.problem import Control, LT_BRICK_1GP, LT_BRICK_8GP, fcc_plane_strain, fcc_tension
+from cpfem_code_amd.problem import Control, LT_BRICK_1GP, LT_BRICK_8GP, fcc_plane_strain, fcc_tension, hcp_compression
 from gpbatch import mesh_state
 from oracle import Oracle, OracleModel
 
@@ -122,3 +122,32 @@ def test_mixed_precision_increment_close_to_fp64():
     assert rel_max(gb["st"], ga["st"]) < 1e-2
     a.close()
     b.close()
+
+
+def test_hcp_increments_with_twins_match_oracle():
+    """Mg-like HCP (18 systems, 6 tensile twins, twinprob = 1) under compression
+    along x: increments with twin activation and DoReorient against the oracle.
+    Points hit 987 on this load path in the reference algorithm too; both sides
+    report it and go on."""
+    ctl = Control(sor_kind=1, sor_tol=1e-12, sor_omega=1.6, sor_maxit=50000)
+    prob = hcp_compression(3, ninc=3, seed=3, control=ctl)
+    om = OracleModel(O64, prob)
+    om.ctl.keep_going = 1        # continue past 987 as the library does (the reference would stop)
+    sv = capi.Solver(prob)
+    twinned = 0
+    for inc in (1, 2):
+        e_o = om.increment(inc)
+        st = sv.increment(inc)
+        assert st["code"] == e_o
+        g = sv.get_state()
+        assert rel_max(g["x"], om.x) < 1e-7
+        same = g["timestwinned"] == om.timestwinned
+        assert same.mean() >= 0.99
+        twinned = int(om.timestwinned.sum())
+        e = np.abs(g["st"][same] - om.st[same]) / np.abs(om.st).max()
+        assert np.median(e) < 1e-7 and e.max() < 2e-3
+        # orientations (reoriented points included) and slip resistances
+        assert np.abs(g["sv"][same, :3] - om.sv[same, :3]).max() < 1e-5
+        assert rel_max(g["sv"][same, 3:prob.nsv], om.sv[same, 3:prob.nsv]) < 1e-4
+    assert twinned > 0
+    sv.close()
