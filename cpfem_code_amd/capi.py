@@ -178,6 +178,7 @@ def _bind_mesh_api(L):
     L.cpfem_nriter.argtypes = [_CTX, C.c_int32, C.POINTER(CpfemStats)]
     L.cpfem_update_gptv.argtypes = [_CTX]
     L.cpfem_reorient.argtypes = [_CTX, _P]
+    L.cpfem_set_tflag.argtypes = [_CTX, C.c_int32]
     L.cpfem_increment.argtypes = [_CTX, C.c_int32, C.POINTER(CpfemStats)]
     L.cpfem_get_system.argtypes = [_CTX, _P, _P]
     L.cpfem_get_matrix.argtypes = [_CTX, _P, _P, _P, _P]
@@ -315,6 +316,9 @@ class Solver:
 
     def update_gptv(self):
         _check(self.L.cpfem_update_gptv(self.h), "cpfem_update_gptv")
+
+    def set_tflag(self, tflag):
+        _check(self.L.cpfem_set_tflag(self.h, int(tflag)), "cpfem_set_tflag")
 
     def reorient(self):
         n = np.zeros(1, np.int32)

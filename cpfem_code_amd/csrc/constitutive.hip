@@ -70,8 +70,10 @@ __device__ __forceinline__ void plastc_c(const DevMaterial<N>& M, Compact<AMAX>&
         for (int l = 0; l < AMAX; ++l) {
             double v = (k == l) ? 1.0 : 0.0;
             if (k < nsa && l < nsa) {
+                // gam/|gam| of ela-plas.f90:186-197 is exactly +-1: copysign gives the same value
                 const double ag = fabs(S.gam[l]);
-                v = (ag > 1.e-14) ? ((S.gam[l] / ag) * (latent(M, S.isa[k], S.isa[l]) * th[l])) / S.s1[k] : 0.0;
+                v = (ag > 1.e-14) ? (copysign(1.0, S.gam[l]) * (latent(M, S.isa[k], S.isa[l]) * th[l])) / S.s1[k]
+                                  : 0.0;
             }
             S.ah[k][l] = v;
         }
@@ -114,7 +116,8 @@ __device__ __forceinline__ void scale_it_c(const DevMaterial<N>& M, Compact<AMAX
 // SLPFTA on the active set; leaves ah in X form and gpre = slips of the last PLASTC.
 template <int N, int AMAX>
 __device__ __forceinline__ void slpfta_c(const DevMaterial<N>& M, const DevControl& C, Compact<AMAX>& S, double dt,
-                                         double* gpre, int& err, int64_t& flops, int64_t& transc) {
+                                         double* gpre, size_t gstride, int& err, int64_t& flops,
+                                         int64_t& transc) {
     const int nsa = S.nsa;
     double th[AMAX];
     int it = 0;
@@ -165,7 +168,7 @@ __device__ __forceinline__ void slpfta_c(const DevMaterial<N>& M, const DevContr
     if (tauerr > 0.5) err = err > 987 ? err : 987;
     plastc_c<N, AMAX>(M, S, dt, th, flops, transc);
 #pragma unroll
-    for (int k = 0; k < AMAX; ++k) gpre[k * 64] = S.gam[k];
+    for (int k = 0; k < AMAX; ++k) gpre[k * gstride] = S.gam[k];
 #pragma unroll
     for (int i = 0; i < AMAX; ++i) {
         if (i < nsa) {
@@ -184,10 +187,10 @@ __device__ __forceinline__ void slpfta_c(const DevMaterial<N>& M, const DevContr
 template <int N, int AMAX>
 __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__ Mp, const DevControl C,
                                               const double dt, const ConstArgs A) {
-    __shared__ double sh_el[21][64];
+    // LDS: the Schmid columns of the active systems (also the staging area of
+    // the active-set compaction) and the active system numbers, interleaved by
+    // lane; the compliance and the last slips live in per-point work arrays
     __shared__ double sh_b[6 * AMAX][64];
-    __shared__ double sh_t[6][64];
-    __shared__ double sh_gp[AMAX][64];
     __shared__ int sh_isa[AMAX][64];
     const int lane = threadIdx.x;
     const int idx = blockIdx.x * 64 + lane;
@@ -205,10 +208,9 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
     if (live) {
         const DevMaterial<N>& M = *Mp;
         const size_t n = (size_t)A.n;
-        double* el = &sh_el[0][lane];
+        double* el = A.el + g;
         double* bc = &sh_b[0][lane];
-        double* tl = &sh_t[0][lane];
-        double* gpre = &sh_gp[0][lane];
+        double* gpre = A.gp + g;
         double de[6], r[3], st[6], t[6], s[6];
         int kerr = 0, iel;
         double deff_kin = 0.0;
@@ -250,10 +252,8 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
         double ph[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) ph[i] = A.sv[i * n + g] + A.dsv[i * n + g];
-        elastc<N>(M, ph[0], ph[1], ph[2], el, 64);
+        elastc<N>(M, ph[0], ph[1], ph[2], el, (int)n);
         tens_rot(r, st, t);
-#pragma unroll
-        for (int i = 0; i < 6; ++i) tl[i * 64] = t[i];
         int64_t flops = 0, transc = 0;
         int iter = 0;
         bool elastic = false;
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
 #pragma unroll
             for (int i = 0; i < 6; ++i)
 #pragma unroll
-                for (int j = 0; j < 6; ++j) lu[i][j] = el[sym21(i, j) * 64];
+                for (int j = 0; j < 6; ++j) lu[i][j] = el[sym21(i, j) * n];
             (void)factor<6>(lu);
 #pragma unroll
             for (int i = 0; i < 6; ++i) {
@@ -326,10 +326,14 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
             double erepmd = 0.0;
             bool done = false, ovf = false;
             while (!done && iter < A.iter_limit) {
-                // ---- SLPINI (crys-cons.f90:143-197) over all systems
-                double gfull[N], tfull[N];
+                // ---- SLPINI (crys-cons.f90:143-197) over all systems.  The
+                // loop stays rolled (one power per system) and stages each
+                // system's slip and resolved shear in the LDS slice of the
+                // Schmid columns, slots 2i and 2i+1; the active ones are then
+                // compacted in place (write slot <= read slot) to 2k, 2k+1.
+                static_assert(6 * AMAX >= 2 * N, "staging needs 2N slots");
                 double gammax = 0.0;
-#pragma unroll
+#pragma unroll 1
                 for (int i = 0; i < N; ++i) {
                     double sdr[3], spn[3], b[6];
                     slip_vectors(M, a, i, sdr, spn);
@@ -337,7 +341,6 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
                     double sum = 0.0;
 #pragma unroll
                     for (int k = 0; k < 6; ++k) sum = sum + b[k] * s[k];
-                    tfull[i] = sum;
                     const double s0i = A.sv[(3 + i) * n + g];
                     double gi;
                     if (M.any_twin && fabs(M.ssang[i]) > 1.5) {
@@ -353,18 +356,20 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
                     } else {
                         gi = dt * spower((sum / s0i), (1.0 / M.am[i]));
                     }
-                    gfull[i] = gi;
+                    bc[(2 * i + 0) * 64] = gi;
+                    bc[(2 * i + 1) * 64] = sum;
                     if (fabs(gi) > gammax) gammax = fabs(gi);
                 }
                 const double cutoff = 1.e-6 * gammax;
                 int nsa = 0;
-#pragma unroll
+#pragma unroll 1
                 for (int i = 0; i < N; ++i) {
-                    if (!(fabs(gfull[i]) < cutoff)) {
+                    const double gi = bc[(2 * i + 0) * 64];
+                    if (!(fabs(gi) < cutoff)) {
                         if (nsa < AMAX) {
-                            // stage in the b slice, read back at fixed slots below
-                            bc[(4 * nsa + 0) * 64] = gfull[i];
-                            bc[(4 * nsa + 1) * 64] = tfull[i];
+                            const double ti = bc[(2 * i + 1) * 64];
+                            bc[(2 * nsa + 0) * 64] = gi;
+                            bc[(2 * nsa + 1) * 64] = ti;
                             sh_isa[nsa][lane] = i;
                         }
                         ++nsa;
@@ -381,8 +386,8 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
                     if (k < nsa) {
                         const int i = sh_isa[k][lane];
                         S.isa[k] = i;
-                        S.gam[k] = bc[(4 * k + 0) * 64];
-                        S.tau[k] = bc[(4 * k + 1) * 64];
+                        S.gam[k] = bc[(2 * k + 0) * 64];
+                        S.tau[k] = bc[(2 * k + 1) * 64];
                         S.s0[k] = A.sv[(3 + i) * n + g];
                     } else {
                         S.isa[k] = 0;
@@ -394,18 +399,16 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
                 flops += 5 * (int64_t)N;
                 transc += 2 * (int64_t)N;
                 // Schmid columns of the active systems (BTMATRX, crys_math.f90:75-115)
+#pragma unroll 1
+                for (int k = 0; k < nsa; ++k) {
+                    double sdr[3], spn[3], b[6];
+                    slip_vectors(M, a, sh_isa[k][lane], sdr, spn);
+                    schmid(sdr, spn, b);
 #pragma unroll
-                for (int k = 0; k < AMAX; ++k) {
-                    if (k < nsa) {
-                        double sdr[3], spn[3], b[6];
-                        slip_vectors(M, a, S.isa[k], sdr, spn);
-                        schmid(sdr, spn, b);
-#pragma unroll
-                        for (int q = 0; q < 6; ++q) bc[(6 * k + q) * 64] = b[q];
-                    }
+                    for (int q = 0; q < 6; ++q) bc[(6 * k + q) * 64] = b[q];
                 }
                 scale_it_c<N, AMAX>(M, S, s, gamcap, C.afactor, flops, transc);
-                slpfta_c<N, AMAX>(M, C, S, dt, gpre, err, flops, transc);
+                slpfta_c<N, AMAX>(M, C, S, dt, gpre, n, err, flops, transc);
                 scale_it_c<N, AMAX>(M, S, s, gamcap, C.afactor, flops, transc);
                 // plastic strain increment and strain-increment error (Cpstrs.f90:96-98)
 #pragma unroll
@@ -420,12 +423,12 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
                 {
                     double sm[6];
 #pragma unroll
-                    for (int k = 0; k < 6; ++k) sm[k] = s[k] - tl[k * 64];
+                    for (int k = 0; k < 6; ++k) sm[k] = s[k] - t[k];
 #pragma unroll
                     for (int i = 0; i < 6; ++i) {
                         double sum = 0.0;
 #pragma unroll
-                        for (int k = 0; k < 6; ++k) sum = sum + el[sym21(i, k) * 64] * sm[k];
+                        for (int k = 0; k < 6; ++k) sum = sum + el[sym21(i, k) * n] * sm[k];
                         erreps[i] = A.de[i * n + g] - epl[i] - sum;
                     }
                 }
@@ -443,7 +446,7 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
 #pragma unroll
                         for (int k = 0; k < AMAX; ++k)
                             if (k < nsa) sum = sum + bc[(6 * k + i) * 64] * w[k];
-                        djac[i][j] = sum + el[sym21(i, j) * 64];
+                        djac[i][j] = sum + el[sym21(i, j) * n];
                     }
                 }
                 if (factor<6>(djac)) err = err > 910 ? err : 910;
@@ -501,7 +504,7 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
                     double sumss = 0.0;
 #pragma unroll
                     for (int k = 0; k < AMAX; ++k)
-                        if (k < nsa) sumss = sumss + fabs(gpre[k * 64]);
+                        if (k < nsa) sumss = sumss + fabs(gpre[k * n]);
                     const double rate = sumss / dt;
 #pragma unroll
                     for (int k = 0; k < AMAX; ++k) {
@@ -520,7 +523,7 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
                         double s1 = s0i;
 #pragma unroll
                         for (int l = 0; l < AMAX; ++l)
-                            if (l < nsa) s1 = s1 + (latent(M, i, S.isa[l]) * th[l]) * fabs(gpre[l * 64]);
+                            if (l < nsa) s1 = s1 + (latent(M, i, S.isa[l]) * th[l]) * fabs(gpre[l * n]);
                         A.dsv[(3 + i) * n + g] = s1 - s0i;
                     }
                 }
@@ -684,6 +687,8 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
     A.ck_s = S.ck_s;
     A.ck_iter = S.ck_iter;
     A.ck_kerr = S.ck_kerr;
+    A.el = S.el;
+    A.gp = S.gp;
     // tier 1, pass 1: every point, at most kFirstPassIters outer iterations
     A.in_list = nullptr;
     A.in_count = nullptr;
@@ -756,13 +761,15 @@ hipError_t alloc_schedule(ConstSchedule& S, int n) {
     if ((e = hipMalloc((void**)&S.list_a, sizeof(int) * (size_t)n)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&S.list_b, sizeof(int) * (size_t)n)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&S.list_ovf, sizeof(int) * (size_t)n)) != hipSuccess) return e;
+    if ((e = hipMalloc((void**)&S.el, sizeof(double) * 21 * (size_t)n)) != hipSuccess) return e;
+    if ((e = hipMalloc((void**)&S.gp, sizeof(double) * kConstMaxWidth * (size_t)n)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&S.counts, sizeof(int) * 4)) != hipSuccess) return e;
     if ((e = hipHostMalloc((void**)&S.host_counts, sizeof(int) * 4, 0)) != hipSuccess) return e;
     return hipSuccess;
 }
 
 void free_schedule(ConstSchedule& S) {
-    void* p[] = {S.ck_s, S.ck_iter, S.ck_kerr, S.list_a, S.list_b, S.list_ovf, S.counts};
+    void* p[] = {S.ck_s, S.ck_iter, S.ck_kerr, S.list_a, S.list_b, S.list_ovf, S.el, S.gp, S.counts};
     for (void* q : p)
         if (q) (void)hipFree(q);
     if (S.host_counts) (void)hipHostFree(S.host_counts);

@@ -300,6 +300,8 @@ struct ConstArgs {
     int* errmax;               // scalar
     uint64_t rng_epoch;
     // schedule
+    double* el;                // [21][n] compliance in specimen axes (work array)
+    double* gp;                // [width][n] slips of the last hardening update (work array)
     double* ck_s;              // [6][n] stress at the next outer iteration
     int* ck_iter;              // [n] outer iterations done
     int* ck_kerr;              // [n] error code of the kinematics / earlier iterations

@@ -8,7 +8,11 @@
 
 namespace cpfem {
 
-// Device scratch of the schedule: checkpoints and the point lists.
+// Widest active set of the compiled kernels.
+constexpr int kConstMaxWidth = 12;
+
+// Device scratch of the schedule: checkpoints, per-point work arrays and the
+// point lists.
 struct ConstSchedule {
     double* ck_s = nullptr;    // [6][n]
     int* ck_iter = nullptr;    // [n]
@@ -16,6 +20,8 @@ struct ConstSchedule {
     int* list_a = nullptr;     // [n] unfinished after the first pass
     int* list_b = nullptr;     // [n] spare
     int* list_ovf = nullptr;   // [n] active set wider than the first kernel
+    double* el = nullptr;      // [21][n] compliance in specimen axes of the current pass
+    double* gp = nullptr;      // [kConstMaxWidth][n] slips of the last hardening update
     int* counts = nullptr;     // [4] device
     int* host_counts = nullptr;// [4] pinned
 };

@@ -1443,6 +1443,37 @@ extern "C" int cpfem_update_gptv(cpfem_ctx* c) {
     return 0;
 }
 
+// Twin tables of material 1 (DoReorient uses material 1 only,
+// Fasolt3.f90:103,124-125), staged on the device at the first reorientation.
+static int twin_tables(Ctx* c) {
+    if (c->twin_tab) return 0;
+    static_assert(sizeof(TwinTables) == 60 * 8 * 7, "layout");
+    TwinTables h;
+    memset(&h, 0, sizeof(h));
+    std::vector<char> hm(material_bytes(c->nss));
+    CPFEM_HIP(hipMemcpy(hm.data(), c->dmat, hm.size(), hipMemcpyDeviceToHost));
+    auto copy = [&h](const auto* M, int nss) {
+        for (int i = 0; i < nss; ++i) {
+            h.ssang[i] = M->ssang[i];
+            for (int k = 0; k < 3; ++k) {
+                h.ab[i][k] = M->ab[i][k];
+                h.an[i][k] = M->an[i][k];
+            }
+        }
+    };
+    if (c->nss == 12)
+        copy(reinterpret_cast<DevMaterial<12>*>(hm.data()), 12);
+    else
+        copy(reinterpret_cast<DevMaterial<18>*>(hm.data()), 18);
+    TwinTables* dT = nullptr;
+    int* dcount = nullptr;
+    if (dalloc(c, &dT, 1) || dalloc(c, &dcount, 1)) return CPFEM_E_HIP;
+    CPFEM_HIP(hipMemcpy(dT, &h, sizeof(h), hipMemcpyHostToDevice));
+    c->twin_tab = dT;
+    c->twin_count = dcount;
+    return 0;
+}
+
 extern "C" int cpfem_reorient(cpfem_ctx* c, int32_t* n) {
     if (!c) return CPFEM_E_ARG;
     int fl = 0;
@@ -1450,49 +1481,27 @@ extern "C" int cpfem_reorient(cpfem_ctx* c, int32_t* n) {
     CPFEM_HIP(hipMemcpy(&fl, c->flags, sizeof(int), hipMemcpyDeviceToHost));
     int cnt = 0;
     if (fl) {
-        // twin tables are only needed here; stage them once per call
-        static_assert(sizeof(TwinTables) == 60 * 8 * 7, "layout");
-        TwinTables* dT = nullptr;
-        int* dcount = nullptr;
-        CPFEM_HIP(hipMalloc((void**)&dT, sizeof(TwinTables)));
-        CPFEM_HIP(hipMalloc((void**)&dcount, sizeof(int)));
-        TwinTables h;
-        memset(&h, 0, sizeof(h));
-        // DevMaterial holds them in the same order: copy back from the device copy
-        {
-            std::vector<char> hm(material_bytes(c->nss));
-            CPFEM_HIP(hipMemcpy(hm.data(), c->dmat, hm.size(), hipMemcpyDeviceToHost));
-            if (c->nss == 12) {
-                auto* M = reinterpret_cast<DevMaterial<12>*>(hm.data());
-                for (int i = 0; i < 12; ++i) {
-                    h.ssang[i] = M->ssang[i];
-                    for (int k = 0; k < 3; ++k) {
-                        h.ab[i][k] = M->ab[i][k];
-                        h.an[i][k] = M->an[i][k];
-                    }
-                }
-            } else {
-                auto* M = reinterpret_cast<DevMaterial<18>*>(hm.data());
-                for (int i = 0; i < 18; ++i) {
-                    h.ssang[i] = M->ssang[i];
-                    for (int k = 0; k < 3; ++k) {
-                        h.ab[i][k] = M->ab[i][k];
-                        h.an[i][k] = M->an[i][k];
-                    }
-                }
-            }
-        }
-        CPFEM_HIP(hipMemcpy(dT, &h, sizeof(h), hipMemcpyHostToDevice));
-        CPFEM_HIP(hipMemset(dcount, 0, sizeof(int)));
+        const int rc = twin_tables(c);
+        if (rc) return rc;
+        int* dcount = (int*)c->twin_count;
+        hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, c->stream, dcount, (size_t)1, 0);
         hipLaunchKernelGGL(k_reorient, dim3(nblk(c->ngps)), dim3(kBlock), 0, c->stream, c->ngps, c->sv, c->twinsys,
-                           c->timestwinned, (const TwinTables*)dT, dcount);
+                           c->timestwinned, (const TwinTables*)c->twin_tab, dcount);
+        hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, c->stream, c->flags, (size_t)1, 0);
+        CPFEM_HIP(hipGetLastError());
+        CPFEM_HIP(hipMemcpyAsync(c->red.host + 6, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         CPFEM_HIP(hipStreamSynchronize(c->stream));
-        CPFEM_HIP(hipMemcpy(&cnt, dcount, sizeof(int), hipMemcpyDeviceToHost));
-        (void)hipFree(dT);
-        (void)hipFree(dcount);
-        CPFEM_HIP(hipMemset(c->flags, 0, sizeof(int)));
+        memcpy(&cnt, c->red.host + 6, sizeof(int));
     }
     if (n) *n = cnt;
+    return 0;
+}
+
+extern "C" int cpfem_set_tflag(cpfem_ctx* c, int32_t tflag) {
+    if (!c) return CPFEM_E_ARG;
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, c->stream, c->flags, (size_t)1, tflag ? 1 : 0);
+    CPFEM_HIP(hipGetLastError());
+    CPFEM_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
 

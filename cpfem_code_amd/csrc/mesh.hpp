@@ -100,6 +100,8 @@ struct Ctx {
     long long *flops = nullptr, *transc = nullptr;
     int* flags = nullptr;        // [0] tflag, [1] errmax
     uint64_t rng_epoch = 0;
+    void* twin_tab = nullptr;    // TwinTables of material 1 (first reorientation)
+    void* twin_count = nullptr;  // [1] reoriented points
     Matrix A;
     Reducer red;
     ConstSchedule sched;

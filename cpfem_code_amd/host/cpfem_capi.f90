@@ -126,6 +126,12 @@ MODULE CPFEM_CAPI
             INTEGER(C_INT32_T), INTENT(OUT) :: n
         END FUNCTION
 
+        INTEGER(C_INT) FUNCTION cpfem_set_tflag(ctx, tflag) BIND(C, name='cpfem_set_tflag')
+            IMPORT :: C_INT, C_INT32_T, C_PTR
+            TYPE(C_PTR), VALUE :: ctx
+            INTEGER(C_INT32_T), VALUE :: tflag
+        END FUNCTION
+
         INTEGER(C_INT) FUNCTION cpfem_cumslip(ctx, inc) BIND(C, name='cpfem_cumslip')
             IMPORT :: C_INT, C_INT32_T, C_PTR
             TYPE(C_PTR), VALUE :: ctx

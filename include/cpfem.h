@@ -166,8 +166,13 @@ int cpfem_cumslip(cpfem_ctx *ctx, int32_t inc);
 int cpfem_power(cpfem_ctx *ctx, double *fdx);
 /* UPDATE_GPTV (Fasolt3.f90:254): commit the point increments. */
 int cpfem_update_gptv(cpfem_ctx *ctx);
-/* DoReorient (Fasolt3.f90:79) if the twin flag is set; returns the count in *n. */
+/* DoReorient (Fasolt3.f90:79) if the twin flag is set, then tflag = 0
+ * (Fasolt3.f90:43-46); returns the count in *n. */
 int cpfem_reorient(cpfem_ctx *ctx, int32_t *n);
+/* Set the twin flag, the module variable tflag (fas-com-mod.f90:17) that SLPINI
+ * raises (crys-cons.f90:171), RDDATA clears (Faslt_io_nowin.f90:133) and the
+ * driver tests before DoReorient (Fasolt3.f90:43). */
+int cpfem_set_tflag(cpfem_ctx *ctx, int32_t tflag);
 /* One full increment: UPDATE_NODE, NRITER, UPDATE_GPTV, DoReorient (Fasolt3.f90:38-46). */
 int cpfem_increment(cpfem_ctx *ctx, int32_t inc, cpfem_stats *stats);
 /* Right-hand side and solution of the last solve (3*nnod each, may be NULL). */

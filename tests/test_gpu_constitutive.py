@@ -83,9 +83,10 @@ def compare_conditioned(mat, b, dt=1e-3, min_good=0.98):
     same_path = g["flops"] == o["flops"]
     assert same_path.mean() >= 0.99, same_path.mean()
     assert np.array_equal(g["err"][same_path], o["err"][same_path])
-    well = same_path & (o["err"] == 0) & (sens <= 1e4 * 1e-13)
-    ill = same_path & (o["err"] == 0) & ~well & np.isfinite(sens)
-    assert well.mean() >= min_good, well.mean()
+    conv = same_path & (o["err"] == 0)
+    well = conv & (sens <= 1e4 * 1e-13)
+    ill = conv & ~well & np.isfinite(sens)
+    assert well.sum() >= min_good * conv.sum(), (well.sum(), conv.sum())
     for k in ("t", "ds", "chi", "srate", "dsv"):
         x, y = g[k], o[k]
         if k == "srate":
