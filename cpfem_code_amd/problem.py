@@ -35,6 +35,14 @@ NTAB = 99    # x_spec(1:15, 1:99, inc)
 # 8-node brick: 1 point (reference SHAPE_12) or 2x2x2 points (new element type)
 LT_BRICK_1GP = 12
 LT_BRICK_8GP = 13
+LT_BRICK_20 = 22     # 20-node brick, 2x2x2 points (SHAPE_22, Shapef.f90:210)
+
+# local coordinates of the 20 nodes of SHAPE_22 (Shapef.f90:191-200): bottom
+# corners 1,3,5,7 and mid-sides 2,4,6,8 anticlockwise, 9-12 halfway up the
+# vertical edges, top face 13-20 like the bottom
+BRICK20 = [(-1, -1, -1), (0, -1, -1), (1, -1, -1), (1, 0, -1), (1, 1, -1), (0, 1, -1), (-1, 1, -1), (-1, 0, -1),
+           (-1, -1, 0), (1, -1, 0), (1, 1, 0), (-1, 1, 0),
+           (-1, -1, 1), (0, -1, 1), (1, -1, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1), (-1, 1, 1), (-1, 0, 1)]
 
 
 def f32(a):
@@ -256,7 +264,8 @@ def brick_problem(n, length=1.0, *, lt=LT_BRICK_8GP, material: Optional[Material
                   faces: Optional[Dict[str, object]] = None, eps_inc=(0.0, 0.0, 1.0e-3),
                   ninc=50, seed=0, texture="random", control: Optional[Control] = None,
                   elastic_elements: Optional[np.ndarray] = None) -> Problem:
-    """Structured n^3 (or (nx,ny,nz)) brick of 8-node hexahedra.
+    """Structured n^3 (or (nx,ny,nz)) brick of hexahedra: 8-node (lt 12, 13) or
+    20-node (lt 22, corner and mid-edge nodes of a doubled grid).
 
     ``faces`` maps 'xmin','xmax','ymin','ymax','zmin','zmax' to 'fixed' (zero
     normal displacement), 'spec' (normal displacement increment
@@ -270,17 +279,6 @@ def brick_problem(n, length=1.0, *, lt=LT_BRICK_8GP, material: Optional[Material
     faces = faces or dict(xmin="fixed", ymin="fixed", zmin="fixed", zmax="spec")
     rng = np.random.default_rng(seed)
 
-    nnx, nny, nnz = nx + 1, ny + 1, nz + 1
-    nnod = nnx * nny * nnz
-    ii, jj, kk = np.meshgrid(np.arange(nnx), np.arange(nny), np.arange(nnz), indexing="ij")
-    # node number = i + nnx*(j + nny*k), x fastest
-    coords = np.zeros((nnod, 3))
-    idx = (ii + nnx * (jj + nny * kk)).ravel()
-    coords[idx, 0] = (ii.ravel() * lx / nx)
-    coords[idx, 1] = (jj.ravel() * ly / ny)
-    coords[idx, 2] = (kk.ravel() * lz / nz)
-    coords = f32(coords)
-
     nel = nx * ny * nz
     ei, ej, ek = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
     # element number x fastest
@@ -288,9 +286,42 @@ def brick_problem(n, length=1.0, *, lt=LT_BRICK_8GP, material: Optional[Material
     base = np.zeros((nel, 3), int)
     base[eidx] = np.stack([ei.ravel(), ej.ravel(), ek.ravel()], 1)
     ln = np.zeros((nel, 20), np.int32)
-    for a, (di, dj, dk) in enumerate(BRICK):
-        ln[:, a] = 1 + (base[:, 0] + di) + nnx * ((base[:, 1] + dj) + nny * (base[:, 2] + dk))
-    npe = 8 if lt == LT_BRICK_8GP else 1
+    if lt == LT_BRICK_20:
+        # grid of half cells; nodes are the points with at most one odd index
+        gx, gy, gz = 2 * nx + 1, 2 * ny + 1, 2 * nz + 1
+        I, J, K = np.meshgrid(np.arange(gx), np.arange(gy), np.arange(gz), indexing="ij")
+        gidx = (I + gx * (J + gy * K)).ravel()
+        keep = ((I % 2) + (J % 2) + (K % 2)).ravel() <= 1
+        num = np.full(gx * gy * gz, -1, np.int64)
+        order = np.sort(gidx[keep])                  # x fastest over the kept points
+        num[order] = np.arange(len(order))
+        nnod = len(order)
+        coords = np.zeros((nnod, 3))
+        oi, oj, ok = order % gx, (order // gx) % gy, order // (gx * gy)
+        coords[:, 0] = oi * lx / (2 * nx)
+        coords[:, 1] = oj * ly / (2 * ny)
+        coords[:, 2] = ok * lz / (2 * nz)
+        for a, (ci, cj, ck) in enumerate(BRICK20):
+            gi = 2 * base[:, 0] + 1 + ci
+            gj = 2 * base[:, 1] + 1 + cj
+            gk = 2 * base[:, 2] + 1 + ck
+            ln[:, a] = 1 + num[gi + gx * (gj + gy * gk)]
+        nd_el = 20
+    else:
+        nnx, nny, nnz = nx + 1, ny + 1, nz + 1
+        nnod = nnx * nny * nnz
+        ii, jj, kk = np.meshgrid(np.arange(nnx), np.arange(nny), np.arange(nnz), indexing="ij")
+        # node number = i + nnx*(j + nny*k), x fastest
+        coords = np.zeros((nnod, 3))
+        idx = (ii + nnx * (jj + nny * kk)).ravel()
+        coords[idx, 0] = (ii.ravel() * lx / nx)
+        coords[idx, 1] = (jj.ravel() * ly / ny)
+        coords[idx, 2] = (kk.ravel() * lz / nz)
+        for a, (di, dj, dk) in enumerate(BRICK):
+            ln[:, a] = 1 + (base[:, 0] + di) + nnx * ((base[:, 1] + dj) + nny * (base[:, 2] + dk))
+        nd_el = 8
+    coords = f32(coords)
+    npe = 1 if lt == LT_BRICK_1GP else 8
     ngps = nel * npe
     lg = np.zeros((nel, 8), np.int32)
     lg[:, :npe] = 1 + np.arange(ngps, dtype=np.int32).reshape(nel, npe)
@@ -357,7 +388,7 @@ def brick_problem(n, length=1.0, *, lt=LT_BRICK_8GP, material: Optional[Material
                 x_spec[inc, t - 1, pat[ax]] = 1.0
                 x_spec[inc, t - 1, disp[ax]] = dcur[ax] if kind == "spec" else 0.0
     prob = Problem(nlmnt=nel, nnod=nnod, ngps=ngps, nsv=nsv,
-                   le=le, lt=np.full(nel, lt, np.int32), nd=np.full(nel, 8, np.int32),
+                   le=le, lt=np.full(nel, lt, np.int32), nd=np.full(nel, nd_el, np.int32),
                    np_=np.full(nel, npe, np.int32), ln=ln, lg=lg, ib=ib,
                    x=coords.reshape(-1).copy(), dx=np.zeros(3 * nnod),
                    sv=sv, tiem=f32(tiem), x_spec=f32(x_spec), f_spec=f32(f_spec),

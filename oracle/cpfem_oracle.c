@@ -811,6 +811,105 @@ static void shape8(int lt, int ig, const real *xnode, real *sf, real *sfd, real 
     *weight = (lt == 12) ? R(8.0) * det : det;
 }
 
+/* Local coordinates of the 20 nodes of SHAPE_22 (Shapef.f90:191-200): bottom
+ * face corners 1,3,5,7 and mid-sides 2,4,6,8 anticlockwise seen from the top,
+ * nodes 9-12 halfway up the vertical edges above 1,3,5,7, top face 13-20 like
+ * the bottom. */
+static const int brick20_loc[20][3] = {
+    {-1, -1, -1}, {0, -1, -1}, {1, -1, -1}, {1, 0, -1}, {1, 1, -1}, {0, 1, -1}, {-1, 1, -1}, {-1, 0, -1},
+    {-1, -1, 0},  {1, -1, 0},  {1, 1, 0},   {-1, 1, 0},
+    {-1, -1, 1},  {0, -1, 1},  {1, -1, 1},  {1, 0, 1},  {1, 1, 1},  {0, 1, 1},  {-1, 1, 1},  {-1, 0, 1}};
+
+/* SHAPE_22 tabulates the shape functions and their local derivatives at the
+ * 2x2x2 points to 6 decimals (Shapef.f90:227-308).  The tables are the
+ * quadratic serendipity functions evaluated at +-g, g = 1/sqrt(3) held in
+ * single precision, each value rounded to 6 decimals: that rule reproduces
+ * every tabulated entry, so the oracle generates them instead of storing
+ * them.  Points in the brick_sign order. */
+static int tab22_ready = 0;
+static double tab22_n[8][20], tab22_d[8][3][20];
+
+static double round6(double v) { return round(v * 1.0e6) / 1.0e6; }
+
+static void brick20_tables(void) {
+    if (tab22_ready) return;
+    const double g = (double)(float)(1.0 / sqrt(3.0));
+    for (int ig = 0; ig < 8; ++ig) {
+        double xi[3];
+        for (int k = 0; k < 3; ++k) xi[k] = brick_sign[ig][k] * g;
+        for (int a = 0; a < 20; ++a) {
+            const int *c = brick20_loc[a];
+            double n, d[3];
+            if (c[0] != 0 && c[1] != 0 && c[2] != 0) {
+                /* corner: (1+xi c)(1+eta c)(1+zeta c)(xi c + eta c + zeta c - 2)/8 */
+                double f[3], sum = -2.0;
+                for (int k = 0; k < 3; ++k) {
+                    f[k] = 1.0 + xi[k] * c[k];
+                    sum += xi[k] * c[k];
+                }
+                n = f[0] * f[1] * f[2] * sum / 8.0;
+                for (int k = 0; k < 3; ++k) {
+                    const double other = f[(k + 1) % 3] * f[(k + 2) % 3];
+                    d[k] = c[k] * other * sum / 8.0 + f[0] * f[1] * f[2] * c[k] / 8.0;
+                }
+            } else {
+                /* mid-side node with c[z] = 0: (1-xi_z^2)(1+xi_p c_p)(1+xi_q c_q)/4 */
+                int z = 0;
+                while (c[z] != 0) ++z;
+                const int p = (z + 1) % 3, q = (z + 2) % 3;
+                const double qz = 1.0 - xi[z] * xi[z], fp = 1.0 + xi[p] * c[p], fq = 1.0 + xi[q] * c[q];
+                n = qz * fp * fq / 4.0;
+                d[z] = -2.0 * xi[z] * fp * fq / 4.0;
+                d[p] = qz * c[p] * fq / 4.0;
+                d[q] = qz * fp * c[q] / 4.0;
+            }
+            tab22_n[ig][a] = round6(n);
+            for (int k = 0; k < 3; ++k) tab22_d[ig][k][a] = round6(d[k]);
+        }
+    }
+    tab22_ready = 1;
+}
+
+/* SHAPE_22, Shapef.f90:210-342: 20-node brick, 2x2x2 points, weight detJ. */
+static void shape20(int ig, const real *xnode, real *sf, real *sfd, real *weight) {
+    brick20_tables();
+    real dndn[20][3];
+    for (int a = 0; a < 20; ++a) {
+        sf[a] = R(tab22_n[ig][a]);
+        for (int j = 0; j < 3; ++j) dndn[a][j] = R(tab22_d[ig][j][a]);
+    }
+    real jac[9]; /* jac(j,k) = sum_a dndn(j,a) x(k,a), [j + 3k] */
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 3; ++j) {
+            real sum = R(0.0);
+            for (int a = 0; a < 20; ++a) sum = sum + dndn[a][j] * xnode[3 * a + k];
+            jac[j + 3 * k] = sum;
+        }
+    real det;
+    invert_3(jac, &det);
+    for (int a = 0; a < 20; ++a)
+        for (int k = 0; k < 3; ++k) {
+            real sum = R(0.0);
+            for (int j = 0; j < 3; ++j) sum = sum + jac[k + 3 * j] * dndn[a][j];
+            sfd[3 * a + k] = sum;
+        }
+    *weight = R(1.0) * det;
+}
+
+/* Shape functions of element type lt at point ig (0-based); returns the
+ * node count, 0 for a type this restatement does not cover. */
+static int shape_el(int lt, int ig, const real *xnode, real *sf, real *sfd, real *weight) {
+    if (lt == 12 || lt == 13) {
+        shape8(lt, ig, xnode, sf, sfd, weight);
+        return 8;
+    }
+    if (lt == 22) {
+        shape20(ig, xnode, sf, sfd, weight);
+        return 20;
+    }
+    return 0;
+}
+
 /* DSTRNS, Sig_eps.f90:25-61: strain / rotation increments at a point. */
 static int dstrns(int npl, const real *sfd, const real *dxl, real smale, real smalr, real *del, real *rl,
                   real *deff) {
@@ -916,8 +1015,11 @@ int orc_elvals(orc_model *m) {
             }
         } else {
             for (int ig = 0; ig < ng; ++ig) {
-                real sf[8], sfd[24], weight;
-                shape8(itp, ig, xl, sf, sfd, &weight);
+                real sf[20], sfd[60], weight;
+                if (shape_el(itp, ig, xl, sf, sfd, &weight) != nn) {
+                    free(wm);
+                    return -2; /* element type not covered */
+                }
                 if (weight < small) note_err(&worst, 920);
                 int j = m->lg[8 * el + ig] - 1;
                 real epl[6], stl[6], dsl[6], del[6], r[3], s[6], d[6][6], defl;
@@ -1387,11 +1489,12 @@ int orc_invert_3(double *mm, double *det) {
     return d != R(0.0);
 }
 void orc_shape(int lt, int ig, const double *xnode, double *sf, double *sfd, double *weight) {
-    real x[24], f[8], d[24], w;
-    for (int i = 0; i < 24; ++i) x[i] = R(xnode[i]);
-    shape8(lt, ig, x, f, d, &w);
-    for (int i = 0; i < 8; ++i) sf[i] = f[i];
-    for (int i = 0; i < 24; ++i) sfd[i] = d[i];
+    const int nn = (lt == 22) ? 20 : 8;
+    real x[60], f[20], d[60], w = R(0.0);
+    for (int i = 0; i < 3 * nn; ++i) x[i] = R(xnode[i]);
+    shape_el(lt, ig, x, f, d, &w);
+    for (int i = 0; i < nn; ++i) sf[i] = f[i];
+    for (int i = 0; i < 3 * nn; ++i) sfd[i] = d[i];
     *weight = w;
 }
 int orc_dstrns(int npl, const double *sfd, const double *dxl, double smale, double smalr, double *del,

@@ -116,6 +116,8 @@ void orc_ctmatrx(double ph1, double phi, double ph2, double *a, double *aug);
 void orc_elastc(const orc_material *m, double ph1, double phi, double ph2, double *el_comp);
 void orc_tens_rot(const double *r, const double *a, double *b);
 int orc_invert_3(double *m, double *det);
+/* Shape functions of element type lt at point ig (0-based): lt 12, 13 take 8
+ * nodes, lt 22 (SHAPE_22) 20; xnode[3*a+k], sfd[3*a+k] = dN_a/dx_k. */
 void orc_shape(int lt, int ig, const double *xnode, double *sf, double *sfd, double *weight);
 int orc_dstrns(int npl, const double *sfd, const double *dxl, double smale, double smalr,
                double *del, double *rl, double *deff);

@@ -172,9 +172,11 @@ class Oracle:
         return m, det[0]
 
     def shape(self, lt, ig, xnode):
-        xnode = np.ascontiguousarray(xnode, np.float64).reshape(8, 3)
-        sf = np.zeros(8)
-        sfd = np.zeros((8, 3))
+        """Shape functions of element type lt (12, 13: 8 nodes; 22: 20 nodes) at point ig (0-based)."""
+        nn = 20 if lt == 22 else 8
+        xnode = np.ascontiguousarray(xnode, np.float64).reshape(nn, 3)
+        sf = np.zeros(nn)
+        sfd = np.zeros((nn, 3))
         w = np.zeros(1)
         self.lib.orc_shape(lt, ig, _ptr(xnode), _ptr(sf), _ptr(sfd), _ptr(w))
         return sf, sfd, w[0]

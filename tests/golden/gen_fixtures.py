@@ -11,6 +11,7 @@ Fixtures:
   dstrss_fcc.npz  DSTRSS on 256 FCC points (seed 11, 10% elastic elements)
   dstrss_hcp.npz  DSTRSS on 128 HCP points with twin systems (seed 12)
   nrstep_fcc.npz  ELVALS+SUVALS then FACSLS on a 3^3 one-point brick mesh
+  nrstep20_fcc.npz  the same step on a 2^3 mesh of 20-node bricks (SHAPE_22)
   incs_fcc.npz    two increments of the reference loop on a 2^3 one-point mesh
 """
 import os
@@ -25,7 +26,7 @@ sys.path.insert(0, HERE)
 
 import refio  # noqa: E402
 from gpbatch import make_gp_batch  # noqa: E402
-from cpfem_code_amd.problem import (LT_BRICK_1GP, Control, fcc_aluminium, fcc_tension,  # noqa: E402
+from cpfem_code_amd.problem import (LT_BRICK_1GP, LT_BRICK_20, Control, fcc_aluminium, fcc_tension,  # noqa: E402
                                     hcp_magnesium, f32)
 
 DT = 1.0e-3
@@ -66,6 +67,19 @@ def main():
                         in_ds=s["ds"].astype(np.float32),
                         err=r["err"], sumcpiter=np.float32(r["sumcpiter"]), fc=r["fc"].astype(np.float32),
                         rhs=r["rhs"].astype(np.float32), is_=r["is"], ss=r["ss"].astype(np.float32),
+                        ds=r["state"]["ds"].astype(np.float32), dsv=r["state"]["dsv"].astype(np.float32),
+                        de=r["state"]["de"].astype(np.float32), dep=r["state"]["dep"].astype(np.float32),
+                        err_sor=r["err_sor"], ddx=r["ddx"].astype(np.float32))
+
+    # the 20-node brick (SHAPE_22): one NRITER step on a 2^3 mesh
+    p = fcc_tension(2, ninc=4, lt=LT_BRICK_20, seed=5)
+    s = nrstep_state(p, 6)
+    r = refio.nrstep(p, s, inc=1)
+    np.savez_compressed(os.path.join(HERE, "nrstep20_fcc.npz"),
+                        in_dx=s["dx"].astype(np.float32), in_st=s["st"].astype(np.float32),
+                        in_ds=s["ds"].astype(np.float32),
+                        err=r["err"], fc=r["fc"].astype(np.float32), rhs=r["rhs"].astype(np.float32),
+                        is_=r["is"], ss=r["ss"].astype(np.float32),
                         ds=r["state"]["ds"].astype(np.float32), dsv=r["state"]["dsv"].astype(np.float32),
                         de=r["state"]["de"].astype(np.float32), dep=r["state"]["dep"].astype(np.float32),
                         err_sor=r["err_sor"], ddx=r["ddx"].astype(np.float32))
