@@ -212,41 +212,16 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
         double* bc = &sh_b[0][lane];
         double* gpre = A.gp + g;
         double de[6], r[3], st[6], t[6], s[6];
-        int kerr = 0, iel;
-        double deff_kin = 0.0;
-        if (A.mesh) {
-            const int e = A.gp_elem[g];
-            iel = A.elem_le[e];
-            if (first) {
-                double xl[8][3], dxl[8][3], sfd[8][3];
+        // strain and rotation increments come from the kinematics kernel (mesh
+        // mode) or the caller (batch mode); in mesh mode that kernel's error
+        // code (920, 905, 906) seeds the point's code
+        int kerr = 0;
+        const int iel = A.mesh ? A.elem_le[A.gp_elem[g]] : A.iel[g];
 #pragma unroll
-                for (int a = 0; a < 8; ++a) {
-                    const int nd = A.elem_nodes[8 * e + a];
+        for (int i = 0; i < 6; ++i) de[i] = A.de[i * n + g];
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        xl[a][k] = A.x[3 * nd + k];
-                        dxl[a][k] = A.dx[3 * nd + k];
-                    }
-                }
-                const double w = brick_shape(A.elem_lt[e], A.gp_local[g], xl, sfd);
-                if (w < 1.e-7) kerr = 920;
-                const int e2 = dstrns(sfd, dxl, A.smale, A.smalr, de, r, deff_kin);
-                if (e2 > kerr) kerr = e2;
-#pragma unroll
-                for (int i = 0; i < 6; ++i) A.de[i * n + g] = de[i];
-#pragma unroll
-                for (int i = 0; i < 3; ++i) A.r[i * n + g] = r[i];
-            }
-        } else {
-            iel = A.iel[g];
-        }
-        if (!first || !A.mesh) {
-#pragma unroll
-            for (int i = 0; i < 6; ++i) de[i] = A.de[i * n + g];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) r[i] = A.r[i * n + g];
-        }
-        if (!first) kerr = A.ck_kerr[g];
+        for (int i = 0; i < 3; ++i) r[i] = A.r[i * n + g];
+        if (!first || A.mesh) kerr = A.ck_kerr[g];
 #pragma unroll
         for (int i = 0; i < 6; ++i) st[i] = A.st[i * n + g];
         double ph[3];

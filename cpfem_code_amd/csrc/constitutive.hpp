@@ -268,20 +268,14 @@ __device__ __forceinline__ double twin_draw(uint64_t key) {
 // ---------------------------------------------------------------------------
 struct ConstArgs {
     int n;                     // points
-    int mesh;                  // 1: kinematics from the element nodes, 0: de/r given
-    // kinematics (mesh mode)
-    const int* gp_elem;
-    const int* gp_local;
-    const int* elem_nodes;
-    const int* elem_lt;
-    const int* elem_le;
-    const double* x;
-    const double* dx;
-    double smale, smalr;
+    int mesh;                  // 1: mesh points (material code of the element, error code
+                               //    of the kinematics kernel in ck_kerr); 0: batch of points
+    const int* gp_elem;        // [n] element of each point (mesh mode)
+    const int* elem_le;        // [nel] material code (mesh mode)
     // per-point arrays, field-major with stride n
     const int* iel;            // batch mode material code
-    double* de;                // [6][n] (written in mesh mode)
-    double* r;                 // [3][n] (written in mesh mode)
+    const double* de;          // [6][n] strain increments (engineering shears)
+    const double* r;           // [3][n] rotation increments
     double* dep;               // [n] deff out
     const double* st;          // [6][n]
     double* ds;                // [6][n] guess in, increment out

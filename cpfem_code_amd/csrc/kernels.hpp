@@ -82,7 +82,8 @@ __device__ __forceinline__ double brick_shape(int lt, int ig, const double xl[8]
 
 // DSTRNS (Sig_eps.f90:25-61): strain (eng. shear) and rotation increments.
 // Returns the FAFERR code of the magnitude traps (905 rotation, 906 strain).
-__device__ __forceinline__ int dstrns(const double sfd[8][3], const double dxl[8][3], double smale, double smalr,
+template <int NN>
+__device__ __forceinline__ int dstrns(const double sfd[NN][3], const double dxl[NN][3], double smale, double smalr,
                                       double del[6], double rl[3], double& deff) {
     double d[3][3];
 #pragma unroll
@@ -91,7 +92,7 @@ __device__ __forceinline__ int dstrns(const double sfd[8][3], const double dxl[8
         for (int j = 0; j < 3; ++j) {
             double s = 0.0;
 #pragma unroll
-            for (int n = 0; n < 8; ++n) s = s + sfd[n][i] * dxl[n][j];
+            for (int n = 0; n < NN; ++n) s = s + sfd[n][i] * dxl[n][j];
             d[i][j] = s;
         }
     double sum = 0.0;

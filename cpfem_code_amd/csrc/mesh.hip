@@ -109,7 +109,8 @@ struct AsmArgs {
     const int* elem_gps;
     const int* elem_lt;
     const int* elem_np;
-    const int* scatter;          // [nel][64] entry 0 of each node pair's block; entry e at +64*e
+    const int* scatter;          // entry 0 of each node pair's block (entry e at +64*e) ...
+    const int* scat_start;       // ... from scat_start[element], pair (a, b) at a*nd + b
     const double* x;
     const double* chi;           // [36][ngps]
     const double* t;             // [6][ngps]
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(256) void k_assemble(const AsmArgs A, T* __restrict
     const int lt = live ? A.elem_lt[e] : 13;
     if (live && lane < 24) {
         const int a = lane / 3, c = lane % 3;
-        sh_x[wave][a][c] = A.x[3 * A.elem_nodes[8 * e + a] + c];
+        sh_x[wave][a][c] = A.x[3 * A.elem_nodes[kMaxNodes * e + a] + c];
     }
     __syncthreads();
     const size_t n = (size_t)A.ngps;
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(256) void k_assemble(const AsmArgs A, T* __restrict
 #pragma unroll
             for (int q = 0; q < 3; ++q) acc[p][q] = acc[p][q] + w * gm[p][q];
     }
-    const int off = A.scatter[64 * e + lane];
+    const int off = A.scatter[A.scat_start[e] + lane];
 #pragma unroll
     for (int p = 0; p < 3; ++p)
 #pragma unroll
@@ -205,9 +206,272 @@ __global__ __launch_bounds__(256) void k_assemble(const AsmArgs A, T* __restrict
             fl[1] = fl[1] + w * (s[1] * sfd[i][1] + s[3] * sfd[i][2] + s[5] * sfd[i][0]);
             fl[2] = fl[2] + w * (s[2] * sfd[i][2] + s[4] * sfd[i][0] + s[3] * sfd[i][1]);
         }
-        const int nd = A.elem_nodes[8 * e + i];
+        const int nd = A.elem_nodes[kMaxNodes * e + i];
 #pragma unroll
         for (int c = 0; c < 3; ++c) A.fc[3 * nd + c] = A.fc[3 * nd + c] + fl[c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 20-node brick (SHAPE_22, Shapef.f90:210-342)
+// ---------------------------------------------------------------------------
+// Tabulated shape functions and local derivatives at the 2x2x2 points, as the
+// reference holds them (6 decimals): the quadratic serendipity functions at
+// +-g, g = 1/sqrt(3) in single precision, each rounded to 6 decimals.  Filled
+// from the host at context creation.
+__constant__ double kTab22N[8][20];
+__constant__ double kTab22D[8][3][20];
+
+// Local coordinates of the 20 nodes: bottom corners 1,3,5,7 and mid-sides
+// 2,4,6,8 anticlockwise seen from the top, 9-12 halfway up the vertical edges,
+// top face 13-20 like the bottom (Shapef.f90:191-200).
+static const int kBrick20Loc[20][3] = {
+    {-1, -1, -1}, {0, -1, -1}, {1, -1, -1}, {1, 0, -1}, {1, 1, -1}, {0, 1, -1}, {-1, 1, -1}, {-1, 0, -1},
+    {-1, -1, 0},  {1, -1, 0},  {1, 1, 0},   {-1, 1, 0},
+    {-1, -1, 1},  {0, -1, 1},  {1, -1, 1},  {1, 0, 1},  {1, 1, 1},  {0, 1, 1},  {-1, 1, 1},  {-1, 0, 1}};
+static const int kBrickSign[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                     {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+
+static hipError_t upload_brick20_tables() {
+    double tn[8][20], td[8][3][20];
+    const double g = (double)(float)(1.0 / sqrt(3.0));
+    auto r6 = [](double v) { return round(v * 1.0e6) / 1.0e6; };
+    for (int ig = 0; ig < 8; ++ig) {
+        double xi[3];
+        for (int k = 0; k < 3; ++k) xi[k] = kBrickSign[ig][k] * g;
+        for (int a = 0; a < 20; ++a) {
+            const int* c = kBrick20Loc[a];
+            double n, d[3];
+            if (c[0] != 0 && c[1] != 0 && c[2] != 0) {
+                double f[3], sum = -2.0;
+                for (int k = 0; k < 3; ++k) {
+                    f[k] = 1.0 + xi[k] * c[k];
+                    sum += xi[k] * c[k];
+                }
+                n = f[0] * f[1] * f[2] * sum / 8.0;
+                for (int k = 0; k < 3; ++k)
+                    d[k] = c[k] * f[(k + 1) % 3] * f[(k + 2) % 3] * sum / 8.0 + f[0] * f[1] * f[2] * c[k] / 8.0;
+            } else {
+                int z = 0;
+                while (c[z] != 0) ++z;
+                const int p = (z + 1) % 3, q = (z + 2) % 3;
+                const double qz = 1.0 - xi[z] * xi[z], fp = 1.0 + xi[p] * c[p], fq = 1.0 + xi[q] * c[q];
+                n = qz * fp * fq / 4.0;
+                d[z] = -2.0 * xi[z] * fp * fq / 4.0;
+                d[p] = qz * c[p] * fq / 4.0;
+                d[q] = qz * fp * c[q] / 4.0;
+            }
+            tn[ig][a] = r6(n);
+            for (int k = 0; k < 3; ++k) td[ig][k][a] = r6(d[k]);
+        }
+    }
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(kTab22N), tn, sizeof(tn));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(kTab22D), td, sizeof(td));
+    return e;
+}
+
+// Global derivatives and weight (detJ) of the 20-node brick at point ig.
+__device__ __forceinline__ double brick20_shape(int ig, const double xl[20][3], double sfd[20][3]) {
+    double jac[3][3];  // jac[j][k] = sum_a dndn(j,a) x(k,a)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+#pragma unroll
+            for (int a = 0; a < 20; ++a) s = s + kTab22D[ig][j][a] * xl[a][k];
+            jac[j][k] = s;
+        }
+    const double det = invert_3(jac);
+#pragma unroll
+    for (int a = 0; a < 20; ++a)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) s = s + jac[k][j] * kTab22D[ig][j][a];
+            sfd[a][k] = s;
+        }
+    return 1.0 * det;
+}
+
+// ---------------------------------------------------------------------------
+// point kinematics (Elvals.f90:87-112): shape functions, weight check (920)
+// and DSTRNS (905, 906) for the points of NN-node elements, one per thread
+// ---------------------------------------------------------------------------
+struct KinArgs {
+    int n;
+    const int* gp_elem;
+    const int* gp_local;
+    const int* elem_nodes;
+    const int* elem_lt;
+    const int* elem_nd;
+    const double* x;
+    const double* dx;
+    double smale, smalr;
+    double* de;    // [6][n]
+    double* r;     // [3][n]
+    int* kerr;     // [n]
+};
+
+template <int NN>
+__global__ __launch_bounds__(kBlock) void k_kin(const KinArgs K) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= K.n) return;
+    const int e = K.gp_elem[g];
+    if (K.elem_nd[e] != NN) return;
+    double xl[NN][3], dxl[NN][3], sfd[NN][3];
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+        const int nd = K.elem_nodes[kMaxNodes * e + a];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            xl[a][k] = K.x[3 * nd + k];
+            dxl[a][k] = K.dx[3 * nd + k];
+        }
+    }
+    double w;
+    if constexpr (NN == 8)
+        w = brick_shape(K.elem_lt[e], K.gp_local[g], xl, sfd);
+    else
+        w = brick20_shape(K.gp_local[g], xl, sfd);
+    int kerr = (w < 1.e-7) ? 920 : 0;
+    double de[6], r[3], deff;
+    const int e2 = dstrns<NN>(sfd, dxl, K.smale, K.smalr, de, r, deff);
+    if (e2 > kerr) kerr = e2;
+    const size_t n = (size_t)K.n;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) K.de[i * n + g] = de[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) K.r[i * n + g] = r[i];
+    K.kerr[g] = kerr;
+}
+
+// ---------------------------------------------------------------------------
+// 20-node element assembly: one 256-thread block per element, thread = node
+// pair (i, j) of the 400 (two per thread for the first 144)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_assemble20(const AsmArgs A, T* __restrict__ vals) {
+    __shared__ double sh_x[20][3];
+    __shared__ double sh_dn64[8][20][3];
+    __shared__ double sh_w64[8];
+    __shared__ double sh_s64[8][6];
+    __shared__ T sh_dn[8][20][3];
+    __shared__ T sh_w[8];
+    __shared__ T sh_d[8][36];
+    __shared__ T sh_s[8][6];
+    const int e = A.elems[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int ng = A.elem_np[e];
+    if (tid < 60) sh_x[tid / 3][tid % 3] = A.x[3 * A.elem_nodes[kMaxNodes * e + tid / 3] + tid % 3];
+    __syncthreads();
+    const size_t n = (size_t)A.ngps;
+    if (tid < ng) {
+        double xl[20][3], sfd[20][3];
+#pragma unroll
+        for (int a = 0; a < 20; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) xl[a][c] = sh_x[a][c];
+        const double w = brick20_shape(tid, xl, sfd);
+#pragma unroll
+        for (int a = 0; a < 20; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                sh_dn64[tid][a][c] = sfd[a][c];
+                sh_dn[tid][a][c] = (T)sfd[a][c];
+            }
+        sh_w64[tid] = w;
+        sh_w[tid] = (T)w;
+        const int gp = A.elem_gps[8 * e + tid];
+#pragma unroll
+        for (int q = 0; q < 36; ++q) sh_d[tid][q] = (T)A.chi[q * n + gp];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const double v = A.t[q * n + gp];
+            sh_s64[tid][q] = v;
+            sh_s[tid][q] = (T)v;
+        }
+    }
+    __syncthreads();
+    const int base = A.scat_start[e];
+    for (int p = tid; p < 400; p += 256) {
+        const int i = p / 20, j = p - 20 * (p / 20);
+        T acc[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) acc[a][b] = (T)0;
+        for (int g = 0; g < ng; ++g) {
+            T a[3], b[3], sg[6], gm[3][3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                a[c] = sh_dn[g][i][c];
+                b[c] = sh_dn[g][j][c];
+            }
+#pragma unroll
+            for (int q = 0; q < 6; ++q) sg[q] = sh_s[g][q];
+            pair_block<T>(a, b, &sh_d[g][0], sg, gm);
+            const T w = sh_w[g];
+#pragma unroll
+            for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+                for (int b2 = 0; b2 < 3; ++b2) acc[a2][b2] = acc[a2][b2] + w * gm[a2][b2];
+        }
+        const int off = A.scatter[base + p];
+#pragma unroll
+        for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+            for (int b2 = 0; b2 < 3; ++b2) {
+                T* dst = vals + off + (3 * a2 + b2) * 64;
+                *dst = *dst + acc[a2][b2];
+            }
+    }
+    // internal forces of node tid (Elvals.f90:122-126)
+    if (tid < 20) {
+        double fl[3] = {0.0, 0.0, 0.0};
+        for (int g = 0; g < ng; ++g) {
+            const double w = sh_w64[g];
+            const double* s = sh_s64[g];
+            const double* d = sh_dn64[g][tid];
+            fl[0] = fl[0] + w * (s[0] * d[0] + s[5] * d[1] + s[4] * d[2]);
+            fl[1] = fl[1] + w * (s[1] * d[1] + s[3] * d[2] + s[5] * d[0]);
+            fl[2] = fl[2] + w * (s[2] * d[2] + s[4] * d[0] + s[3] * d[1]);
+        }
+        const int nd = A.elem_nodes[kMaxNodes * e + tid];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) A.fc[3 * nd + c] = A.fc[3 * nd + c] + fl[c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 'tie' elements (no points, Elvals.f90:62-77): penalty efix on DOF k of the
+// first node and -efix coupling it to DOF k of the second; lt 1-3 tie DOF lt,
+// lt 4-6 the two DOFs 1+mod(lt-3,3) and the next one.  One thread per element.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_ties(int nt, const int* __restrict__ elems, const int* __restrict__ elem_lt,
+                       const int* __restrict__ scatter, const int* __restrict__ scat_start, double efix,
+                       T* __restrict__ vals) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const int e = elems[t];
+    const int itp = elem_lt[e];
+    const int b = scat_start[e];
+    const int off11 = scatter[b + 0], off12 = scatter[b + 1];  // pairs (1,1) and (1,2) of the 2 nodes
+    int dofs[2] = {-1, -1};
+    if (itp >= 1 && itp <= 3) {
+        dofs[0] = itp - 1;
+    } else if (itp >= 4 && itp <= 6) {
+        dofs[0] = (itp - 3) % 3;
+        dofs[1] = (dofs[0] + 1) % 3;
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int k = dofs[q];
+        if (k < 0) continue;
+        vals[off11 + 4 * k * 64] = vals[off11 + 4 * k * 64] + (T)efix;
+        vals[off12 + 4 * k * 64] = vals[off12 + 4 * k * 64] + (T)(-efix);
     }
 }
 
@@ -651,13 +915,17 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-static int build_structure(Ctx* c, const cpfem_mesh* m, std::vector<int>& en, std::vector<int>& eg) {
+// Sparsity (the node blocks SETPTA's row lists cover, Stif_sol.f90:98-149),
+// node and element colourings, the row-group layout and the scatter map.
+// en: [nel][kMaxNodes] 0-based nodes, nde: nodes per element, kind: ElemKind.
+static int build_structure(Ctx* c, const cpfem_mesh* m, const std::vector<int>& en, const std::vector<int>& nde,
+                           const std::vector<int>& kind) {
     const int nel = m->nlmnt, nnod = m->nnod;
     // node neighbour lists (nodes sharing an element), self first
     std::vector<std::vector<int>> nb(nnod);
     for (int e = 0; e < nel; ++e)
-        for (int a = 0; a < 8; ++a)
-            for (int b = 0; b < 8; ++b) nb[en[8 * e + a]].push_back(en[8 * e + b]);
+        for (int a = 0; a < nde[e]; ++a)
+            for (int b = 0; b < nde[e]; ++b) nb[en[kMaxNodes * e + a]].push_back(en[kMaxNodes * e + b]);
     int S = 0;
     size_t nnzb = 0;
     for (int i = 0; i < nnod; ++i) {
@@ -714,29 +982,37 @@ static int build_structure(Ctx* c, const cpfem_mesh* m, std::vector<int>& en, st
             cols[col_index(gpos[r], P, s)] = s < (int)nb[nd].size() ? nb[nd][s] : nd;
         diag_off[nd] = (int)val_index(gpos[r], P, 0, 0);
     }
-    // element scatter map: entry 0 of the block of each node pair
-    std::vector<int> scat((size_t)nel * 64);
+    // element scatter map: entry 0 of the block of each node pair (a, b) at
+    // scat_start[e] + a*nd + b
+    std::vector<int> scat_start(nel);
+    size_t nscat = 0;
+    for (int e = 0; e < nel; ++e) {
+        scat_start[e] = (int)nscat;
+        nscat += (size_t)nde[e] * nde[e];
+    }
+    if (nscat > 0x7fffffffull) return CPFEM_E_ARG;
+    std::vector<int> scat(std::max<size_t>(nscat, 1));
     for (int e = 0; e < nel; ++e)
-        for (int a = 0; a < 8; ++a) {
-            const int ni = en[8 * e + a];
+        for (int a = 0; a < nde[e]; ++a) {
+            const int ni = en[kMaxNodes * e + a];
             const auto& v = nb[ni];
-            for (int b = 0; b < 8; ++b) {
-                const int nj = en[8 * e + b];
+            for (int b = 0; b < nde[e]; ++b) {
+                const int nj = en[kMaxNodes * e + b];
                 int slot = 0;
                 if (nj != ni) slot = (int)(std::lower_bound(v.begin() + 1, v.end(), nj) - v.begin());
-                scat[64 * e + 8 * a + b] = (int)val_index(gpos[node_row[ni]], P, slot, 0);
+                scat[scat_start[e] + a * nde[e] + b] = (int)val_index(gpos[node_row[ni]], P, slot, 0);
             }
         }
     // element colouring (elements sharing a node differ)
     std::vector<std::vector<int>> node_elems(nnod);
     for (int e = 0; e < nel; ++e)
-        for (int a = 0; a < 8; ++a) node_elems[en[8 * e + a]].push_back(e);
+        for (int a = 0; a < nde[e]; ++a) node_elems[en[kMaxNodes * e + a]].push_back(e);
     std::vector<int> ecol(nel, -1);
     int necol = 0;
     for (int e = 0; e < nel; ++e) {
         used.assign(256, 0);
-        for (int a = 0; a < 8; ++a)
-            for (int f : node_elems[en[8 * e + a]])
+        for (int a = 0; a < nde[e]; ++a)
+            for (int f : node_elems[en[kMaxNodes * e + a]])
                 if (ecol[f] >= 0) used[ecol[f]] = 1;
         int k = 0;
         while (k < 256 && used[k]) ++k;
@@ -744,13 +1020,14 @@ static int build_structure(Ctx* c, const cpfem_mesh* m, std::vector<int>& en, st
         ecol[e] = k;
         necol = std::max(necol, k + 1);
     }
-    c->ecstart.assign(necol + 1, 0);
-    for (int e = 0; e < nel; ++e) c->ecstart[ecol[e] + 1]++;
-    for (int k = 0; k < necol; ++k) c->ecstart[k + 1] += c->ecstart[k];
+    // elements ordered by colour, then kind
+    c->ekstart.assign(3 * necol + 1, 0);
+    for (int e = 0; e < nel; ++e) c->ekstart[3 * ecol[e] + kind[e] + 1]++;
+    for (int k = 0; k < 3 * necol; ++k) c->ekstart[k + 1] += c->ekstart[k];
     std::vector<int> ebc(nel);
     {
-        std::vector<int> pos(c->ecstart.begin(), c->ecstart.end() - 1);
-        for (int e = 0; e < nel; ++e) ebc[pos[ecol[e]]++] = e;
+        std::vector<int> pos(c->ekstart.begin(), c->ekstart.end() - 1);
+        for (int e = 0; e < nel; ++e) ebc[pos[3 * ecol[e] + kind[e]]++] = e;
     }
     c->A.R = R;
     c->A.S = S;
@@ -760,7 +1037,8 @@ static int build_structure(Ctx* c, const cpfem_mesh* m, std::vector<int>& en, st
     c->A.nnzb = nnzb;
     const size_t vbytes = (c->precision == 1) ? sizeof(float) : sizeof(double);
     if (dalloc(c, &c->A.diag_off, nnod) || dalloc(c, &c->A.cols, cols.size()) ||
-        dalloc(c, &c->A.scatter, (size_t)nel * 64) || dalloc(c, &c->elem_by_color, nel))
+        dalloc(c, &c->A.scatter, scat.size()) || dalloc(c, &c->A.scat_start, nel) ||
+        dalloc(c, &c->elem_by_color, nel))
         return CPFEM_E_HIP;
     char* vp = nullptr;
     if (dalloc(c, &vp, c->A.nvals() * vbytes)) return CPFEM_E_HIP;
@@ -768,6 +1046,8 @@ static int build_structure(Ctx* c, const cpfem_mesh* m, std::vector<int>& en, st
     CPFEM_HIP(hipMemcpyAsync(c->A.diag_off, diag_off.data(), nnod * sizeof(int), hipMemcpyHostToDevice, c->stream));
     CPFEM_HIP(hipMemcpyAsync(c->A.cols, cols.data(), cols.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     CPFEM_HIP(hipMemcpyAsync(c->A.scatter, scat.data(), scat.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    CPFEM_HIP(hipMemcpyAsync(c->A.scat_start, scat_start.data(), nel * sizeof(int), hipMemcpyHostToDevice,
+                             c->stream));
     CPFEM_HIP(hipMemcpyAsync(c->elem_by_color, ebc.data(), nel * sizeof(int), hipMemcpyHostToDevice, c->stream));
     CPFEM_HIP(hipStreamSynchronize(c->stream));
     return 0;
@@ -790,29 +1070,46 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
     if (ctl->precision == 1 && ctl->sor_kind != 1) return CPFEM_E_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CPFEM_E_NOGPU;
-    // element tables
+    // element tables.  Types: 12 (8-node brick, 1 point, SHAPE_12), 13 (8-node
+    // brick, 2x2x2 points), 22 (20-node brick, 2x2x2 points, SHAPE_22) and the
+    // two-node 'tie' elements 1-6 without points (Elvals.f90:62-77).
     const int nel = m->nlmnt;
-    std::vector<int> en((size_t)nel * 8), eg((size_t)nel * 8, -1), lt(nel), le(nel), np(nel);
+    std::vector<int> en((size_t)nel * kMaxNodes, 0), eg((size_t)nel * 8, -1), lt(nel), le(nel), np(nel), nde(nel),
+        kind(nel);
     std::vector<int> gpe(m->ngps, -1), gpl(m->ngps, -1);
+    int npts8 = 0, npts20 = 0;
     for (int e = 0; e < nel; ++e) {
-        if (m->nd[e] != 8 || (m->lt[e] != 12 && m->lt[e] != 13)) return CPFEM_E_ARG;
-        if ((m->lt[e] == 12 && m->np[e] != 1) || (m->lt[e] == 13 && m->np[e] != 8)) return CPFEM_E_ARG;
-        if (m->le[e] == 0 || abs(m->le[e]) > 1) return CPFEM_E_ARG;  // one material set
-        lt[e] = m->lt[e];
-        le[e] = m->le[e];
-        np[e] = m->np[e];
-        for (int a = 0; a < 8; ++a) {
-            const int nd = m->ln[20 * e + a] - 1;
-            if (nd < 0 || nd >= m->nnod) return CPFEM_E_ARG;
-            en[8 * e + a] = nd;
+        const int t = m->lt[e], nd = m->nd[e], ng = m->np[e];
+        if (t == 12 && nd == 8 && ng == 1) {
+            kind[e] = kElem8;
+        } else if (t == 13 && nd == 8 && ng == 8) {
+            kind[e] = kElem8;
+        } else if (t == 22 && nd == 20 && ng == 8) {
+            kind[e] = kElem20;
+        } else if (t >= 1 && t <= 6 && nd == 2 && ng == 0) {
+            kind[e] = kTie;
+        } else {
+            return CPFEM_E_ARG;
         }
-        for (int g = 0; g < np[e]; ++g) {
+        if (kind[e] != kTie && (m->le[e] == 0 || abs(m->le[e]) > 1)) return CPFEM_E_ARG;  // one material set
+        lt[e] = t;
+        le[e] = m->le[e];
+        np[e] = ng;
+        nde[e] = nd;
+        for (int a = 0; a < nd; ++a) {
+            const int node = m->ln[20 * e + a] - 1;
+            if (node < 0 || node >= m->nnod) return CPFEM_E_ARG;
+            en[kMaxNodes * e + a] = node;
+        }
+        for (int g = 0; g < ng; ++g) {
             const int gp = m->lg[8 * e + g] - 1;
             if (gp < 0 || gp >= m->ngps || gpe[gp] >= 0) return CPFEM_E_ARG;
             eg[8 * e + g] = gp;
             gpe[gp] = e;
             gpl[gp] = g;
         }
+        if (kind[e] == kElem8) npts8 += ng;
+        if (kind[e] == kElem20) npts20 += ng;
     }
     for (int g = 0; g < m->ngps; ++g)
         if (gpe[g] < 0) return CPFEM_E_ARG;
@@ -827,6 +1124,8 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
     c->ctl = *ctl;
     c->precision = ctl->precision;
     c->tiem.assign(m->tiem, m->tiem + m->ninc1);
+    c->npts8 = npts8;
+    c->npts20 = npts20;
     memset(&c->stats, 0, sizeof(c->stats));
     int rc = 0;
 #define TRY(x)                      \
@@ -845,6 +1144,7 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
         TRY(dalloc(c, &c->elem_lt, nel));
         TRY(dalloc(c, &c->elem_le, nel));
         TRY(dalloc(c, &c->elem_np, nel));
+        TRY(dalloc(c, &c->elem_nd, nel));
         TRY(dalloc(c, &c->gp_elem, G));
         TRY(dalloc(c, &c->gp_local, G));
         TRY(dalloc(c, &c->ib, c->nnod));
@@ -855,6 +1155,8 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
         TRY(hipMemcpyAsync(c->elem_lt, lt.data(), nel * 4, hipMemcpyHostToDevice, c->stream));
         TRY(hipMemcpyAsync(c->elem_le, le.data(), nel * 4, hipMemcpyHostToDevice, c->stream));
         TRY(hipMemcpyAsync(c->elem_np, np.data(), nel * 4, hipMemcpyHostToDevice, c->stream));
+        TRY(hipMemcpyAsync(c->elem_nd, nde.data(), nel * 4, hipMemcpyHostToDevice, c->stream));
+        if (npts20 > 0) TRY(upload_brick20_tables());
         TRY(hipMemcpyAsync(c->gp_elem, gpe.data(), G * 4, hipMemcpyHostToDevice, c->stream));
         TRY(hipMemcpyAsync(c->gp_local, gpl.data(), G * 4, hipMemcpyHostToDevice, c->stream));
         TRY(hipMemcpyAsync(c->ib, m->ib, c->nnod * 4, hipMemcpyHostToDevice, c->stream));
@@ -911,7 +1213,7 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
             TRY(hipStreamSynchronize(c->stream));
         }
         TRY(alloc_schedule(c->sched, c->ngps));
-        rc = build_structure(c, m, en, eg);
+        rc = build_structure(c, m, en, nde, kind);
         if (rc) goto fail;
     }
     *out = c;
@@ -1018,19 +1320,29 @@ static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
     c->rng_epoch += 1;
     CPFEM_HIP(hipEventRecord(c->ev[0], s));
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, c->flags + 1, (size_t)1, 0);
+    // point kinematics (strain and rotation increments, element checks)
+    KinArgs K;
+    K.n = c->ngps;
+    K.gp_elem = c->gp_elem;
+    K.gp_local = c->gp_local;
+    K.elem_nodes = c->elem_nodes;
+    K.elem_lt = c->elem_lt;
+    K.elem_nd = c->elem_nd;
+    K.x = c->x;
+    K.dx = c->dx;
+    K.smale = c->ctl.smale;
+    K.smalr = c->ctl.smalr;
+    K.de = c->de;
+    K.r = c->dr;
+    K.kerr = c->sched.ck_kerr;
+    if (c->npts8 > 0) hipLaunchKernelGGL(k_kin<8>, dim3(nblk(c->ngps)), dim3(kBlock), 0, s, K);
+    if (c->npts20 > 0) hipLaunchKernelGGL(k_kin<20>, dim3(nblk(c->ngps)), dim3(kBlock), 0, s, K);
     ConstArgs A;
     memset(&A, 0, sizeof(A));
     A.n = c->ngps;
     A.mesh = 1;
     A.gp_elem = c->gp_elem;
-    A.gp_local = c->gp_local;
-    A.elem_nodes = c->elem_nodes;
-    A.elem_lt = c->elem_lt;
     A.elem_le = c->elem_le;
-    A.x = c->x;
-    A.dx = c->dx;
-    A.smale = c->ctl.smale;
-    A.smalr = c->ctl.smalr;
     A.de = c->de;
     A.r = c->dr;
     A.dep = c->dep;
@@ -1072,19 +1384,40 @@ static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
     AA.elem_lt = c->elem_lt;
     AA.elem_np = c->elem_np;
     AA.scatter = c->A.scatter;
+    AA.scat_start = c->A.scat_start;
     AA.x = c->x;
     AA.chi = c->chi;
     AA.t = c->t;
     AA.ngps = c->ngps;
     AA.fc = c->fc;
-    for (size_t k = 0; k + 1 < c->ecstart.size(); ++k) {
-        AA.nel_color = c->ecstart[k + 1] - c->ecstart[k];
-        AA.elems = c->elem_by_color + c->ecstart[k];
-        const int nb = (AA.nel_color + 3) / 4;
-        if (c->precision == 1)
-            hipLaunchKernelGGL(k_assemble<float>, dim3(nb), dim3(256), 0, s, AA, (float*)c->A.vals);
-        else
-            hipLaunchKernelGGL(k_assemble<double>, dim3(nb), dim3(256), 0, s, AA, (double*)c->A.vals);
+    const bool f32 = c->precision == 1;
+    const int ncol_el = ((int)c->ekstart.size() - 1) / 3;
+    for (int k = 0; k < ncol_el; ++k) {
+        for (int q = 0; q < 3; ++q) {
+            const int b0 = c->ekstart[3 * k + q], cnt = c->ekstart[3 * k + q + 1] - b0;
+            if (cnt == 0) continue;
+            AA.nel_color = cnt;
+            AA.elems = c->elem_by_color + b0;
+            if (q == kElem8) {
+                const int nb = (cnt + 3) / 4;
+                if (f32)
+                    hipLaunchKernelGGL(k_assemble<float>, dim3(nb), dim3(256), 0, s, AA, (float*)c->A.vals);
+                else
+                    hipLaunchKernelGGL(k_assemble<double>, dim3(nb), dim3(256), 0, s, AA, (double*)c->A.vals);
+            } else if (q == kElem20) {
+                if (f32)
+                    hipLaunchKernelGGL(k_assemble20<float>, dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
+                else
+                    hipLaunchKernelGGL(k_assemble20<double>, dim3(cnt), dim3(256), 0, s, AA, (double*)c->A.vals);
+            } else {
+                if (f32)
+                    hipLaunchKernelGGL(k_ties<float>, dim3(nblk(cnt)), dim3(kBlock), 0, s, cnt, AA.elems, c->elem_lt,
+                                       c->A.scatter, c->A.scat_start, c->ctl.efix, (float*)c->A.vals);
+                else
+                    hipLaunchKernelGGL(k_ties<double>, dim3(nblk(cnt)), dim3(kBlock), 0, s, cnt, AA.elems, c->elem_lt,
+                                       c->A.scatter, c->A.scat_start, c->ctl.efix, (double*)c->A.vals);
+            }
+        }
     }
     CPFEM_HIP(hipGetLastError());
     CPFEM_HIP(hipEventRecord(c->ev[2], s));

@@ -38,6 +38,9 @@ struct Reducer {
     int nblocks = 0;
 };
 
+constexpr int kMaxNodes = 20;                    // nodes per element (20-node brick)
+enum ElemKind { kElem8 = 0, kElem20 = 1, kTie = 2 };
+
 constexpr int kRowLanes = 9;                     // lanes per matrix row
 constexpr int kGroupRows = 64 / kRowLanes;       // rows per group (= per wave)
 
@@ -63,7 +66,9 @@ struct Matrix {
     int* cols = nullptr;       // [ngroups][P][64] neighbour node of each slot
     void* vals = nullptr;      // [ngroups][P][9][64] double or float
     int* diag_off = nullptr;   // [nnod] index of entry 0 of each node's diagonal block
-    int* scatter = nullptr;    // [nel][64] index of entry 0 of the block of each element node pair
+    int* scatter = nullptr;    // per element nd*nd entries from scat_start[e]: index of entry 0 of the
+                               // block of node pair (a, b) at a*nd + b
+    int* scat_start = nullptr; // [nel]
     size_t nnzb = 0;           // true (unpadded) blocks
     size_t nvals() const { return (size_t)ngroups * P * 576; }
 };
@@ -75,15 +80,19 @@ struct Ctx {
     void* dmat = nullptr;
     hipStream_t stream = nullptr;
     // mesh
-    int* elem_nodes = nullptr;   // [nel][8] 0-based
+    int* elem_nodes = nullptr;   // [nel][kMaxNodes] 0-based
     int* elem_gps = nullptr;     // [nel][8] 0-based (unused slots -1)
+    int* elem_nd = nullptr;      // [nel] nodes of each element
     int* elem_lt = nullptr;      // [nel]
     int* elem_le = nullptr;      // [nel]
     int* elem_np = nullptr;      // [nel]
     int* gp_elem = nullptr;      // [ngps]
     int* gp_local = nullptr;     // [ngps]
     int* ib = nullptr;           // [nnod]
-    std::vector<int> ecstart;    // element colour ranges
+    // elements by colour and, within a colour, by kind (kElem8, kElem20, kTie):
+    // kind q of colour k is elem_by_color[ekstart[3k+q] .. ekstart[3k+q+1])
+    std::vector<int> ekstart;
+    int npts8 = 0, npts20 = 0;   // points of 8-node and of 20-node elements
     int* elem_by_color = nullptr;
     double* x_spec = nullptr;    // [ninc1][99][15]
     double* f_spec = nullptr;    // [ninc1][99][3]
