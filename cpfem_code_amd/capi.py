@@ -179,6 +179,7 @@ def _bind_mesh_api(L):
     L.cpfem_update_gptv.argtypes = [_CTX]
     L.cpfem_reorient.argtypes = [_CTX, _P]
     L.cpfem_set_tflag.argtypes = [_CTX, C.c_int32]
+    L.cpfem_volume_average.argtypes = [_CTX, _P]
     L.cpfem_increment.argtypes = [_CTX, C.c_int32, C.POINTER(CpfemStats)]
     L.cpfem_get_system.argtypes = [_CTX, _P, _P]
     L.cpfem_get_matrix.argtypes = [_CTX, _P, _P, _P, _P]
@@ -316,6 +317,12 @@ class Solver:
 
     def update_gptv(self):
         _check(self.L.cpfem_update_gptv(self.h), "cpfem_update_gptv")
+
+    def volume_average(self):
+        """Volume, mean stress (6), mean strain ep (6) and mean eps over all points."""
+        out = np.zeros(14)
+        _check(self.L.cpfem_volume_average(self.h, _ptr(out)), "cpfem_volume_average")
+        return dict(volume=float(out[0]), st=out[1:7].copy(), ep=out[7:13].copy(), eps=float(out[13]))
 
     def set_tflag(self, tflag):
         _check(self.L.cpfem_set_tflag(self.h, int(tflag)), "cpfem_set_tflag")

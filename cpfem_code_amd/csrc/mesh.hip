@@ -348,6 +348,54 @@ __global__ __launch_bounds__(kBlock) void k_kin(const KinArgs K) {
 }
 
 // ---------------------------------------------------------------------------
+// volume averages over the points (the macroscopic state): point weights are
+// the integration weights at the current coordinates; four sums per launch,
+// group 0: (weight, weight*eps, weight*st11, weight*st22), 1: (st33, st23,
+// st31, st12), 2: (ep11, ep22, ep33, ep23), 3: (ep31, ep12, 0, 0)
+// ---------------------------------------------------------------------------
+template <int NN>
+__global__ __launch_bounds__(kBlock) void k_vol_sums(const KinArgs K, const double* __restrict__ st,
+                                                     const double* __restrict__ ep, const double* __restrict__ eps,
+                                                     int group, double* partial, int slot0) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < K.n) {
+        const int e = K.gp_elem[g];
+        if (K.elem_nd[e] == NN) {
+            double xl[NN][3], sfd[NN][3];
+#pragma unroll
+            for (int a = 0; a < NN; ++a) {
+                const int nd = K.elem_nodes[kMaxNodes * e + a];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) xl[a][k] = K.x[3 * nd + k];
+            }
+            double w;
+            if constexpr (NN == 8)
+                w = brick_shape(K.elem_lt[e], K.gp_local[g], xl, sfd);
+            else
+                w = brick20_shape(K.gp_local[g], xl, sfd);
+            const size_t n = (size_t)K.n;
+            if (group == 0) {
+                v[0] = w;
+                v[1] = w * eps[g];
+                v[2] = w * st[0 * n + g];
+                v[3] = w * st[1 * n + g];
+            } else if (group == 1) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v[q] = w * st[(2 + q) * n + g];
+            } else if (group == 2) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v[q] = w * ep[q * n + g];
+            } else {
+                v[0] = w * ep[4 * n + g];
+                v[1] = w * ep[5 * n + g];
+            }
+        }
+    }
+    block_partials<4>(v, partial, slot0 + blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
 // 20-node element assembly: one 256-thread block per element, thread = node
 // pair (i, j) of the 400 (two per thread for the first 144)
 // ---------------------------------------------------------------------------
@@ -1195,7 +1243,8 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
         TRY(dalloc(c, &c->flags, 8));
         // reduction scratch: enough partial slots for the largest launch
         // (row-group launches use at most nnod / (4 * kGroupRows) + 8 blocks per colour)
-        const int nbr = std::max(nblk(std::max<size_t>(std::max<size_t>(G, D), 8 * (size_t)c->nnod)) + 64,
+        // (the volume averages use two point-sized launches)
+        const int nbr = std::max(nblk(std::max<size_t>(std::max<size_t>(2 * G, D), 8 * (size_t)c->nnod)) + 64,
                                  (int)((size_t)c->nnod / (kWavesPerBlock * kGroupRows)) + 16 * 64 + 64);
         c->red.nblocks = nbr;
         TRY(dalloc(c, &c->red.partial, (size_t)nbr * 4));
@@ -1827,6 +1876,44 @@ extern "C" int cpfem_reorient(cpfem_ctx* c, int32_t* n) {
         memcpy(&cnt, c->red.host + 6, sizeof(int));
     }
     if (n) *n = cnt;
+    return 0;
+}
+
+extern "C" int cpfem_volume_average(cpfem_ctx* c, double* out) {
+    if (!c || !out) return CPFEM_E_ARG;
+    KinArgs K;
+    memset(&K, 0, sizeof(K));
+    K.n = c->ngps;
+    K.gp_elem = c->gp_elem;
+    K.gp_local = c->gp_local;
+    K.elem_nodes = c->elem_nodes;
+    K.elem_lt = c->elem_lt;
+    K.elem_nd = c->elem_nd;
+    K.x = c->x;
+    const int nb = nblk(c->ngps);
+    if (2 * nb > c->red.nblocks) return CPFEM_E_ARG;
+    double sums[16];
+    for (int group = 0; group < 4; ++group) {
+        if (c->npts8 > 0)
+            hipLaunchKernelGGL(k_vol_sums<8>, dim3(nb), dim3(kBlock), 0, c->stream, K, (const double*)c->st,
+                               (const double*)c->ep, (const double*)c->eps, group, c->red.partial, 0);
+        else
+            hipLaunchKernelGGL(k_fill<double>, dim3(nblk(4 * (size_t)nb)), dim3(kBlock), 0, c->stream,
+                               c->red.partial, 4 * (size_t)nb, 0.0);
+        if (c->npts20 > 0)
+            hipLaunchKernelGGL(k_vol_sums<20>, dim3(nb), dim3(kBlock), 0, c->stream, K, (const double*)c->st,
+                               (const double*)c->ep, (const double*)c->eps, group, c->red.partial, nb);
+        else
+            hipLaunchKernelGGL(k_fill<double>, dim3(nblk(4 * (size_t)nb)), dim3(kBlock), 0, c->stream,
+                               c->red.partial + 4 * (size_t)nb, 4 * (size_t)nb, 0.0);
+        CPFEM_HIP(hipGetLastError());
+        reduce_now(c, 2 * nb, sums + 4 * group);
+    }
+    const double vol = sums[0];
+    out[0] = vol;
+    for (int i = 0; i < 6; ++i) out[1 + i] = vol > 0.0 ? sums[2 + i] / vol : 0.0;
+    for (int i = 0; i < 6; ++i) out[7 + i] = vol > 0.0 ? sums[8 + i] / vol : 0.0;
+    out[13] = vol > 0.0 ? sums[1] / vol : 0.0;
     return 0;
 }
 

@@ -132,6 +132,12 @@ MODULE CPFEM_CAPI
             INTEGER(C_INT32_T), VALUE :: tflag
         END FUNCTION
 
+        INTEGER(C_INT) FUNCTION cpfem_volume_average(ctx, avg) BIND(C, name='cpfem_volume_average')
+            IMPORT :: C_INT, C_PTR, C_DOUBLE
+            TYPE(C_PTR), VALUE :: ctx
+            REAL(C_DOUBLE), INTENT(OUT) :: avg(14)
+        END FUNCTION
+
         INTEGER(C_INT) FUNCTION cpfem_cumslip(ctx, inc) BIND(C, name='cpfem_cumslip')
             IMPORT :: C_INT, C_INT32_T, C_PTR
             TYPE(C_PTR), VALUE :: ctx

@@ -169,6 +169,12 @@ int cpfem_update_gptv(cpfem_ctx *ctx);
 /* DoReorient (Fasolt3.f90:79) if the twin flag is set, then tflag = 0
  * (Fasolt3.f90:43-46); returns the count in *n. */
 int cpfem_reorient(cpfem_ctx *ctx, int32_t *n);
+/* Volume averages over all points at the current coordinates (the macroscopic
+ * stress-strain state that CSVMKR's element means feed, data_out.f90:113-235):
+ * out[0] = volume, out[1..6] = mean stress, out[7..12] = mean total strain ep
+ * (engineering shears), out[13] = mean effective strain eps.  Weights are the
+ * points' integration weights (Gauss weight x detJ). */
+int cpfem_volume_average(cpfem_ctx *ctx, double *out);
 /* Set the twin flag, the module variable tflag (fas-com-mod.f90:17) that SLPINI
  * raises (crys-cons.f90:171), RDDATA clears (Faslt_io_nowin.f90:133) and the
  * driver tests before DoReorient (Fasolt3.f90:43). */

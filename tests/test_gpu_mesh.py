@@ -239,3 +239,32 @@ def test_reorient_matches_oracle():
     assert np.array_equal(g["twinsys"], om.twinsys) and np.array_equal(g["timestwinned"], om.timestwinned)
     assert np.abs(g["sv"][:, :3] - om.sv[:, :3]).max() < 1e-12
     sv.close()
+
+
+@pytest.mark.parametrize("lt", [LT_BRICK_8GP, LT_BRICK_20])
+def test_volume_average_matches_point_sums(lt):
+    """Volume-weighted means of stress and strain over the points after two
+    increments, against weights from the oracle's shape functions at the final
+    coordinates and the downloaded point state."""
+    ctl = Control(sor_kind=1, sor_tol=1e-10, sor_omega=1.6, sor_maxit=50000)
+    prob = fcc_tension(2, ninc=3, lt=lt, seed=3, control=ctl)
+    sv = capi.Solver(prob)
+    for inc in (1, 2):
+        sv.increment(inc)
+    g = sv.get_state()
+    avg = sv.volume_average()
+    x = g["x"].reshape(-1, 3)
+    w = np.zeros(prob.ngps)
+    nn = 20 if lt == LT_BRICK_20 else 8
+    for e in range(prob.nlmnt):
+        xn = x[prob.ln[e, :nn] - 1]
+        for ig in range(prob.np_[e]):
+            w[prob.lg[e, ig] - 1] = O64.shape(int(prob.lt[e]), ig, xn)[2]
+    vol = w.sum()
+    assert abs(avg["volume"] - vol) <= 1e-12 * vol
+    assert rel_max(avg["st"], (w[:, None] * g["st"]).sum(0) / vol) < 1e-12
+    assert rel_max(avg["ep"], (w[:, None] * g["ep"]).sum(0) / vol) < 1e-12
+    assert abs(avg["eps"] - (w * g["eps"]).sum() / vol) <= 1e-12 * max(abs(avg["eps"]), 1e-30)
+    # uniaxial tension along z: the mean axial strain equals the prescribed one
+    assert abs(avg["ep"][2] - 2e-3) < 1e-4
+    sv.close()
