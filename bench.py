@@ -62,15 +62,50 @@ def load_traffic():
     return {k: v["hbm_bytes_per_launch"] for k, v in d.get("kernels", {}).items()}
 
 
-def run_path(prob, precision, warmup, steps, capi):
+def traced_increment(sv, inc, ctl):
+    """One increment through the per-step entry points (UPDATE_NODE, then ELVALS,
+    SUVALS, FACSLS and the update per Newton iteration, then the cumulative slip,
+    UPDATE_GPTV and DoReorient), with one progress line per Newton iteration.
+    Same state and stopping test as cpfem_increment; the record is the change
+    of the library's counters over the increment."""
+    before = sv.stats()
+    t0 = time.perf_counter()
+    sv.update_node()
+    nit, code = 0, 0
+    while True:
+        nit += 1
+        c1, _, _ = sv.elvals(inc)
+        if c1 < 0 or (c1 > 500 and c1 not in (987, 910)):
+            code = max(code, c1)
+            break
+        sfc = sv.suvals(inc)
+        c2, sweeps, err = sv.facsls()
+        sdx, alp2 = sv.nr_update()
+        code = max(code, c1, c2)
+        log(f"[bench]   inc {inc} iteration {nit}: sdx {sdx:.3e} sfc {sfc:.3e} alp2 {alp2:.3f} sweeps {sweeps} "
+            f"residual {err:.2e} code {max(c1, c2)}")
+        if not (sdx >= ctl.toler and nit < ctl.maxit):
+            break
+    sv.cumslip(inc)
+    sv.update_gptv()
+    sv.reorient()
+    after = sv.stats()
+    rec = {k: after[k] - before[k] for k in after if k not in ("nit", "sor_sweeps", "err", "sdx", "sfc", "sumcpiter",
+                                                                "alp2", "sor_error")}
+    rec.update(nit=nit, code=code, t_total_ms=1e3 * (time.perf_counter() - t0))
+    return rec
+
+
+def run_path(prob, precision, warmup, steps, capi, trace=False):
     t0 = time.time()
     sv = capi.Solver(prob, precision=precision)
     log(f"[bench] precision={precision} setup {time.time() - t0:.1f} s")
     import torch
+    step = (lambda i: traced_increment(sv, i, prob.control)) if trace else sv.increment
     inc = 0
     for _ in range(warmup):
         inc += 1
-        st = sv.increment(inc)
+        st = step(inc)
         log(f"[bench] warmup inc {inc}: nit {st['nit']} sweeps {st['sor_sweeps_total']} "
             f"{st['t_total_ms']:.1f} ms code {st['code']}")
     torch.cuda.synchronize()
@@ -78,7 +113,7 @@ def run_path(prob, precision, warmup, steps, capi):
     t0 = time.perf_counter()
     for _ in range(steps):
         inc += 1
-        st = sv.increment(inc)
+        st = step(inc)
         recs.append(st)
         log(f"[bench] inc {inc}: nit {st['nit']} sweeps {st['sor_sweeps_total']} total {st['t_total_ms']:.1f} ms "
             f"const {st['t_const_ms']:.1f} asm {st['t_asm_ms']:.1f} sor {st['t_sor_ms']:.1f} code {st['code']} "
@@ -121,9 +156,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--n", type=int, default=None, help="mesh size (default: the workload's)")
+    ap.add_argument("--workload", default="plane_strain",
+                    choices=["plane_strain", "tension", "hcp_compression"],
+                    help="plane_strain: FCC 64^3 plane-strain compression (the headline line); tension: FCC "
+                         "uniaxial tension (32^3 FP64 drop-in case); hcp_compression: Mg-like HCP with twins (48^3)")
     ap.add_argument("--paths", default="fp64,mixed")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--trace-iterations", action="store_true",
+                    help="drive each increment through the per-step calls and print every Newton iteration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md, "
@@ -132,18 +173,32 @@ def main():
     import torch
     assert torch.cuda.is_available(), "no GPU"
     from cpfem_code_amd import capi
-    from cpfem_code_amd.problem import Control, fcc_plane_strain
+    from cpfem_code_amd.problem import Control, fcc_plane_strain, fcc_tension, hcp_compression
     name, cus = capi.device_info()
     ctl = Control(sor_kind=1, sor_tol=1e-8, sor_omega=1.6, sor_maxit=200000)
     t0 = time.time()
-    prob = fcc_plane_strain(args.n, ninc=args.warmup + args.steps + 1, seed=2, control=ctl)
+    ninc = args.warmup + args.steps + 1
+    if args.workload == "plane_strain":
+        args.n = args.n or 64
+        prob = fcc_plane_strain(args.n, ninc=ninc, seed=2, control=ctl)
+        wl = f"FCC Al plane-strain compression, {args.n}^3 8-node bricks x 8 points, eps_zz=-1e-3/increment"
+    elif args.workload == "tension":
+        args.n = args.n or 32
+        prob = fcc_tension(args.n, ninc=ninc, seed=0, control=ctl)
+        wl = f"FCC Al uniaxial tension along z, {args.n}^3 8-node bricks x 8 points, eps_zz=1e-3/increment"
+    else:
+        args.n = args.n or 48
+        prob = hcp_compression(args.n, ninc=ninc, seed=3, control=ctl)
+        wl = (f"Mg-like HCP (12 slip + 6 twin systems, twinprob 1) compression along x, basal fibre, "
+              f"{args.n}^3 8-node bricks x 8 points, eps_xx=-1e-3/increment")
+    wl += ", SOR omega 1.6 to 1e-8 relative residual"
     log(f"[bench] {name} {cus} CUs; problem {args.n}^3: {prob.nlmnt} elements, {prob.ngps} points, "
         f"{prob.ndof} DOFs (generated in {time.time() - t0:.1f} s)")
     results = {}
     last_state = None
     for path in args.paths.split(","):
         prec = 0 if path == "fp64" else 1
-        recs, wall, state = run_path(prob, prec, args.warmup, args.steps, capi)
+        recs, wall, state = run_path(prob, prec, args.warmup, args.steps, capi, trace=args.trace_iterations)
         results[path] = (recs, wall)
         if prec == 0:
             last_state = state
@@ -207,9 +262,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded mesh, texture and load path; no external data)",
-        "config": {"workload": f"FCC Al plane-strain compression, {args.n}^3 8-node bricks x 8 points, "
-                               f"eps_zz=-1e-3/increment, SOR omega 1.6 to 1e-8 relative residual",
-                   "elements": prob.nlmnt, "points": prob.ngps, "dofs": prob.ndof},
+        "config": {"workload": wl, "elements": prob.nlmnt, "points": prob.ngps, "dofs": prob.ndof},
         "s_per_increment_fp64": wall64 / args.steps,
         "gp_updates_per_s_constitutive": gp_updates / t_const,
         "nr_iterations_per_increment": nit / args.steps,

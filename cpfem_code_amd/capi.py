@@ -179,6 +179,7 @@ def _bind_mesh_api(L):
     L.cpfem_update_gptv.argtypes = [_CTX]
     L.cpfem_reorient.argtypes = [_CTX, _P]
     L.cpfem_set_tflag.argtypes = [_CTX, C.c_int32]
+    L.cpfem_cumslip.argtypes = [_CTX, C.c_int32]
     L.cpfem_volume_average.argtypes = [_CTX, _P]
     L.cpfem_increment.argtypes = [_CTX, C.c_int32, C.POINTER(CpfemStats)]
     L.cpfem_get_system.argtypes = [_CTX, _P, _P]
@@ -314,6 +315,9 @@ class Solver:
         d = st.as_dict()
         d["code"] = code
         return d
+
+    def cumslip(self, inc):
+        _check(self.L.cpfem_cumslip(self.h, inc), "cpfem_cumslip")
 
     def update_gptv(self):
         _check(self.L.cpfem_update_gptv(self.h), "cpfem_update_gptv")
