@@ -117,10 +117,11 @@ def read_frs(path: str) -> Dict[str, np.ndarray]:
     nlmnt, nnod, ngps, nsv = (int(v) for v in r.ints())
     out.update(nlmnt=nlmnt, nnod=nnod, ngps=ngps, nsv=nsv)
     le, lt = np.zeros(nlmnt, np.int32), np.zeros(nlmnt, np.int32)
+    nde, npe = np.zeros(nlmnt, np.int32), np.zeros(nlmnt, np.int32)
     ln, lg = np.zeros((nlmnt, 20), np.int32), np.zeros((nlmnt, 8), np.int32)
     for e in range(nlmnt):
         a = r.ints()
-        le[e], lt[e] = a[0], a[1]
+        le[e], lt[e], nde[e], npe[e] = a[0], a[1], a[2], a[3]
         ln[e, :a[2]] = r.ints()
         lg[e, :a[3]] = r.ints()
     ib = np.zeros(nnod, np.int32)
@@ -140,6 +141,22 @@ def read_frs(path: str) -> Dict[str, np.ndarray]:
         for k in ("ep", "de", "st", "ds", "rt", "dr"):
             st[k][g] = r.reals()
     out.update(le=le, lt=lt, ln=ln, lg=lg, ib=ib, x=x, dx=dx, fc=fc, eps=eps, dep=dep, sv=sv, dsv=dsv, **st)
+    # element node and point counts, then the material records (Faslt_io_nowin.f90:77-107)
+    out["nd"], out["np"] = nde, npe
+    nmatl, nmpar = (int(v) for v in r.ints())
+    mats = []
+    for _ in range(nmatl):
+        n = int(r.ints()[0])
+        twinlim = int(r.ints()[0])
+        twinprob = float(r.reals()[0])
+        e = r.reals()
+        ssang = r.reals()
+        abn = np.array([r.reals() for _ in range(n)])
+        ampar = np.array([r.reals() for _ in range(n)])
+        smatrx = np.array([r.reals() for _ in range(n)])
+        mats.append(dict(nss=n, twinlim=twinlim, twinprob=twinprob, e=e, ssang=ssang, ab=abn[:, :3], an=abn[:, 3:],
+                         ampar=ampar, smatrx=smatrx))
+    out["materials"] = mats
     return out
 
 

@@ -1,0 +1,301 @@
+"""Post-processing of FASOLT3 results: the tables of the reference's CSVMKR.
+
+CSVMKR (src/csvmkr) reads a .frs file and writes either one row per
+integration point (GPSTATE, data_out.f90:7-100) or one row of element means
+(ELSTATE, data_out.f90:113-235).  This module computes the same rows from a
+state read with :func:`cpfem_code_amd.fsd.read_frs` (or downloaded from the
+device) and writes them in CSVMKR's layout: the two header lines, then per
+row an I5 label and comma-separated G13.5 values.
+
+Row contents (48 values):
+  weight, point coordinates, st(6), ds(6), ep(6), de(6), rt(3), dr(3),
+  Euler angles (3) and their increments (3), elastic strain from the stress
+  through the compliance of the orientation (ELASEPS, data_out.f90:262-358),
+  effective strain eps, von Mises stress.
+Element rows use the element volume (sum of point weights), the centroid of
+the nodes, the plain mean over the element's points of the state arrays, the
+quaternion mean of the orientations and of the orientation increments
+(QTNEUL, QTNNRM, EULQTN, fspl_orir.f90:31-125), and end with the material
+code.  Elements without points (ties) are dropped and the rest renumbered
+as CSVMKR's reader does it (data_in.f90:44-62, :func:`kept_elements`).
+
+Two properties of the reference are kept because they define its numbers:
+the point weights of one-point bricks are 8 detJ at the centre and those of
+20-node bricks use SHAPE_22's 6-decimal tables (WEIGHT_4, WEIGHT_2), and the
+von Mises stress of GPSTATE takes its fifth component from the point numbered
+(element nodes + 1) for an element's first point and from point 7 for the
+others (the loop variable left over in data_out.f90:81).  The elastic strain
+is ELASEPS's: its normal-normal compliance block is the rotated crystal
+compliance, its normal-shear terms are half of that, so the columns are the
+reference's rather than the rotated-compliance strain.  Element type 13
+(8-node brick, 2x2x2 points) is not known to CSVMKR; its weights here are the
+solver's (detJ at +-1/sqrt(3)).  CSVMKR's reader has no read for the
+slip-system angles record of a material (data_in.f90:94-110) and so fails on
+files with more than one material; this module reads every material and uses
+the element's own.
+
+Input: the dict of :func:`cpfem_code_amd.fsd.read_frs`, or
+:func:`state_tables` for a problem and a state pulled from the device
+(``capi.Solver.get_state``).
+"""
+from __future__ import annotations
+
+import math
+from typing import Dict, List, Sequence
+
+import numpy as np
+
+from .problem import BRICK20
+
+GP_HEADER = (
+    "FAFNER3 Gauss point state values",
+    "int. point,volume, x, y, z, stress(11),stress(22),stress(33),stress(23),stress(31),stress(12),"
+    "inc. s(11),inc. s(22),inc. s(33),inc. s(23),inc. s(31),inc. s(12),"
+    "strain(11),strain(22),strain(33),strain(23),strain(31),strain(12),"
+    "inc. e(11),inc. e(22),inc. e(33),inc. e(23),inc. e(31),inc. e(12),"
+    "rotation(1),rotation(2),rotation(3),inc. rot.(1),inc. rot.(2),inc. rot.(3),"
+    "phi1,Phi,phi2,inc. phi1,inc. Phi,inc. phi22,"
+    "el_eps(11), el_eps(22), el_eps(33), el_eps(23), el_eps(31), el_eps(12),vM eps,vM",
+)
+EL_HEADER = (
+    "FAFNER3 element mean state values",
+    "element, volume, x, y, z,stress(11),stress(22),stress(33),stress(23),stress(31),stress(12),"
+    "inc. s(11),inc. s(22),inc. s(33),inc. s(23),inc. s(31),inc. s(12),"
+    "strain(11),strain(22),strain(33),strain(23),strain(31),strain(12),"
+    "inc. e(11),inc. e(22),inc. e(33),inc. e(23),inc. e(31),inc. e(12),"
+    "rotation(1),rotation(2),rotation(3),inc. rot.(1),inc. rot.(2),inc. rot.(3),"
+    "phi1,Phi,phi2,inc. phi1,inc. Phi,inc. phi22,"
+    "el_eps(11), el_eps(22), el_eps(33), el_eps(23), el_eps(31), el_eps(12),vM eps,vM sig, phase",
+)
+
+_SIGN8 = np.array([(-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1),
+                   (-1, -1, 1), (1, -1, 1), (1, 1, 1), (-1, 1, 1)], float)
+_LOC20 = np.array(BRICK20, float)
+
+
+def _brick8(xi):
+    f = 1.0 + _SIGN8 * xi[None, :]
+    n = f.prod(1) / 8.0
+    d = np.stack([_SIGN8[:, 0] * f[:, 1] * f[:, 2], f[:, 0] * _SIGN8[:, 1] * f[:, 2],
+                  f[:, 0] * f[:, 1] * _SIGN8[:, 2]]) / 8.0
+    return n, d
+
+
+def _brick20_tables():
+    """SHAPE_22 / WEIGHT_2 tables: serendipity functions at +-g, g = 1/sqrt(3) in
+    single precision, rounded to 6 decimals (this reproduces the tabulated values)."""
+    g = float(np.float32(1.0 / math.sqrt(3.0)))
+    tn, td = np.zeros((8, 20)), np.zeros((8, 3, 20))
+    for ig in range(8):
+        xi = _SIGN8[ig] * g
+        for a, c in enumerate(_LOC20):
+            if np.all(c != 0):
+                f = 1.0 + xi * c
+                s = float(xi @ c) - 2.0
+                n = f.prod() * s / 8.0
+                d = [c[k] * f[(k + 1) % 3] * f[(k + 2) % 3] * s / 8.0 + f.prod() * c[k] / 8.0 for k in range(3)]
+            else:
+                z = int(np.where(c == 0)[0][0])
+                p, q = (z + 1) % 3, (z + 2) % 3
+                qz, fp, fq = 1.0 - xi[z] ** 2, 1.0 + xi[p] * c[p], 1.0 + xi[q] * c[q]
+                n = qz * fp * fq / 4.0
+                d = [0.0, 0.0, 0.0]
+                d[z] = -2.0 * xi[z] * fp * fq / 4.0
+                d[p] = qz * c[p] * fq / 4.0
+                d[q] = qz * fp * c[q] / 4.0
+            tn[ig, a] = round(n * 1e6) / 1e6
+            td[ig, :, a] = [round(v * 1e6) / 1e6 for v in d]
+    return tn, td
+
+
+_TAB20 = None
+
+
+def point_weight(lt: int, ig: int, xnode: np.ndarray):
+    """(weight, point coordinates) of point ig (0-based) of an element with node
+    coordinates xnode[nn][3] (WEIGHT_4 for lt 12, WEIGHT_2 for lt 22)."""
+    global _TAB20
+    if lt == 12:
+        n, d = _brick8(np.zeros(3))
+        scale = 8.0
+    elif lt == 13:
+        n, d = _brick8(_SIGN8[ig] / math.sqrt(3.0))
+        scale = 1.0
+    elif lt == 22:
+        if _TAB20 is None:
+            _TAB20 = _brick20_tables()
+        n, d = _TAB20[0][ig], _TAB20[1][ig]
+        scale = 1.0
+    else:
+        raise ValueError(f"element type {lt} not supported")
+    jac = d @ xnode            # jac[j, k] = sum_a dN_a/dxi_j x_a,k
+    return scale * float(np.linalg.det(jac)), n @ xnode
+
+
+def elastic_strain(e9: Sequence[float], ph1: float, phi: float, ph2: float, stress: np.ndarray) -> np.ndarray:
+    """ELASEPS (data_out.f90:262-358): compliance in specimen axes (engineering
+    shears) from the crystal compliances e(1:9) and Bunge angles, times the stress."""
+    c1, s1, c2, s2, c, s = math.cos(ph1), math.sin(ph1), math.cos(ph2), math.sin(ph2), math.cos(phi), math.sin(phi)
+    a = np.array([[c1 * c2 - s1 * s2 * c, -c1 * s2 - s1 * c2 * c, s1 * s],
+                  [s1 * c2 + c1 * s2 * c, -s1 * s2 + c1 * c2 * c, -c1 * s],
+                  [s2 * s, c2 * s, c]])
+    e = np.asarray(e9, float)
+    pairs = [(0, 0), (1, 1), (2, 2), (1, 2), (2, 0), (0, 1)]   # Voigt index -> tensor index pair
+    comp = np.zeros((6, 6))
+    for iset in range(6):
+        i, j = pairs[iset]
+        mult = 1.0 if iset < 3 else 2.0
+        for jset in range(iset, 6):
+            k, l = pairs[jset]
+            t1 = sum(a[i, m] * a[j, m] * a[k, m] * a[l, m] * e[m] for m in range(3))
+            t2 = (a[i, 1] * a[j, 1] * a[k, 2] * a[l, 2] * e[3] + a[i, 2] * a[j, 2] * a[k, 1] * a[l, 1] * e[3]
+                  + a[i, 2] * a[j, 2] * a[k, 0] * a[l, 0] * e[4] + a[i, 0] * a[j, 0] * a[k, 2] * a[l, 2] * e[4]
+                  + a[i, 0] * a[j, 0] * a[k, 1] * a[l, 1] * e[5] + a[i, 1] * a[j, 1] * a[k, 0] * a[l, 0] * e[5])
+            t3 = (a[i, 1] * a[j, 2] * a[k, 1] * a[l, 2] * e[6] + a[i, 2] * a[j, 1] * a[k, 2] * a[l, 1] * e[6]
+                  + a[i, 2] * a[j, 0] * a[k, 2] * a[l, 0] * e[7] + a[i, 0] * a[j, 2] * a[k, 0] * a[l, 2] * e[7]
+                  + a[i, 0] * a[j, 1] * a[k, 0] * a[l, 1] * e[8] + a[i, 1] * a[j, 0] * a[k, 1] * a[l, 0] * e[8])
+            comp[iset, jset] = mult * (t1 + t2 + 0.5 * t3)
+            comp[jset, iset] = comp[iset, jset]
+    return comp @ np.asarray(stress, float)
+
+
+def von_mises(s: Sequence[float]) -> float:
+    """VONM_STRESS (data_out.f90:848-859)."""
+    h = (s[0] + s[1] + s[2]) / 3.0
+    d = np.array([s[0] - h, s[1] - h, s[2] - h])
+    return math.sqrt(1.5 * (float(d @ d) + 2.0 * (s[3] ** 2 + s[4] ** 2 + s[5] ** 2)))
+
+
+def quaternion(ph1: float, phi: float, ph2: float) -> np.ndarray:
+    """QTNEUL: Euler symmetrics of Bunge angles, in the positive hemisphere."""
+    ch, sh = math.cos(phi / 2.0), math.sin(phi / 2.0)
+    x = np.array([ch * math.cos((ph1 + ph2) / 2.0), -sh * math.cos((ph2 - ph1) / 2.0),
+                  sh * math.sin((ph2 - ph1) / 2.0), -ch * math.sin((ph1 + ph2) / 2.0)])
+    return -x if x[0] < 0.0 else x
+
+
+def euler(a: np.ndarray):
+    """EULQTN: Bunge angles of Euler symmetrics a(0:3)."""
+    cphi = 1.0 - 2.0 * (a[1] * a[1] + a[2] * a[2])
+    phi = math.acos(cphi) if cphi > -1.0 else 3.14159265359
+    if abs(cphi) >= 1.0:
+        ph1 = 0.5 * math.atan2(2.0 * (a[1] * a[2] - a[0] * a[3]), 1.0 - 2.0 * (a[2] * a[2] + a[3] * a[3]))
+        ph2 = ph1
+    else:
+        ph1 = math.atan2(a[1] * a[3] - a[0] * a[2], -(a[2] * a[3] + a[0] * a[1]))
+        ph2 = math.atan2(a[1] * a[3] + a[0] * a[2], a[2] * a[3] - a[0] * a[1])
+    return ph1, phi, ph2
+
+
+def kept_elements(npe: Sequence[int]) -> List[int]:
+    """Element numbers (0-based, in output order) that CSVMKR keeps after dropping
+    the elements without points (data_in.f90:44-62).
+
+    The reference shifts the remaining elements down over a dropped one only
+    while it is not the last of the shortened list, so a pointless element that
+    ends up second to last takes the element after it out with it."""
+    idx = list(range(len(npe)))
+    n, i = len(idx), 0
+    while i < n:
+        if int(npe[idx[i]]) == 0:
+            n -= 1
+            if i < n - 1:
+                idx[i:n] = idx[i + 1:n + 1]
+        else:
+            i += 1
+    return idx[:n]
+
+
+def _element(d: Dict, e: int):
+    nn, ng, lt, iel = int(d["nd"][e]), int(d["np"][e]), int(d["lt"][e]), int(d["le"][e])
+    if not 1 <= iel <= len(d["materials"]):
+        raise ValueError(f"element {e + 1}: material code {iel} has no material record")
+    xl = np.asarray(d["x"]).reshape(-1, 3)[np.asarray(d["ln"][e, :nn]) - 1]
+    return nn, ng, lt, iel, xl, np.asarray(d["lg"][e, :ng]) - 1
+
+
+def gpstate(d: Dict) -> List[tuple]:
+    """GPSTATE rows (point number, 48 values) of a state dict from read_frs."""
+    st = np.asarray(d["st"])
+    rows = []
+    for e in kept_elements(d["np"]):
+        nn, ng, lt, iel, xl, gps = _element(d, e)
+        e9 = d["materials"][iel - 1]["e"]
+        for ig, gp in enumerate(gps):
+            w, xg = point_weight(lt, ig, xl)
+            strain = elastic_strain(e9, *d["sv"][gp, :3], st[gp])
+            # fifth stress component: point nn+1 for the first point, point 7 after
+            # (data_out.f90:81-82); points past the file's last hold zero
+            k = nn if ig == 0 else 6
+            s5 = st[k, 4] if k < len(st) else 0.0
+            vm = von_mises([st[gp, 0], st[gp, 1], st[gp, 2], st[gp, 3], s5, st[gp, 5]])
+            vals = ([w], xg, st[gp], d["ds"][gp], d["ep"][gp], d["de"][gp], d["rt"][gp], d["dr"][gp],
+                    d["sv"][gp, :3], d["dsv"][gp, :3], strain, [d["eps"][gp]], [vm])
+            rows.append((int(gp) + 1, np.concatenate([np.ravel(v) for v in vals])))
+    return rows
+
+
+def elstate(d: Dict) -> List[tuple]:
+    """ELSTATE rows (element number, 48 values, material code) of a state dict."""
+    rows = []
+    for k, e in enumerate(kept_elements(d["np"])):
+        nn, ng, lt, iel, xl, gps = _element(d, e)
+        vol = sum(point_weight(lt, ig, xl)[0] for ig in range(ng))
+        means = [np.asarray(d[key])[gps].mean(0) for key in ("st", "ds", "ep", "de", "rt", "dr")]
+        q = sum(quaternion(*d["sv"][g, :3]) for g in gps)
+        dq = sum(quaternion(*d["dsv"][g, :3]) for g in gps)
+        ph = euler(q / math.sqrt(float(q @ q)))
+        dph = euler(dq / math.sqrt(float(dq @ dq)))
+        strain = elastic_strain(d["materials"][iel - 1]["e"], *ph, means[0])
+        vals = [[vol], xl.sum(0) / nn, *means, ph, dph, strain, [np.asarray(d["eps"])[gps].mean()],
+                [von_mises(means[0])]]
+        rows.append((k + 1, np.concatenate([np.ravel(v) for v in vals]), iel))
+    return rows
+
+
+def state_tables(prob, state: Dict) -> Dict:
+    """The dict gpstate/elstate read, from a Problem and a state dict (x, dx, eps,
+    sv, dsv, ep, de, st, ds, rt, dr as capi.Solver.get_state returns them)."""
+    d = {k: np.asarray(state[k]) for k in ("x", "eps", "sv", "dsv", "ep", "de", "st", "ds", "rt", "dr")}
+    d.update(nlmnt=prob.nlmnt, nd=prob.nd, np=prob.np_, lt=prob.lt, le=prob.le, ln=prob.ln, lg=prob.lg,
+             materials=[{"e": np.asarray(m.e, float)} for m in prob.materials])
+    return d
+
+
+def g13_5(v: float) -> str:
+    """Fortran G13.5: F9.(5-k) and four blanks when the value rounded to five
+    significant digits has decimal exponent k in 0..5 (0.1 <= |v| < 1e5), else
+    E13.5; zero prints as F9.4."""
+    v = float(v)
+    if v == 0.0:
+        return f"{0.0:9.4f}    "
+    mant, ex = f"{abs(v):.4e}".split("e")          # rounded to five significant digits
+    k = int(ex) + 1                                  # exponent of the 0.ddddd form
+    if 0 <= k <= 4:
+        return f"{v:9.{5 - k}f}    "
+    if k == 5:
+        return f"{v:8.0f}.    "                       # F9.0 keeps the decimal point
+    digits = mant.replace(".", "")
+    return f"{'-' if v < 0 else ' '}0.{digits}E{'-' if k < 0 else '+'}{abs(k):02d}".rjust(13)
+
+
+def write_csv(path: str, rows: List[tuple], element: bool) -> None:
+    """Write GPSTATE (element=False) or ELSTATE rows in CSVMKR's layout."""
+    with open(path, "w") as fh:
+        for line in (EL_HEADER if element else GP_HEADER):
+            fh.write(line + "\n")
+        for r in rows:
+            text = f"{r[0]:5d}" + "".join("," + g13_5(v) for v in r[1])
+            if element:
+                text += f",{r[2]:5d}"
+            fh.write(text + "\n")
+
+
+def read_csv(path: str) -> np.ndarray:
+    """Values of a CSVMKR-layout file (labels and values of every row) as floats."""
+    out = []
+    with open(path) as fh:
+        for line in list(fh)[2:]:
+            if line.strip():
+                out.append([float(t) for t in line.split(",")])
+    return np.array(out)

@@ -179,3 +179,24 @@ def increments(p, state, nsteps):
         s["err"] = err
         res.append(s)
     return res
+
+
+FASOLT3 = os.path.join(REPO, "oracle", "_ref", "fasolt3")
+CSVMKR = os.path.join(REPO, "oracle", "_ref", "csvmkr")
+
+
+def fasolt3(workdir: str, name: str, save: int) -> None:
+    """Run the reference driver on workdir/name.fsd (with fafcdf.dat and dirname
+    there) with one intermediate save (Fasolt3.f90:28-32): it writes
+    name-<save>.frs and, at the end, name-<last increment>.frs."""
+    subprocess.run([FASOLT3], input=f"{name}\n1\n{save}\n\n", text=True, check=True, cwd=workdir,
+                   capture_output=True, timeout=600)
+
+
+def csvmkr(workdir: str, frs: str, element: bool, out: str) -> str:
+    """Run CSVMKR on workdir/frs.frs; element means (mode 1) or point values (mode 2).
+    Returns the text of the .csv it writes."""
+    subprocess.run([CSVMKR], input=f"{frs}\n{1 if element else 2}\n{out}\n", text=True, check=True,
+                   cwd=workdir, capture_output=True, timeout=120)
+    with open(os.path.join(workdir, out + ".csv")) as fh:
+        return fh.read()
