@@ -1114,7 +1114,9 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
     for (int k = 1; k < nmat; ++k)
         if (mats[k].nss != nss) return CPFEM_E_ARG;
     if (material_bytes(nss) == 0) return CPFEM_E_NSS;
-    if (m->nsv != 3 + nss) return CPFEM_E_ARG;
+    // sv(1:3) Euler angles, sv(4:3+nss) slip resistances; FS3PRE may declare
+    // more state variables (fs3p-gps.f90:30-31), which are carried unchanged
+    if (m->nsv < 3 + nss || m->nsv > 63) return CPFEM_E_ARG;
     if (ctl->precision == 1 && ctl->sor_kind != 1) return CPFEM_E_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CPFEM_E_NOGPU;

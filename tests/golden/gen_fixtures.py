@@ -13,6 +13,9 @@ Fixtures:
   nrstep_fcc.npz  ELVALS+SUVALS then FACSLS on a 3^3 one-point brick mesh
   nrstep20_fcc.npz  the same step on a 2^3 mesh of 20-node bricks (SHAPE_22)
   incs_fcc.npz    two increments of the reference loop on a 2^3 one-point mesh
+  fs3pre.npz      .fsd files the reference FS3PRE wrote for five answer sets
+                  (8- and 20-node bricks, the four boundary options, ties),
+                  with the .fmt and .ori inputs they read
   csvmkr.npz      CSVMKR tables (point and element mode) of two .frs files: the
                   reference driver's state after increment 2 of a 2^3 one-point
                   FCC tension run, and a synthetic state on a 2^3 mesh of
@@ -103,6 +106,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "incs_fcc.npz"), **arrs)
 
     csvmkr_fixture()
+    fs3pre_fixture()
 
 
 def synthetic_frs(path):
@@ -150,6 +154,59 @@ def csvmkr_fixture():
             out[f"{tag}_gp"] = np.array(refio.csvmkr(td, frs, False, f"{tag}_gp"))
             out[f"{tag}_el"] = np.array(refio.csvmkr(td, frs, True, f"{tag}_el"))
     np.savez_compressed(os.path.join(HERE, "csvmkr.npz"), **out)
+
+
+def _fmt_text(m):
+    """A material as .fmt text (nmpar 6), the layout FS3PRE reads (fs3p-matl.f90:67-103)."""
+    n = m.nss
+    rows = ["6", str(n), " ".join(f"{v:.7e}" for v in m.e), " ".join(f"{v:.4f}" for v in m.ssang[:n])]
+    rows += [" ".join(f"{v:.6f}" for v in list(m.ab[i]) + list(m.an[i])) for i in range(n)]
+    rows += [" ".join(f"{v:.6g}" for v in m.ampar[i, :6]) for i in range(n)]
+    rows += [" ".join(f"{v:.6g}" for v in m.smatrx[i, :n]) for i in range(n)]
+    return "\n".join(rows) + "\n"
+
+
+# answer sets in FS3PRE's order: divisions, lengths, element set, boundary
+# option, slip systems, orientation option 4, sets and fractions, .ori name,
+# materials, .fmt name, slip resistances, elastic fraction, strain
+# increments, increments, twin limit and probability, ties, output name
+FS3PRE_CASES = {
+    "b8_x2": "3\n2\n2\n1.5\n1.0\n1.0\n1\n2\n12\n4\n1\n1\n\npts\n1\n\nal\ny\n16.0\nn\n"
+             "0.001 -0.001\n3\n1\n1.0\nn\nn\nn\nb8_x2\n",
+    "b8_x0_ties": "3\n3\n2\n1.0\n1.0\n0.7\n1\n0\n12\n4\n1\n1\n\npts\n1\n\nal\ny\n20.0\nn\n"
+                  "0.001 0.0 -0.001\n2\n1\n1.0\ny\ny\nn\nb8_x0_ties\n",
+    "b8_x1_hcp": "2\n2\n3\n1.0\n1.0\n1.5\n1\n1\n21\n4\n1\n1\n\npts\n1\n\nmg\nn\n"
+                 "5 5 5 40 40 40 80 80 80 80 80 80 12 12 12 12 12 12\nn\n0.0 0.0 -0.001\n4\n1\n1.0\n"
+                 "n\nn\nn\nb8_x1_hcp\n",
+    "b20_x3_ties": "2\n2\n2\n1.0\n1.0\n1.0\n2\n3\n12\n4\n1\n1\n\npts\n1\n\nal\nn\n"
+                   "16 16 16 17 17 17 18 18 18 19 19 19\nn\n-0.001\n2\n0\n0.5\ny\nn\ny\nb20_x3_ties\n",
+    "b20_x2": "2\n2\n1\n1.0\n1.0\n0.5\n2\n2\n12\n4\n1\n1\n\npts\n1\n\nal\ny\n20.0\nn\n"
+              "0.0005 0.0005\n3\n1\n1.0\nn\nn\nn\nb20_x2\n",
+}
+
+
+def fs3pre_fixture():
+    import subprocess
+    rng = np.random.default_rng(31)
+    ori = "".join(f"   1.0000000e+00   {rng.uniform(0, 6.2832):.7e}   {np.arccos(rng.uniform(-1, 1)):.7e}   "
+                  f"{rng.uniform(0, 6.2832):.7e}\n" for _ in range(400))
+    with open(os.path.join(refio.REPO, "..", "reference", "sample_input_files", "aluminium2.fmt")) as fh:
+        al = fh.read()
+    files = {"al.fmt": al, "mg.fmt": _fmt_text(hcp_magnesium()), "pts.ori": ori}
+    out = {f"file_{k}": np.array(v) for k, v in files.items()}
+    for name, answers in FS3PRE_CASES.items():
+        with tempfile.TemporaryDirectory() as td:
+            for k, v in files.items():
+                with open(os.path.join(td, k), "w") as fh:
+                    fh.write(v)
+            with open(os.path.join(td, "dirname"), "w") as fh:
+                fh.write("'./' './' './'\n")
+            subprocess.run([refio.FS3PRE], input=answers, text=True, check=True, cwd=td, capture_output=True,
+                           timeout=120)
+            with open(os.path.join(td, name + ".fsd"), "rb") as fh:
+                out[f"{name}_fsd"] = np.frombuffer(fh.read(), np.uint8)
+        out[f"{name}_answers"] = np.array(answers)
+    np.savez_compressed(os.path.join(HERE, "fs3pre.npz"), **out)
 
 
 if __name__ == "__main__":

@@ -183,6 +183,7 @@ def increments(p, state, nsteps):
 
 FASOLT3 = os.path.join(REPO, "oracle", "_ref", "fasolt3")
 CSVMKR = os.path.join(REPO, "oracle", "_ref", "csvmkr")
+FS3PRE = os.path.join(REPO, "oracle", "_ref", "fs3pre")
 
 
 def fasolt3(workdir: str, name: str, save: int) -> None:
