@@ -16,6 +16,9 @@ Fixtures:
   fs3pre.npz      .fsd files the reference FS3PRE wrote for five answer sets
                   (8- and 20-node bricks, the four boundary options, ties),
                   with the .fmt and .ori inputs they read
+  pipeline.npz    the reference chain on answer set b8_x2: FS3PRE's .fsd, then
+                  fasolt3 (three increments, reference solver settings, the
+                  sample fafcdf.dat) and CSVMKR's element table of the last .frs
   csvmkr.npz      CSVMKR tables (point and element mode) of two .frs files: the
                   reference driver's state after increment 2 of a 2^3 one-point
                   FCC tension run, and a synthetic state on a 2^3 mesh of
@@ -107,6 +110,7 @@ def main():
 
     csvmkr_fixture()
     fs3pre_fixture()
+    pipeline_fixture()
 
 
 def synthetic_frs(path):
@@ -207,6 +211,30 @@ def fs3pre_fixture():
                 out[f"{name}_fsd"] = np.frombuffer(fh.read(), np.uint8)
         out[f"{name}_answers"] = np.array(answers)
     np.savez_compressed(os.path.join(HERE, "fs3pre.npz"), **out)
+
+
+def pipeline_fixture():
+    """FS3PRE -> fasolt3 -> CSVMKR on answer set b8_x2."""
+    import subprocess
+    z = np.load(os.path.join(HERE, "fs3pre.npz"))
+    with tempfile.TemporaryDirectory() as td:
+        for k in z.keys():
+            if k.startswith("file_"):
+                with open(os.path.join(td, k[5:]), "w") as fh:
+                    fh.write(str(z[k]))
+        with open(os.path.join(td, "dirname"), "w") as fh:
+            fh.write("'./' './' './' './' './'\n")
+        with open(os.path.join(refio.REPO, "..", "reference", "sample_input_files", "fafcdf.dat")) as fh:
+            fafcdf = fh.read()
+        with open(os.path.join(td, "fafcdf.dat"), "w") as fh:
+            fh.write(fafcdf)
+        subprocess.run([refio.FS3PRE], input=str(z["b8_x2_answers"]), text=True, check=True, cwd=td,
+                       capture_output=True, timeout=120)
+        refio.fasolt3(td, "b8_x2", 2)
+        with open(os.path.join(td, "b8_x2-3.frs"), "rb") as fh:
+            frs = np.frombuffer(fh.read(), np.uint8)
+        el = refio.csvmkr(td, "b8_x2-3", True, "el")
+    np.savez_compressed(os.path.join(HERE, "pipeline.npz"), frs=frs, el=np.array(el), fafcdf=np.array(fafcdf))
 
 
 if __name__ == "__main__":
