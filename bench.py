@@ -90,8 +90,10 @@ def traced_increment(sv, inc, ctl):
     sv.update_gptv()
     sv.reorient()
     after = sv.stats()
+    keep = ("sor_omega", "sor_wpb")
     rec = {k: after[k] - before[k] for k in after if k not in ("nit", "sor_sweeps", "err", "sdx", "sfc", "sumcpiter",
-                                                                "alp2", "sor_error")}
+                                                                "alp2", "sor_error") + keep}
+    rec.update({k: after[k] for k in keep})
     rec.update(nit=nit, code=code, t_total_ms=1e3 * (time.perf_counter() - t0))
     return rec
 
@@ -103,11 +105,14 @@ def run_path(prob, precision, warmup, steps, capi, trace=False):
     import torch
     step = (lambda i: traced_increment(sv, i, prob.control)) if trace else sv.increment
     inc = 0
+    warm = []
     for _ in range(warmup):
         inc += 1
         st = step(inc)
+        warm.append(st)
         log(f"[bench] warmup inc {inc}: nit {st['nit']} sweeps {st['sor_sweeps_total']} "
-            f"{st['t_total_ms']:.1f} ms code {st['code']}")
+            f"{st['t_total_ms']:.1f} ms code {st['code']} omega {st['sor_omega']:.3g} waves/block {st['sor_wpb']} "
+            f"tune {st['t_tune_ms']:.1f} ms ({st['tune_sweeps']} sweeps)")
     torch.cuda.synchronize()
     recs = []
     t0 = time.perf_counter()
@@ -123,7 +128,7 @@ def run_path(prob, precision, warmup, steps, capi, trace=False):
     wall = time.perf_counter() - t0
     state = sv.get_state()
     sv.close()
-    return recs, wall, state
+    return recs, wall, state, warm
 
 
 def cpu_baseline(prob, state, seconds, capi_mod):
@@ -163,9 +168,10 @@ def main():
                          "uniaxial tension (32^3 FP64 drop-in case); hcp_compression: Mg-like HCP with twins (48^3)")
     ap.add_argument("--paths", default="fp64,mixed")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--omega-variant", type=float, default=1.9,
-                    help="also run the mixed path with this SOR omega (reported beside the headline, which "
-                         "uses the configured 1.6); 0 skips it")
+    ap.add_argument("--omega-variants", default="auto,1.9",
+                    help="comma list of SOR omegas (a number, or 'auto' for the omega and launch shape tuned at "
+                         "the first solve) to run the mixed path with as well, reported beside the headline, "
+                         "which uses the configured 1.6; empty skips them")
     ap.add_argument("--trace-iterations", action="store_true",
                     help="drive each increment through the per-step calls and print every Newton iteration")
     args = ap.parse_args()
@@ -201,23 +207,32 @@ def main():
     last_state = None
     for path in args.paths.split(","):
         prec = 0 if path == "fp64" else 1
-        recs, wall, state = run_path(prob, prec, args.warmup, args.steps, capi, trace=args.trace_iterations)
+        recs, wall, state, _ = run_path(prob, prec, args.warmup, args.steps, capi, trace=args.trace_iterations)
         results[path] = (recs, wall)
         if prec == 0:
             last_state = state
-    omega_run = None
-    if args.omega_variant > 0 and "mixed" in results:
+    omega_runs = []
+    variants = [v for v in args.omega_variants.split(",") if v.strip()] if "mixed" in results else []
+    for v in variants:
         import dataclasses
-        pv = dataclasses.replace(prob, control=dataclasses.replace(ctl, sor_omega=args.omega_variant))
-        recs_o, wall_o, _ = run_path(pv, 1, args.warmup, args.steps, capi)
-        omega_run = {
-            "omega": args.omega_variant, "precision": "mixed",
+        om = 0.0 if v.strip() == "auto" else float(v)
+        pv = dataclasses.replace(prob, control=dataclasses.replace(ctl, sor_omega=om))
+        recs_o, wall_o, _, warm_o = run_path(pv, 1, args.warmup, args.steps, capi)
+        run = {
+            "omega": recs_o[-1]["sor_omega"], "set_as": v.strip(), "precision": "mixed",
+            "waves_per_workgroup": recs_o[-1]["sor_wpb"],
             "s_per_increment": wall_o / args.steps,
             "sweeps_per_increment": sum(r["sor_sweeps_total"] for r in recs_o) / args.steps,
             "nr_iterations_per_increment": sum(r["nit"] for r in recs_o) / args.steps,
             "gp_updates_per_s": sum(r["gp_updates"] for r in recs_o) / wall_o,
             "note": "same workload and 1e-8 relative residual with the SOR relaxation factor set in cpfem.dat; "
                     "not the headline value, which keeps the configured omega 1.6"}
+        if om == 0.0:
+            run["tune_s_once"] = sum(r["t_tune_ms"] for r in warm_o + recs_o) * 1e-3
+            run["tune_sweeps_once"] = sum(r["tune_sweeps"] for r in warm_o + recs_o)
+            run["note"] += ("; 'auto': omega <= 0, tuned with the sweep launch shape at the first solve (in the "
+                            "untimed warm-up increment, cost tune_s_once)")
+        omega_runs.append(run)
     recs64, wall64 = results["fp64"]
     nit = sum(r["nit"] for r in recs64)
     gp_updates = sum(r["gp_updates"] for r in recs64)
@@ -262,8 +277,8 @@ def main():
         extra["mixed_nr_iterations_per_increment"] = sum(r["nit"] for r in recs32) / args.steps
         extra["mixed_speedup_vs_fp64"] = wall64 / wall32
         extra["faferr_codes_per_increment_mixed"] = [r["code"] for r in recs32]
-    if omega_run:
-        extra["mixed_with_omega"] = omega_run
+    if omega_runs:
+        extra["mixed_with_omega"] = omega_runs
     shares = {"sor_fp64": t_sor, "constitutive": t_const, "assembly": t_asm}
     dom = max(shares, key=shares.get)
     cpu = cpu_baseline(prob, last_state, args.cpu_seconds, capi) if args.cpu_seconds > 0 else None

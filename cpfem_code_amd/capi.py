@@ -155,7 +155,9 @@ class CpfemStats(C.Structure):
                 ("t_suvals_ms", C.c_double), ("t_sor_ms", C.c_double), ("t_update_ms", C.c_double),
                 ("t_total_ms", C.c_double), ("flops_const", C.c_int64), ("transc_const", C.c_int64),
                 ("gp_updates", C.c_int64), ("pts_continued", C.c_int64), ("pts_wide", C.c_int64),
-                ("t_sweep_ms", C.c_double), ("sweep_launches", C.c_int64)]
+                ("t_sweep_ms", C.c_double), ("sweep_launches", C.c_int64),
+                ("sor_omega", C.c_double), ("t_tune_ms", C.c_double), ("sor_wpb", C.c_int32),
+                ("tune_sweeps", C.c_int32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -185,6 +187,12 @@ def _bind_mesh_api(L):
     L.cpfem_get_system.argtypes = [_CTX, _P, _P]
     L.cpfem_get_matrix.argtypes = [_CTX, _P, _P, _P, _P]
     L.cpfem_get_stats.argtypes = [_CTX, C.POINTER(CpfemStats)]
+    L.cpfem_autotune.argtypes = [_CTX, C.c_int32, _P, _P]
+
+
+# cpfem_autotune flags
+TUNE_LAUNCH = 1
+TUNE_OMEGA = 2
 
 
 # Codes after which the device Newton loop goes on (the caller decides, as the
@@ -302,6 +310,22 @@ class Solver:
         err = np.zeros(1)
         code = _check(self.L.cpfem_facsls(self.h, _ptr(sw), _ptr(err)), "cpfem_facsls", ok_codes=(904,))
         return code, int(sw[0]), float(err[0])
+
+    def autotune(self, launch=True, omega=True):
+        """Tune the sweeps on the system of the last suvals: launch shape
+        (waves per workgroup) and/or relaxation factor.  Returns (omega, waves)."""
+        om = np.zeros(1)
+        wpb = np.zeros(1, np.int32)
+        flags = (TUNE_LAUNCH if launch else 0) | (TUNE_OMEGA if omega else 0)
+        _check(self.L.cpfem_autotune(self.h, flags, _ptr(om), _ptr(wpb)), "cpfem_autotune")
+        return float(om[0]), int(wpb[0])
+
+    def set_sweeps(self, omega=0.0, waves=0):
+        """Apply a relaxation factor and/or waves per workgroup (0 keeps it)."""
+        om = np.array([float(omega)])
+        wpb = np.array([int(waves)], np.int32)
+        _check(self.L.cpfem_autotune(self.h, 0, _ptr(om), _ptr(wpb)), "cpfem_autotune")
+        return float(om[0]), int(wpb[0])
 
     def nr_update(self):
         sdx = np.zeros(1)

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <vector>
 
 #include "common.hpp"
@@ -691,7 +692,7 @@ __global__ __launch_bounds__(kBlock) void k_sor_color(const SorArgs A, const T* 
                                                       int pslot0) {
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     const int lane = threadIdx.x & 63;
-    const int w = xcd_block(A.per_xcd) * kWavesPerBlock + (threadIdx.x >> 6);
+    const int w = xcd_block(A.per_xcd) * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
     const int l = lane / kGroupRows, i = lane - l * kGroupRows;
     const bool live = (l < kRowLanes) && (w * kGroupRows + i < A.nrows) && (*A.stop == 0);
     const size_t g = (size_t)A.gbase + w;
@@ -1118,6 +1119,7 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
     // more state variables (fs3p-gps.f90:30-31), which are carried unchanged
     if (m->nsv < 3 + nss || m->nsv > 63) return CPFEM_E_ARG;
     if (ctl->precision == 1 && ctl->sor_kind != 1) return CPFEM_E_ARG;
+    if (!(ctl->sor_omega < 2.0)) return CPFEM_E_ARG;   // SOR on an SPD system converges for 0 < omega < 2
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CPFEM_E_NOGPU;
     // element tables.  Types: 12 (8-node brick, 1 point, SHAPE_12), 13 (8-node
@@ -1173,6 +1175,8 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
     c->nmat = nmat;
     c->ctl = *ctl;
     c->precision = ctl->precision;
+    c->omega_auto = ctl->sor_omega <= 0.0;
+    if (c->omega_auto) c->ctl.sor_omega = 1.0;   // until the first solve tunes it
     c->tiem.assign(m->tiem, m->tiem + m->ninc1);
     c->npts8 = npts8;
     c->npts20 = npts20;
@@ -1247,7 +1251,7 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
         // (row-group launches use at most nnod / (4 * kGroupRows) + 8 blocks per colour)
         // (the volume averages use two point-sized launches)
         const int nbr = std::max(nblk(std::max<size_t>(std::max<size_t>(2 * G, D), 8 * (size_t)c->nnod)) + 64,
-                                 (int)((size_t)c->nnod / (kWavesPerBlock * kGroupRows)) + 16 * 64 + 64);
+                                 (int)((size_t)c->nnod / kGroupRows) + 16 * 64 + 64);
         c->red.nblocks = nbr;
         TRY(dalloc(c, &c->red.partial, (size_t)nbr * 4));
         TRY(dalloc(c, &c->red.result, 8));
@@ -1369,6 +1373,7 @@ static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
     hipStream_t s = c->stream;
     const double dt = c->tiem[inc - 1];
     c->rng_epoch += 1;
+    c->assembled = false;
     CPFEM_HIP(hipEventRecord(c->ev[0], s));
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, c->flags + 1, (size_t)1, 0);
     // point kinematics (strain and rotation increments, element checks)
@@ -1525,6 +1530,7 @@ static int suvals_impl(Ctx* c, int inc, double* sfc) {
     c->stats.t_suvals_ms += elapsed(c->ev[3], c->ev[4]);
     c->stats.sfc = sqrt(r[0]);
     if (sfc) *sfc = c->stats.sfc;
+    c->assembled = true;
     return 0;
 }
 
@@ -1538,8 +1544,8 @@ static const int kSweepBatch = 16;
 // Most batches between two residual checks of the residual-based stopping rule.
 static const int kMaxCheckBatches = 64;
 
-// Blocks per XCD of a launch over ng row groups.
-static int group_per_xcd(int ng) { return ((ng + kWavesPerBlock - 1) / kWavesPerBlock + kXcd - 1) / kXcd; }
+// Blocks per XCD of a launch over ng row groups, wpb groups (waves) per block.
+static int group_per_xcd(int ng, int wpb = kWavesPerBlock) { return ((ng + wpb - 1) / wpb + kXcd - 1) / kXcd; }
 
 template <typename T, typename V, bool ACC>
 static void sor_sweep(Ctx* c, const T* vals, const V* b, V* y, double* partial) {
@@ -1552,21 +1558,21 @@ static void sor_sweep(Ctx* c, const T* vals, const V* b, V* y, double* partial) 
     for (int k = 0; k < c->A.ncolor; ++k) {
         S.nrows = c->A.cstart[k + 1] - c->A.cstart[k];
         S.gbase = c->A.gstart[k];
-        S.per_xcd = group_per_xcd(c->A.gstart[k + 1] - c->A.gstart[k]);
+        S.per_xcd = group_per_xcd(c->A.gstart[k + 1] - c->A.gstart[k], c->sor_wpb);
         const int nb = S.per_xcd * kXcd;
         if (S.P == 3)
-            hipLaunchKernelGGL((k_sor_color<T, V, ACC, 3>), dim3(nb), dim3(kBlock), 0, c->stream, S, vals, b, y,
-                               partial, slot);
+            hipLaunchKernelGGL((k_sor_color<T, V, ACC, 3>), dim3(nb), dim3(64 * c->sor_wpb), 0, c->stream, S, vals,
+                               b, y, partial, slot);
         else
-            hipLaunchKernelGGL((k_sor_color<T, V, ACC, 0>), dim3(nb), dim3(kBlock), 0, c->stream, S, vals, b, y,
-                               partial, slot);
+            hipLaunchKernelGGL((k_sor_color<T, V, ACC, 0>), dim3(nb), dim3(64 * c->sor_wpb), 0, c->stream, S, vals,
+                               b, y, partial, slot);
         slot += nb;
     }
 }
 
 static int sor_partial_blocks(Ctx* c) {
     int slot = 0;
-    for (int k = 0; k < c->A.ncolor; ++k) slot += kXcd * group_per_xcd(c->A.gstart[k + 1] - c->A.gstart[k]);
+    for (int k = 0; k < c->A.ncolor; ++k) slot += kXcd * group_per_xcd(c->A.gstart[k + 1] - c->A.gstart[k], c->sor_wpb);
     return slot;
 }
 
@@ -1641,8 +1647,15 @@ static double residual(Ctx* c, const double* b, const double* y, double* res) {
     return r[1] > 0.0 ? sqrt(r[0] / r[1]) : 0.0;
 }
 
+static int autotune_impl(Ctx* c, int flags);
+
 // FACSLS: solve A ddx = rhs from a zero start.
 static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out) {
+    if (c->omega_auto) {
+        const int rc = autotune_impl(c, CPFEM_TUNE_LAUNCH | CPFEM_TUNE_OMEGA);
+        if (rc) return rc;
+        c->omega_auto = false;
+    }
     hipStream_t s = c->stream;
     const int D = 3 * c->nnod;
     const int maxit = c->ctl.sor_maxit;
@@ -1720,10 +1733,192 @@ static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out) {
     c->stats.sor_sweeps = sweeps;
     c->stats.sor_sweeps_total += sweeps;
     c->stats.sor_error = err;
+    c->stats.sor_omega = c->ctl.sor_omega;
+    c->stats.sor_wpb = c->sor_wpb;
     if (sweeps_out) *sweeps_out = sweeps;
     if (err_out) *err_out = err;
     if (code > c->stats.err) c->stats.err = code;
     return code;
+}
+
+// ---------------------------------------------------------------------------
+// Sweep tuning.  The launch shape changes only how row groups are packed into
+// workgroups (each wave relaxes its own rows), so the iterates are the same at
+// every shape; it is chosen by timing replays of the solve's own sweep graph.
+// The relaxation factor changes the iterates: each candidate of a ladder runs
+// two windows of W sweeps from ddx = 0 (in mixed precision each window is one
+// refinement cycle, as in the solve), and the sweeps to the solve's tolerance
+// are extrapolated from the residuals after each window at the second
+// window's geometric rate.  The ladder is walked upwards and stops two steps
+// past the best candidate (the sweep count is unimodal in omega for SOR on an
+// SPD system).
+static void drop_graphs(Ctx* c) {
+    for (int i = 0; i < 3; ++i)
+        if (c->graphs[i]) {
+            (void)hipGraphExecDestroy(c->graphs[i]);
+            c->graphs[i] = nullptr;
+        }
+}
+
+// Graph the solve replays for this context's settings.
+static int solve_graph_kind(const Ctx* c) { return c->precision == 1 ? 2 : (c->ctl.sor_kind == 0 ? 0 : 1); }
+
+// Device time (ms) of `reps` replays of the solve's sweep graph from a zero start.
+static hipError_t time_sweep_graph(Ctx* c, int reps, float* ms) {
+    hipStream_t s = c->stream;
+    const int D = 3 * c->nnod;
+    const int which = solve_graph_kind(c);
+    hipGraphExec_t gx;
+    hipError_t e = sweep_graph(c, which, &gx);
+    if (e != hipSuccess) return e;
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+        if (which == 2) {
+            hipLaunchKernelGGL(k_to_float, dim3(nblk(D)), dim3(kBlock), 0, s, (const double*)c->rhs, c->rhs32, D);
+            hipLaunchKernelGGL(k_fill<float>, dim3(nblk(D)), dim3(kBlock), 0, s, c->del32, (size_t)D, 0.0f);
+        } else {
+            hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, s, c->ddx, (size_t)D, 0.0);
+        }
+        hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, c->flags + 2, (size_t)2, 0);
+        if ((e = hipEventRecord(c->ev[7], s)) != hipSuccess) return e;
+        for (int k = 0; k < reps; ++k)
+            if ((e = hipGraphLaunch(gx, s)) != hipSuccess) return e;
+        if ((e = hipEventRecord(c->ev[8], s)) != hipSuccess) return e;
+        if ((e = hipEventSynchronize(c->ev[8])) != hipSuccess) return e;
+        if (rep > 0) best = std::min(best, elapsed(c->ev[7], c->ev[8]));   // the first is a warm-up
+    }
+    *ms = best;
+    return hipSuccess;
+}
+
+// One window of W sweeps continuing from the current ddx (and, in mixed
+// precision, the residual held in xold); returns the relative residual.
+static hipError_t tune_window(Ctx* c, int W, double* rel) {
+    hipStream_t s = c->stream;
+    const int D = 3 * c->nnod;
+    const bool mixed = c->precision == 1;
+    hipGraphExec_t gx;
+    hipError_t e = sweep_graph(c, mixed ? 2 : 1, &gx);
+    if (e != hipSuccess) return e;
+    if (mixed) {
+        hipLaunchKernelGGL(k_to_float, dim3(nblk(D)), dim3(kBlock), 0, s, (const double*)c->xold, c->rhs32, D);
+        hipLaunchKernelGGL(k_fill<float>, dim3(nblk(D)), dim3(kBlock), 0, s, c->del32, (size_t)D, 0.0f);
+    }
+    for (int k = 0; k < W / kSweepBatch; ++k)
+        if ((e = hipGraphLaunch(gx, s)) != hipSuccess) return e;
+    if (mixed) hipLaunchKernelGGL(k_add_float, dim3(nblk(D)), dim3(kBlock), 0, s, c->ddx, (const float*)c->del32, D);
+    *rel = residual(c, c->rhs, c->ddx, mixed ? c->xold : nullptr);
+    return hipGetLastError();
+}
+
+// Sweeps to the tolerance predicted from the residuals r1, r2 after W and 2W sweeps.
+static double predicted_sweeps(double r1, double r2, int W, double tol) {
+    if (!(r1 > 0.0) || !std::isfinite(r1)) return r1 == 0.0 ? 0.0 : 1e300;
+    if (r1 <= tol) return W * log(tol) / log(std::min(r1, 0.999));
+    if (!(r2 > 0.0) || !std::isfinite(r2) || r2 >= r1) return r2 == 0.0 ? W : 1e300;
+    const double rate = log(r2 / r1) / W;
+    return r2 <= tol ? W + log(tol / r1) / rate : 2.0 * W + log(tol / r2) / rate;
+}
+
+static int autotune_impl(Ctx* c, int flags) {
+    if (!c->assembled) return CPFEM_E_ARG;
+    hipStream_t s = c->stream;
+    const int D = 3 * c->nnod;
+    const auto t0 = std::chrono::steady_clock::now();
+    double* saved = nullptr;
+    CPFEM_HIP(hipMalloc((void**)&saved, D * sizeof(double)));
+    int rc = 0;
+    auto fail = [&](hipError_t e) {
+        rc = e == hipSuccess ? 0 : CPFEM_E_HIP;
+        return rc;
+    };
+    (void)hipMemcpyAsync(saved, c->ddx, D * sizeof(double), hipMemcpyDeviceToDevice, s);
+    if (flags & CPFEM_TUNE_LAUNCH) {
+        int best_w = c->sor_wpb;
+        float best_ms = 1e30f;
+        for (int w : {4, 2, 1}) {
+            c->sor_wpb = w;
+            if (sor_partial_blocks(c) > c->red.nblocks) continue;
+            drop_graphs(c);
+            float ms = 0.0f;
+            if (fail(time_sweep_graph(c, 4, &ms))) break;
+            c->stats.tune_sweeps += 3 * 4 * kSweepBatch;
+            if (ms < 0.98f * best_ms) {
+                best_ms = ms;
+                best_w = w;
+            }
+        }
+        c->sor_wpb = best_w;
+        drop_graphs(c);
+    }
+    if (!rc && (flags & CPFEM_TUNE_OMEGA)) {
+        static const double ladder[] = {1.0, 1.3, 1.5, 1.65, 1.75, 1.82, 1.87, 1.91, 1.94, 1.96, 1.98};
+        // window: an eighth of the sweep limit, 64 to 256 sweeps, whole batches
+        const int W = std::max(64, std::min(256, c->ctl.sor_maxit / 8)) / kSweepBatch * kSweepBatch;
+        const double tol = c->precision == 1 ? std::max(c->ctl.sor_tol, 1e-12) : std::max(c->ctl.sor_tol, 1e-14);
+        double best_om = c->ctl.sor_omega, best_n = 1e300;
+        int since_best = 0;
+        for (double om : ladder) {
+            c->ctl.sor_omega = om;
+            drop_graphs(c);
+            hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, s, c->ddx, (size_t)D, 0.0);
+            if (c->precision == 1)
+                (void)hipMemcpyAsync(c->xold, c->rhs, D * sizeof(double), hipMemcpyDeviceToDevice, s);
+            double r1 = 0.0, r2 = 0.0;
+            if (fail(tune_window(c, W, &r1))) break;
+            if (r1 > c->ctl.sor_tol && std::isfinite(r1))
+                if (fail(tune_window(c, W, &r2))) break;
+            c->stats.tune_sweeps += (r2 > 0.0 ? 2 : 1) * W;
+            const double n = predicted_sweeps(r1, r2, W, tol);
+            if (n < best_n) {
+                best_n = n;
+                best_om = om;
+                since_best = 0;
+            } else if (++since_best >= 2) {
+                break;
+            }
+        }
+        c->ctl.sor_omega = best_om;
+        drop_graphs(c);
+    }
+    (void)hipMemcpyAsync(c->ddx, saved, D * sizeof(double), hipMemcpyDeviceToDevice, s);
+    const hipError_t e = hipStreamSynchronize(s);
+    (void)hipFree(saved);
+    if (!rc && e != hipSuccess) rc = CPFEM_E_HIP;
+    c->stats.t_tune_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    c->stats.sor_omega = c->ctl.sor_omega;
+    c->stats.sor_wpb = c->sor_wpb;
+    return rc;
+}
+
+extern "C" int cpfem_autotune(cpfem_ctx* c, int32_t flags, double* omega, int32_t* waves_per_block) {
+    if (!c || (flags & ~(CPFEM_TUNE_LAUNCH | CPFEM_TUNE_OMEGA))) return CPFEM_E_ARG;
+    if (flags == 0) {
+        // apply the given settings
+        const double om = omega ? *omega : 0.0;
+        const int w = waves_per_block ? *waves_per_block : 0;
+        if (!(om < 2.0) || (w != 0 && w != 1 && w != 2 && w != 4)) return CPFEM_E_ARG;
+        if (w > 0) {
+            const int keep = c->sor_wpb;
+            c->sor_wpb = w;
+            if (sor_partial_blocks(c) > c->red.nblocks) {
+                c->sor_wpb = keep;
+                return CPFEM_E_ARG;
+            }
+        }
+        if (om > 0.0) {
+            c->ctl.sor_omega = om;
+            c->omega_auto = false;
+        }
+        drop_graphs(c);
+    } else {
+        const int rc = autotune_impl(c, flags);
+        if (rc) return rc;
+    }
+    if (flags & CPFEM_TUNE_OMEGA) c->omega_auto = false;
+    if (omega) *omega = c->ctl.sor_omega;
+    if (waves_per_block) *waves_per_block = c->sor_wpb;
+    return 0;
 }
 
 extern "C" int cpfem_facsls(cpfem_ctx* c, int32_t* sweeps, double* error) {

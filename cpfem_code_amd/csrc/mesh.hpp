@@ -115,6 +115,9 @@ struct Ctx {
     Reducer red;
     ConstSchedule sched;
     hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};  // sweep batches: FP64 update test, FP64, FP32
+    int sor_wpb = 4;             // waves per workgroup of the colour sweeps (cpfem_autotune)
+    bool omega_auto = false;     // sor_omega <= 0 at creation: tuned at the first solve
+    bool assembled = false;      // A and rhs hold a system (set by SUVALS)
     // timing and counters of the last calls
     static constexpr int kEvents = 10;   // [7], [8]: around each sweep batch
     hipEvent_t ev[kEvents] = {};

@@ -45,6 +45,8 @@ MODULE CPFEM_CAPI
         INTEGER(C_INT64_T) :: flops_const, transc_const, gp_updates, pts_continued, pts_wide
         REAL(C_DOUBLE) :: t_sweep_ms
         INTEGER(C_INT64_T) :: sweep_launches
+        REAL(C_DOUBLE) :: sor_omega, t_tune_ms
+        INTEGER(C_INT32_T) :: sor_wpb, tune_sweeps
     END TYPE cpfem_stats
 
     INTERFACE
@@ -154,6 +156,14 @@ MODULE CPFEM_CAPI
             IMPORT :: C_INT, C_PTR, cpfem_stats
             TYPE(C_PTR), VALUE :: ctx
             TYPE(cpfem_stats), INTENT(OUT) :: st
+        END FUNCTION
+
+        INTEGER(C_INT) FUNCTION cpfem_autotune(ctx, flags, omega, waves_per_block) BIND(C, name='cpfem_autotune')
+            IMPORT :: C_INT, C_INT32_T, C_PTR, C_DOUBLE
+            TYPE(C_PTR), VALUE :: ctx
+            INTEGER(C_INT32_T), VALUE :: flags
+            REAL(C_DOUBLE), INTENT(OUT) :: omega
+            INTEGER(C_INT32_T), INTENT(OUT) :: waves_per_block
         END FUNCTION
     END INTERFACE
 END MODULE CPFEM_CAPI

@@ -14,6 +14,8 @@
 ! sweeps, stop on the relative update; Stif_sol.f90:173-210) can be changed
 ! in an optional file cpfem.dat in the same directory as fafcdf.dat:
 !   omega  sor_kind(0 update, 1 residual)  sor_tol  sor_maxit  precision(0 FP64, 1 mixed)
+! omega <= 0 lets the library choose omega and the sweep launch shape on the
+! first system it solves (cpfem_autotune).
 !-----------------------------------------------------------------------
 MODULE HOST_STATE
     USE ISO_C_BINDING

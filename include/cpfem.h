@@ -50,7 +50,7 @@ typedef struct {
     double alpha, afactor;
     double efix, smalp, smalf;
     double smald, smale, smalr;
-    double sor_omega;     /* Stif_sol.f90:173 alphal */
+    double sor_omega;     /* Stif_sol.f90:173 alphal; in (0, 2), or <= 0: chosen by cpfem_autotune at the first solve */
     double sor_tol;       /* tolerance of the SOR stopping test */
     int32_t sor_kind;     /* 0: |dx|^2/|x|^2 < tol^2 (Stif_sol.f90:208-210); 1: |D^-1(b-Ax)|/|D^-1 b| < tol, D = diag(A) */
     int32_t precision;    /* 0: FP64 throughout; 1: FP32 element products and SOR sweeps, FP64 residuals */
@@ -130,6 +130,10 @@ typedef struct {
     int64_t pts_wide;         /* point updates whose active set exceeded the register-resident width */
     double t_sweep_ms;        /* device time of the sweep-kernel batches alone (events around each batch) */
     int64_t sweep_launches;   /* colour-sweep kernel launches in those batches */
+    double sor_omega;         /* relaxation factor of the last FACSLS (the tuned one when set by cpfem_autotune) */
+    double t_tune_ms;         /* time spent in cpfem_autotune during the call (automatic omega) */
+    int32_t sor_wpb;          /* waves per workgroup of the colour-sweep launches */
+    int32_t tune_sweeps;      /* sweeps run by cpfem_autotune during the call */
 } cpfem_stats;
 
 /* Upload mesh, materials and control; build the block sparsity, colourings
@@ -189,6 +193,24 @@ int cpfem_get_system(cpfem_ctx *ctx, double *rhs, double *ddx);
 /* Assembled matrix as COO in DOF numbering (0-based); call with NULL arrays to get *nnz. */
 int cpfem_get_matrix(cpfem_ctx *ctx, int64_t *nnz, int32_t *rows, int32_t *cols, double *vals);
 int cpfem_get_stats(cpfem_ctx *ctx, cpfem_stats *stats);
+
+/* Tuning of FACSLS's sweeps on the system assembled by the last cpfem_suvals.
+ * No counterpart in the reference, whose FACSLS is a fixed loop with omega
+ * compiled in (Stif_sol.f90:173); for callers that solve many systems of one
+ * mesh.  With CPFEM_TUNE_LAUNCH the colour-sweep kernel is timed at 1, 2 and 4
+ * waves per workgroup and the fastest kept (the iterates do not depend on it).
+ * With CPFEM_TUNE_OMEGA each relaxation factor of a ladder in [1, 1.98] runs
+ * two windows of sweeps from a zero start, the number of sweeps to the solve's
+ * tolerance is extrapolated from the measured residuals, and the factor with
+ * the fewest is kept.  The system and ddx are left as they were; the sweep
+ * graphs are rebuilt.  omega and waves_per_block (either may be NULL) return
+ * the settings in force.  With flags 0 nothing is tuned and no system is
+ * needed: a positive *omega (< 2) and a *waves_per_block of 1, 2 or 4 are
+ * applied as given (0 keeps the setting).  A context created with
+ * sor_omega <= 0 runs both tunings at its first solve. */
+#define CPFEM_TUNE_LAUNCH 1
+#define CPFEM_TUNE_OMEGA 2
+int cpfem_autotune(cpfem_ctx *ctx, int32_t flags, double *omega, int32_t *waves_per_block);
 
 /* Device name and compute units of the current device (diagnostics). */
 int cpfem_device_info(char *name, int32_t len, int32_t *cus);
