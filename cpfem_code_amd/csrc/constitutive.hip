@@ -38,11 +38,11 @@ __device__ __forceinline__ void plastc_c(const DevMaterial<N>& M, Compact<AMAX>&
                                          int64_t& flops, int64_t& transc) {
     const int nsa = S.nsa;
     double sumss = 0.0;
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
     for (int k = 0; k < AMAX; ++k)
         if (k < nsa) sumss = sumss + fabs(S.gam[k]);
     const double rate = sumss / dt;
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
     for (int k = 0; k < AMAX; ++k) {
         if (k < nsa) {
             const int j = M.uniform_par ? 0 : S.isa[k];
@@ -54,19 +54,19 @@ __device__ __forceinline__ void plastc_c(const DevMaterial<N>& M, Compact<AMAX>&
             th[k] = 0.0;
         }
     }
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
     for (int k = 0; k < AMAX; ++k) {
         if (k < nsa) {
             double s = S.s0[k];
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
             for (int l = 0; l < AMAX; ++l)
                 if (l < nsa) s = s + (latent(M, S.isa[k], S.isa[l]) * th[l]) * fabs(S.gam[l]);
             S.s1[k] = s;
         }
     }
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
     for (int k = 0; k < AMAX; ++k) {
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
         for (int l = 0; l < AMAX; ++l) {
             double v = (k == l) ? 1.0 : 0.0;
             if (k < nsa && l < nsa) {
@@ -78,8 +78,9 @@ __device__ __forceinline__ void plastc_c(const DevMaterial<N>& M, Compact<AMAX>&
             S.ah[k][l] = v;
         }
     }
-    flops += 3 * (int64_t)N * N + 6 * N + 3 * (int64_t)nsa * nsa;
-    transc += 4 * N;
+    const int ns = nsys(M);
+    flops += 3 * (int64_t)ns * ns + 6 * ns + 3 * (int64_t)nsa * nsa;
+    transc += 4 * ns;
 }
 
 // SCALE_IT on the active set; amax starts at am of system 1 (Cpstrs.f90:200).
@@ -88,7 +89,7 @@ __device__ __forceinline__ void scale_it_c(const DevMaterial<N>& M, Compact<AMAX
                                            double afactor, int64_t& flops, int64_t& transc) {
     double amax = M.am[0];
     double gammax = fabs(S.gam[0]);
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
     for (int k = 1; k < AMAX; ++k) {
         if (k < S.nsa && fabs(S.gam[k]) > gammax) {
             gammax = fabs(S.gam[k]);
@@ -101,12 +102,12 @@ __device__ __forceinline__ void scale_it_c(const DevMaterial<N>& M, Compact<AMAX
         if (smult == 0.0) smult = 0.004 / amax;
 #pragma unroll
         for (int i = 0; i < 6; ++i) s[i] = smult * s[i];
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
         for (int k = 0; k < AMAX; ++k) {
             S.gam[k] = scale * S.gam[k];
             S.tau[k] = smult * S.tau[k];
         }
-        flops += S.nsa + 2 * N + 6;
+        flops += S.nsa + 2 * nsys(M) + 6;
         transc += 2;
     } else {
         flops += S.nsa;
@@ -126,11 +127,11 @@ __device__ __forceinline__ void slpfta_c(const DevMaterial<N>& M, const DevContr
     while (more) {
         ++it;
         plastc_c<N, AMAX>(M, S, dt, th, flops, transc);
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
         for (int i = 0; i < AMAX; ++i) {
             if (i < nsa) {
                 const double gamn = S.gam[i];
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
                 for (int j = 0; j < AMAX; ++j)
                     if (j < nsa) S.ah[j][i] = gamn * S.ah[j][i];
                 S.ah[i][i] = S.ah[i][i] + rate_m(M, S.isa[i]);
@@ -138,7 +139,7 @@ __device__ __forceinline__ void slpfta_c(const DevMaterial<N>& M, const DevContr
         }
         if (factor<AMAX>(S.ah)) err = err > 910 ? err : 910;
         double e[AMAX];
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
         for (int k = 0; k < AMAX; ++k) {
             double v = 0.0;
             if (k < nsa) {
@@ -148,17 +149,17 @@ __device__ __forceinline__ void slpfta_c(const DevMaterial<N>& M, const DevContr
             e[k] = v;
         }
         tauerr = 0.0;
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
         for (int k = 0; k < AMAX; ++k)
             if (k < nsa) tauerr = tauerr + e[k] * e[k];
         tauerr = sqrt(tauerr);
         solver<AMAX>(S.ah, e);
         double errmax = 0.0;
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
         for (int k = 0; k < AMAX; ++k)
             if (k < nsa && fabs(e[k]) > errmax) errmax = fabs(e[k]);
         const double scale = (errmax > 5.0) ? 5.0 / errmax : 1.0;
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
         for (int k = 0; k < AMAX; ++k)
             if (k < nsa) S.gam[k] = S.gam[k] * exp(scale * e[k]);
         more = (it < C.maxit) && (tauerr > M.conv);
@@ -167,12 +168,12 @@ __device__ __forceinline__ void slpfta_c(const DevMaterial<N>& M, const DevContr
     }
     if (tauerr > 0.5) err = err > 987 ? err : 987;
     plastc_c<N, AMAX>(M, S, dt, th, flops, transc);
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
     for (int k = 0; k < AMAX; ++k) gpre[k * gstride] = S.gam[k];
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
     for (int i = 0; i < AMAX; ++i) {
         if (i < nsa) {
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
             for (int j = 0; j < AMAX; ++j)
                 if (j < nsa) S.ah[j][i] = S.tau[j] * S.ah[j][i];
             const double ag = fabs(S.gam[i]);
@@ -190,7 +191,8 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
     // LDS: the Schmid columns of the active systems (also the staging area of
     // the active-set compaction) and the active system numbers, interleaved by
     // lane; the compliance and the last slips live in per-point work arrays
-    __shared__ double sh_b[6 * AMAX][64];
+    constexpr int kSlots = 6 * AMAX > 2 * N ? 6 * AMAX : 2 * N;
+    __shared__ double sh_b[kSlots][64];
     __shared__ int sh_isa[AMAX][64];
     const int lane = threadIdx.x;
     const int idx = blockIdx.x * 64 + lane;
@@ -207,6 +209,7 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
     bool want_cont = false, want_ovf = false;
     if (live) {
         const DevMaterial<N>& M = *Mp;
+        const int NS = nsys(M);
         const size_t n = (size_t)A.n;
         double* el = A.el + g;
         double* bc = &sh_b[0][lane];
@@ -289,7 +292,7 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
             ctmatrx(ph[0], ph[1], ph[2], a, aug);
             const double gamcap = 6.0 * fj2vom(false, de);
             if (first) {
-                flops += 14 + 1 + 12 + 60 * (int64_t)N;
+                flops += 14 + 1 + 12 + 60 * (int64_t)NS;
                 transc += 6;
             }
             int twinsys = A.twinsys[g];
@@ -306,10 +309,9 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
                 // system's slip and resolved shear in the LDS slice of the
                 // Schmid columns, slots 2i and 2i+1; the active ones are then
                 // compacted in place (write slot <= read slot) to 2k, 2k+1.
-                static_assert(6 * AMAX >= 2 * N, "staging needs 2N slots");
                 double gammax = 0.0;
 #pragma unroll 1
-                for (int i = 0; i < N; ++i) {
+                for (int i = 0; i < NS; ++i) {
                     double sdr[3], spn[3], b[6];
                     slip_vectors(M, a, i, sdr, spn);
                     schmid(sdr, spn, b);
@@ -338,7 +340,7 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
                 const double cutoff = 1.e-6 * gammax;
                 int nsa = 0;
 #pragma unroll 1
-                for (int i = 0; i < N; ++i) {
+                for (int i = 0; i < NS; ++i) {
                     const double gi = bc[(2 * i + 0) * 64];
                     if (!(fabs(gi) < cutoff)) {
                         if (nsa < AMAX) {
@@ -356,7 +358,7 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
                 }
                 ++iter;
                 S.nsa = nsa;
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
                 for (int k = 0; k < AMAX; ++k) {
                     if (k < nsa) {
                         const int i = sh_isa[k][lane];
@@ -371,8 +373,8 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
                         S.s0[k] = 1.0;
                     }
                 }
-                flops += 5 * (int64_t)N;
-                transc += 2 * (int64_t)N;
+                flops += 5 * (int64_t)NS;
+                transc += 2 * (int64_t)NS;
                 // Schmid columns of the active systems (BTMATRX, crys_math.f90:75-115)
 #pragma unroll 1
                 for (int k = 0; k < nsa; ++k) {
@@ -389,7 +391,7 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
 #pragma unroll
                 for (int i = 0; i < 6; ++i) {
                     double sum = 0.0;
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
                     for (int k = 0; k < AMAX; ++k)
                         if (k < nsa) sum = sum + bc[(6 * k + i) * 64] * S.gam[k];
                     epl[i] = sum;
@@ -412,13 +414,13 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
 #pragma unroll
                 for (int j = 0; j < 6; ++j) {
                     double w[AMAX];
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
                     for (int k = 0; k < AMAX; ++k) w[k] = (k < nsa) ? bc[(6 * k + j) * 64] : 0.0;
                     solver<AMAX>(S.ah, w);
 #pragma unroll
                     for (int i = 0; i < 6; ++i) {
                         double sum = 0.0;
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
                         for (int k = 0; k < AMAX; ++k)
                             if (k < nsa) sum = sum + bc[(6 * k + i) * 64] * w[k];
                         djac[i][j] = sum + el[sym21(i, j) * n];
@@ -433,7 +435,7 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
 #pragma unroll
                 for (int i = 0; i < 6; ++i) s[i] = s[i] + erreps[i];
                 done = !((iter < C.maxit) && (erepmd > C.toler));
-                flops += 24 * (int64_t)N + 90 + 72 * (int64_t)nsa + 36 + 44 + factor_flops(nsa) +
+                flops += 24 * (int64_t)NS + 90 + 72 * (int64_t)nsa + 36 + 44 + factor_flops(nsa) +
                          6 * solver_flops(nsa) + factor_flops(6) + solver_flops(6) + 28;
             }
             if (tflag) atomicOr(A.tflag, 1);
@@ -441,13 +443,13 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
                 // ---------------- finish CPSTRS and DSTRSS ----------------
                 const int nsa = S.nsa;
 #pragma unroll
-                for (int i = 0; i < N; ++i) A.srate[i * n + g] = 0.0;
-#pragma unroll
+                for (int i = 0; i < NS; ++i) A.srate[i * n + g] = 0.0;
+#pragma unroll kUnrollA<AMAX>
                 for (int k = 0; k < AMAX; ++k)
                     if (k < nsa) A.srate[S.isa[k] * n + g] = fabs(S.gam[k] / dt);
                 // ECHROT (crys_math.f90:137-167)
                 double rtn[3] = {0.0, 0.0, 0.0};
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
                 for (int k = 0; k < AMAX; ++k) {
                     if (k < nsa) {
                         double sdr[3], spn[3];
@@ -477,11 +479,11 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
                 {
                     double th[AMAX];
                     double sumss = 0.0;
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
                     for (int k = 0; k < AMAX; ++k)
                         if (k < nsa) sumss = sumss + fabs(gpre[k * n]);
                     const double rate = sumss / dt;
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
                     for (int k = 0; k < AMAX; ++k) {
                         th[k] = 0.0;
                         if (k < nsa) {
@@ -493,10 +495,10 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
                         }
                     }
 #pragma unroll
-                    for (int i = 0; i < N; ++i) {
+                    for (int i = 0; i < NS; ++i) {
                         const double s0i = A.sv[(3 + i) * n + g];
                         double s1 = s0i;
-#pragma unroll
+#pragma unroll kUnrollA<AMAX>
                         for (int l = 0; l < AMAX; ++l)
                             if (l < nsa) s1 = s1 + (latent(M, i, S.isa[l]) * th[l]) * fabs(gpre[l * n]);
                         A.dsv[(3 + i) * n + g] = s1 - s0i;
@@ -532,7 +534,7 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
                 }
                 A.dep[g] = fj2vom(false, epl);
                 A.erepmd[g] = erepmd;
-                flops += 6 * (int64_t)N + 25 + 6 * solver_flops(6) + 82;
+                flops += 6 * (int64_t)NS + 25 + 6 * solver_flops(6) + 82;
                 A.flops[g] = flops;
                 A.transc[g] = transc;
                 A.err[g] = err;
@@ -567,8 +569,9 @@ void fill_material(DevMaterial<N>& d, const cpfem_material& m, double toler) {
     d.uniform_par = 1;
     d.two_valued = 1;
     d.hd = m.smatrx[0][0];
-    d.hq = N > 1 ? m.smatrx[0][1] : 0.0;
-    for (int i = 0; i < N; ++i) {
+    const int ns = N == kNGeneric ? m.nss : N;
+    d.hq = ns > 1 ? m.smatrx[0][1] : 0.0;
+    for (int i = 0; i < ns; ++i) {
         d.ssang[i] = m.ssang[i];
         if (fabs(m.ssang[i]) > 1.5) d.any_twin = 1;
         if (m.ssang[i] != 0.0) d.any_unidir = 1;
@@ -584,7 +587,7 @@ void fill_material(DevMaterial<N>& d, const cpfem_material& m, double toler) {
         d.nsat[i] = m.ampar[i][5];
         for (int q = 0; q < 6; ++q)
             if (m.ampar[i][q] != m.ampar[0][q]) d.uniform_par = 0;
-        for (int k = 0; k < N; ++k) {
+        for (int k = 0; k < ns; ++k) {
             d.smatrx[i][k] = m.smatrx[k][i];  // Fortran smatrx(i,k)
             const double want = (i == k) ? d.hd : d.hq;
             if (d.smatrx[i][k] != want) d.two_valued = 0;
@@ -608,13 +611,17 @@ size_t material_bytes(int nss) {
     switch (nss) {
         case 12: return sizeof(DevMaterial<12>);
         case 18: return sizeof(DevMaterial<18>);
-        default: return 0;
+        default: return (nss >= 1 && nss <= kNGeneric) ? sizeof(DevMaterial<kNGeneric>) : 0;
     }
 }
 
 void build_material(int nss, void* host, const cpfem_material& m, double toler) {
-    if (nss == 12) fill_material<12>(*reinterpret_cast<DevMaterial<12>*>(host), m, toler);
-    if (nss == 18) fill_material<18>(*reinterpret_cast<DevMaterial<18>*>(host), m, toler);
+    if (nss == 12)
+        fill_material<12>(*reinterpret_cast<DevMaterial<12>*>(host), m, toler);
+    else if (nss == 18)
+        fill_material<18>(*reinterpret_cast<DevMaterial<18>*>(host), m, toler);
+    else
+        fill_material<kNGeneric>(*reinterpret_cast<DevMaterial<kNGeneric>*>(host), m, toler);
 }
 
 // Width of the first kernel (active systems held in registers) and the
@@ -625,6 +632,7 @@ void build_material(int nss, void* host, const cpfem_material& m, double toler) 
 constexpr int kAmax = CPFEM_AMAX;
 constexpr int kAmaxMid = 8;
 constexpr int kAmaxWide = 12;
+constexpr int kAmaxGeneric = kConstMaxWidth;   // fourth tier, generic system counts only
 constexpr int kFirstPassIters = 8;
 
 __global__ void k_zero4(int* p) {
@@ -712,8 +720,27 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
             hipLaunchKernelGGL(k_set1, dim3(1), dim3(64), 0, stream, S.counts + 2, 0);
             launch_one<N, kAmaxWide>(dmat, C, dt, A, S.host_counts[0], stream);
             if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
-            // more than kAmaxWide active systems at one point: not compiled in
-            if (S.host_counts[2] > 0) return hipErrorNotSupported;
+            counts[3] = S.host_counts[2];
+            if constexpr (N == kNGeneric) {
+                // tier 4: overflow of tier 3 (list_ovf); materials with more than
+                // 18 systems can activate up to kAmaxGeneric
+                if (S.host_counts[2] > 0) {
+                    hipLaunchKernelGGL(k_set1, dim3(1), dim3(64), 0, stream, S.counts + 3, S.host_counts[2]);
+                    hipLaunchKernelGGL(k_set1, dim3(1), dim3(64), 0, stream, S.counts + 0, 0);
+                    A.in_list = S.list_ovf;
+                    A.in_count = S.counts + 3;
+                    A.cont_list = S.list_b;
+                    A.cont_count = S.counts + 1;
+                    A.ovf_list = S.list_a;
+                    A.ovf_count = S.counts + 0;
+                    launch_one<N, kAmaxGeneric>(dmat, C, dt, A, S.host_counts[2], stream);
+                    if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
+                    if (S.host_counts[0] > 0) return hipErrorNotSupported;
+                }
+            } else {
+                // more than kAmaxWide active systems at one point: not compiled in
+                if (S.host_counts[2] > 0) return hipErrorNotSupported;
+            }
         }
     }
     return hipGetLastError();
@@ -724,7 +751,9 @@ hipError_t run_constitutive(int nss, const void* dmat, const DevControl& C, doub
     switch (nss) {
         case 12: return schedule<12>(dmat, C, dt, A, S, stream, counts);
         case 18: return schedule<18>(dmat, C, dt, A, S, stream, counts);
-        default: return hipErrorInvalidValue;
+        default:
+            if (nss >= 1 && nss <= kNGeneric) return schedule<kNGeneric>(dmat, C, dt, A, S, stream, counts);
+            return hipErrorInvalidValue;
     }
 }
 
@@ -850,6 +879,7 @@ extern "C" int cpfem_dstrss_batch(const cpfem_material* mat, const cpfem_control
     he = run_constitutive(nss, dm.p, C, dt, A, S, 0, counts);
     if (he == hipSuccess) he = hipDeviceSynchronize();
     free_schedule(S);
+    if (he == hipErrorNotSupported) return CPFEM_E_WIDTH;
     if (he != hipSuccess) return CPFEM_E_HIP;
     CPFEM_HIP(hipMemcpy(h.data(), dd.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
     CPFEM_HIP(hipMemcpy(hi.data(), di.p, hi.size() * sizeof(int32_t), hipMemcpyDeviceToHost));

@@ -30,6 +30,11 @@
 namespace cpfem {
 
 // Material tables of one material set, FP64, in the layout the kernel reads.
+// Slip-system counts with kernels of their own; every other count up to
+// CPFEM_MAXSS runs in the kernels compiled for kNGeneric systems, whose loops
+// over the systems take the material's nss at run time.
+constexpr int kNGeneric = 60;
+
 template <int N>
 struct DevMaterial {
     int nss, twinlim;
@@ -51,6 +56,12 @@ struct DevMaterial {
     int any_unidir;      // 1: some ssang != 0 (sign test needed)
     double hd, hq;
 };
+
+// Systems of a material: N, or nss for the generic kernels.
+template <int N>
+__device__ __forceinline__ int nsys(const DevMaterial<N>& M) {
+    return N == kNGeneric ? M.nss : N;
+}
 
 struct DevControl {
     double toler, alpha, afactor;
@@ -92,18 +103,24 @@ __device__ __forceinline__ double fj2vom(bool lstress, const double x[6]) {
     return sqrt(sum / 4.5);
 }
 
+// Unroll count of loops over an active set of width A: whole for the register
+// tiers, none for the wide generic tier, whose arrays live in scratch memory
+// anyway and whose unrolled code would be far larger than the instruction cache.
+template <int A>
+constexpr int kUnrollA = A > 12 ? 1 : A;
+
 // Doolittle LU without pivoting on an n x n register matrix w[row][col].
 template <int n>
 __device__ __forceinline__ bool factor(double (&w)[n][n]) {
     bool fail = false;
-#pragma unroll
+#pragma unroll kUnrollA<n>
     for (int k = 0; k < n - 1; ++k) {
         fail = fail || (w[k][k] == 0.0);
         const double piv = w[k][k];
-#pragma unroll
+#pragma unroll kUnrollA<n>
         for (int i = k + 1; i < n; ++i) {
             w[i][k] = w[i][k] / piv;
-#pragma unroll
+#pragma unroll kUnrollA<n>
             for (int j = k + 1; j < n; ++j) w[i][j] = w[i][j] - w[i][k] * w[k][j];
         }
     }
@@ -113,14 +130,14 @@ __device__ __forceinline__ bool factor(double (&w)[n][n]) {
 
 template <int n>
 __device__ __forceinline__ void solver(const double (&w)[n][n], double (&x)[n]) {
-#pragma unroll
+#pragma unroll kUnrollA<n>
     for (int i = 1; i < n; ++i)
-#pragma unroll
+#pragma unroll kUnrollA<n>
         for (int j = 0; j < i; ++j) x[i] = x[i] - w[i][j] * x[j];
     x[n - 1] = x[n - 1] / w[n - 1][n - 1];
-#pragma unroll
+#pragma unroll kUnrollA<n>
     for (int i = n - 2; i >= 0; --i) {
-#pragma unroll
+#pragma unroll kUnrollA<n>
         for (int j = i + 1; j < n; ++j) x[i] = x[i] - w[i][j] * x[j];
         x[i] = x[i] / w[i][i];
     }

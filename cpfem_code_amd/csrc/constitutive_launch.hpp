@@ -8,8 +8,9 @@
 
 namespace cpfem {
 
-// Widest active set of the compiled kernels.
-constexpr int kConstMaxWidth = 12;
+// Widest active set of the compiled kernels: 12 for the kernels of 12 and 18
+// systems (twin systems never enter the active set), 24 for the generic ones.
+constexpr int kConstMaxWidth = 24;
 
 // Device scratch of the schedule: checkpoints, per-point work arrays and the
 // point lists.

@@ -1174,6 +1174,7 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
     c->ninc1 = m->ninc1;
     c->nmat = nmat;
     c->ctl = *ctl;
+    c->mat0 = mats[0];
     c->precision = ctl->precision;
     c->omega_auto = ctl->sor_omega <= 0.0;
     if (c->omega_auto) c->ctl.sor_omega = 1.0;   // until the first solve tunes it
@@ -1420,7 +1421,11 @@ static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
     A.rng_epoch = c->rng_epoch;
     const DevControl C = make_control(c->ctl);
     int counts[4] = {0, 0, 0, 0};
-    CPFEM_HIP(run_constitutive(c->nss, c->dmat, C, dt, A, c->sched, s, counts));
+    {
+        const hipError_t he = run_constitutive(c->nss, c->dmat, C, dt, A, c->sched, s, counts);
+        if (he == hipErrorNotSupported) return CPFEM_E_WIDTH;
+        CPFEM_HIP(he);
+    }
     c->stats.pts_continued += counts[0];
     c->stats.pts_wide += counts[1];
     CPFEM_HIP(hipEventRecord(c->ev[1], s));
@@ -2029,21 +2034,13 @@ static int twin_tables(Ctx* c) {
     static_assert(sizeof(TwinTables) == 60 * 8 * 7, "layout");
     TwinTables h;
     memset(&h, 0, sizeof(h));
-    std::vector<char> hm(material_bytes(c->nss));
-    CPFEM_HIP(hipMemcpy(hm.data(), c->dmat, hm.size(), hipMemcpyDeviceToHost));
-    auto copy = [&h](const auto* M, int nss) {
-        for (int i = 0; i < nss; ++i) {
-            h.ssang[i] = M->ssang[i];
-            for (int k = 0; k < 3; ++k) {
-                h.ab[i][k] = M->ab[i][k];
-                h.an[i][k] = M->an[i][k];
-            }
+    for (int i = 0; i < c->nss; ++i) {
+        h.ssang[i] = c->mat0.ssang[i];
+        for (int k = 0; k < 3; ++k) {
+            h.ab[i][k] = c->mat0.ab[i][k];
+            h.an[i][k] = c->mat0.an[i][k];
         }
-    };
-    if (c->nss == 12)
-        copy(reinterpret_cast<DevMaterial<12>*>(hm.data()), 12);
-    else
-        copy(reinterpret_cast<DevMaterial<18>*>(hm.data()), 18);
+    }
     TwinTables* dT = nullptr;
     int* dcount = nullptr;
     if (dalloc(c, &dT, 1) || dalloc(c, &dcount, 1)) return CPFEM_E_HIP;

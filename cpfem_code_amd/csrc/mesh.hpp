@@ -77,6 +77,7 @@ struct Ctx {
     int nlmnt = 0, nnod = 0, ngps = 0, nsv = 0, nss = 0, ninc1 = 0, nmat = 0;
     int precision = 0;
     cpfem_control ctl;
+    cpfem_material mat0;         // host copy of material 1
     void* dmat = nullptr;
     hipStream_t stream = nullptr;
     // mesh

@@ -23,6 +23,7 @@ reference and the FP64 path read identical inputs.
 from __future__ import annotations
 
 import dataclasses
+import itertools
 import math
 from typing import Dict, List, Optional
 
@@ -201,11 +202,46 @@ def _mb_plane(h, k, i, l, ca):
     return n
 
 
-def hcp_magnesium(ca=1.624) -> Material:
+def bcc_iron(tau0=50.0, taus=150.0, h0=300.0, latent=1.4, m=0.02) -> Material:
+    """BCC iron: the 12 {110}<111> and 12 {112}<111> systems (24, a count with no
+    kernel of its own); C11=231.4, C12=134.7, C44=116.4 GPa, Voce hardening."""
+    dirs, norms = [], []
+    for d in ((1, 1, 1), (-1, 1, 1), (1, -1, 1), (1, 1, -1)):
+        d = np.array(d, float)
+        for fam in ((1, 1, 0), (1, 1, 2)):
+            cand = set()
+            for p in itertools.permutations(fam):
+                for sg in itertools.product((1, -1), repeat=3):
+                    nrm = tuple(a * b for a, b in zip(p, sg))
+                    if np.dot(nrm, d) == 0 and tuple(-x for x in nrm) not in cand:
+                        cand.add(nrm)
+            for nrm in sorted(cand, reverse=True):
+                dirs.append(d)
+                norms.append(np.array(nrm, float))
+    d = np.array(dirs)
+    n = np.array(norms)
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    n /= np.linalg.norm(n, axis=1)[:, None]
+    nss = len(d)
+    assert nss == 24 and np.allclose((d * n).sum(1), 0.0)
+    ampar = np.zeros((nss, 10))
+    ampar[:, 0] = m
+    ampar[:, 1] = h0
+    ampar[:, 3] = 1.0
+    ampar[:, 4] = taus
+    sm = np.full((nss, nss), latent)
+    np.fill_diagonal(sm, 1.0)
+    return Material(nss=nss, e=cubic_compliance(231.4, 134.7, 116.4), ssang=np.zeros(nss), ab=f32(d), an=f32(n),
+                    ampar=f32(ampar), smatrx=f32(sm), crss=f32(np.full(nss, tau0)))
+
+
+def hcp_magnesium(ca=1.624, pyramidal_a=False) -> Material:
     """Mg-like HCP: 3 basal, 3 prismatic, 6 pyramidal <c+a>, 6 {10-12} tensile twins.
 
     Elastic C11=59.7 C12=26.2 C13=21.7 C33=61.7 C44=16.4 GPa; CRSS 5/40/80/12 MPa.
     Twins carry ssang = 86.3 degrees (crys-cons.f90:165 treats |ssang| > 1.5 as twin).
+    pyramidal_a adds the 6 {10-11}<11-20> pyramidal <a> systems (CRSS 60 MPa)
+    after the prismatic ones: 24 systems, a count with no kernel of its own.
     """
     dirs, norms, kind = [], [], []
     basal_d = [(2, -1, -1, 0), (-1, 2, -1, 0), (-1, -1, 2, 0)]
@@ -214,6 +250,11 @@ def hcp_magnesium(ca=1.624) -> Material:
     prism = [((0, 1, -1, 0), (2, -1, -1, 0)), ((1, 0, -1, 0), (-1, 2, -1, 0)), ((1, -1, 0, 0), (1, 1, -2, 0))]
     for p, d in prism:
         dirs.append(_mb_direction(*d, ca)); norms.append(_mb_plane(*p, ca)); kind.append(1)
+    if pyramidal_a:
+        for p in ((1, 0, -1, 1), (0, 1, -1, 1), (-1, 1, 0, 1), (-1, 0, 1, 1), (0, -1, 1, 1), (1, -1, 0, 1)):
+            nv = _mb_plane(*p, ca)
+            dv = [v for v in (_mb_direction(*b, ca) for b in basal_d) if abs(np.dot(v, nv)) < 1e-12]
+            dirs.append(dv[0]); norms.append(nv); kind.append(4)
     pyr = [((1, 1, -2, 2), (-1, -1, 2, 3)), ((-1, 2, -1, 2), (1, -2, 1, 3)), ((-2, 1, 1, 2), (2, -1, -1, 3)),
            ((-1, -1, 2, 2), (1, 1, -2, 3)), ((1, -2, 1, 2), (-1, 2, -1, 3)), ((2, -1, -1, 2), (-2, 1, 1, 3))]
     for p, d in pyr:
@@ -227,7 +268,7 @@ def hcp_magnesium(ca=1.624) -> Material:
     n /= np.linalg.norm(n, axis=1)[:, None]
     assert np.allclose((d * n).sum(1), 0.0, atol=1e-12)
     nss = len(kind)
-    crss_by_kind = [5.0, 40.0, 80.0, 12.0]
+    crss_by_kind = [5.0, 40.0, 80.0, 12.0, 60.0]
     ampar = np.zeros((nss, 10))
     ampar[:, 0] = 0.02
     ampar[:, 1] = 200.0

@@ -26,8 +26,10 @@ extern "C" {
 
 #define CPFEM_E_ARG (-1)       /* bad argument or unsupported size */
 #define CPFEM_E_HIP (-2)       /* HIP runtime error */
-#define CPFEM_E_NSS (-3)       /* number of slip systems not compiled in */
+#define CPFEM_E_NSS (-3)       /* number of slip systems outside 1..CPFEM_MAXSS */
 #define CPFEM_E_NOGPU (-4)     /* no gfx950 device */
+#define CPFEM_E_WIDTH (-5)     /* a point's active set is wider than the compiled kernels: 12 systems
+                                  for materials of 12 or 18 systems, 24 for other counts */
 
 /* One material set as held in the .fsd file (Faslt_io_nowin.f90:77-107). */
 typedef struct {

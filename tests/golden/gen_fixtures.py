@@ -10,6 +10,8 @@ in REAL(4), so every stored real is float32.
 Fixtures:
   dstrss_fcc.npz  DSTRSS on 256 FCC points (seed 11, 10% elastic elements)
   dstrss_hcp.npz  DSTRSS on 128 HCP points with twin systems (seed 12)
+  dstrss_bcc.npz  DSTRSS on 256 BCC points, 24 systems (seed 13)
+  dstrss_hcp24.npz  DSTRSS on 128 HCP points, 18 slip + 6 twin systems (seed 14)
   nrstep_fcc.npz  ELVALS+SUVALS then FACSLS on a 3^3 one-point brick mesh
   nrstep20_fcc.npz  the same step on a 2^3 mesh of 20-node bricks (SHAPE_22)
   incs_fcc.npz    two increments of the reference loop on a 2^3 one-point mesh
@@ -39,8 +41,8 @@ sys.path.insert(0, HERE)
 import refio  # noqa: E402
 from gpbatch import make_gp_batch  # noqa: E402
 from cpfem_code_amd import fsd  # noqa: E402
-from cpfem_code_amd.problem import (LT_BRICK_1GP, LT_BRICK_20, Control, fcc_aluminium, fcc_tension,  # noqa: E402
-                                    hcp_magnesium, f32, random_euler)
+from cpfem_code_amd.problem import (LT_BRICK_1GP, LT_BRICK_20, Control, bcc_iron, fcc_aluminium,  # noqa: E402
+                                    fcc_tension, hcp_magnesium, f32, random_euler)
 
 DT = 1.0e-3
 
@@ -71,6 +73,8 @@ def main():
     assert refio.available(), "build the reference first: make -C oracle ref"
     dstrss_fixture("dstrss_fcc.npz", fcc_aluminium(), 256, 11, elastic_frac=0.1, stress=50.0)
     dstrss_fixture("dstrss_hcp.npz", hcp_magnesium(), 128, 12, elastic_frac=0.1, stress=25.0)
+    dstrss_fixture("dstrss_bcc.npz", bcc_iron(), 256, 13, elastic_frac=0.1, stress=120.0)
+    dstrss_fixture("dstrss_hcp24.npz", hcp_magnesium(pyramidal_a=True), 128, 14, elastic_frac=0.1, stress=25.0)
 
     p = fcc_tension(3, ninc=4, lt=LT_BRICK_1GP, seed=5)
     s = nrstep_state(p, 6)

@@ -9,7 +9,7 @@ import pytest
 
 from conftest import golden
 from cpfem_code_amd import capi
-from cpfem_code_amd.problem import Control, fcc_aluminium, hcp_magnesium
+from cpfem_code_amd.problem import Control, bcc_iron, fcc_aluminium, hcp_magnesium
 from gpbatch import make_gp_batch
 from oracle import Oracle
 
@@ -114,6 +114,25 @@ def test_hcp_fresh_batches_with_twins(seed):
     b = make_gp_batch(mat, 500, seed=300 + seed, elastic_frac=0.05, stress=30.0)
     g, o = compare_conditioned(mat, b)
     assert g["tflag"] == 1 and (g["twinsys"] > 0).any()
+
+
+@pytest.mark.parametrize("name", ["dstrss_bcc.npz", "dstrss_hcp24.npz"])
+def test_generic_system_count_fixtures(name):
+    """24 systems run in the kernels whose loops take nss at run time (no kernel
+    of their own); the fixtures are the compiled reference's, which the float
+    oracle reproduces bit for bit."""
+    z = golden(name)
+    mat = bcc_iron() if "bcc" in name else hcp_magnesium(pyramidal_a=True)
+    b = {k[3:]: z[k] for k in z.files if k.startswith("in_")}
+    compare_conditioned(mat, b, dt=float(z["dt"]))
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_bcc_fresh_batches(seed):
+    mat = bcc_iron()
+    b = make_gp_batch(mat, 1000, seed=500 + seed, elastic_frac=0.05, stress=110.0)
+    g, o = compare_conditioned(mat, b)
+    assert (g["deff"] > 0).mean() > 0.5
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3])

@@ -14,8 +14,8 @@ import pytest
 import scipy.sparse.linalg as spla
 
 from cpfem_code_amd import capi, prepro
-from cpfem_code_amd.problem import (Control, LT_BRICK_1GP, LT_BRICK_8GP, LT_BRICK_20, fcc_plane_strain, fcc_tension,
-                                    hcp_compression)
+from cpfem_code_amd.problem import (Control, LT_BRICK_1GP, LT_BRICK_8GP, LT_BRICK_20, bcc_iron, brick_problem,
+                                    fcc_plane_strain, fcc_tension, hcp_compression)
 from gpbatch import mesh_state
 from oracle import Oracle, OracleModel
 
@@ -158,6 +158,30 @@ def test_increments_match_oracle():
         e = np.abs(g["st"] - om.st) / np.abs(om.st).max()
         assert np.median(e) < 1e-7 and e.max() < 2e-3
         assert rel_max(g["sv"][:, 3:prob.nsv], om.sv[:, 3:prob.nsv]) < 1e-4
+    sv.close()
+
+
+def test_bcc_increments_match_oracle():
+    """A 24-system BCC material (generic system-count kernels) through whole
+    increments of uniaxial tension."""
+    ctl = Control(sor_kind=1, sor_tol=1e-12, sor_omega=1.6, sor_maxit=50000)
+    prob = brick_problem(3, material=bcc_iron(), ninc=3, seed=13,
+                         faces=dict(xmin="fixed", ymin="fixed", zmin="fixed", zmax="spec"),
+                         eps_inc=(0.0, 0.0, 2.0e-3), control=ctl)
+    om = OracleModel(O64, prob)
+    om.ctl.keep_going = 1
+    sv = capi.Solver(prob)
+    for inc in (1, 2):
+        e_o = om.increment(inc)
+        st = sv.increment(inc)
+        assert st["code"] == e_o
+        assert st["nit"] == om.m.nit
+        g = sv.get_state()
+        assert rel_max(g["x"], om.x) < 1e-7
+        e = np.abs(g["st"] - om.st) / np.abs(om.st).max()
+        assert np.median(e) < 1e-7 and e.max() < 2e-3
+        assert rel_max(g["sv"][:, 3:prob.nsv], om.sv[:, 3:prob.nsv]) < 1e-4
+    assert np.abs(om.sv[:, 3:prob.nsv] - prob.sv[:, 3:prob.nsv]).max() > 0   # the increments harden
     sv.close()
 
 

@@ -9,7 +9,8 @@ import numpy as np
 import pytest
 
 from conftest import golden
-from cpfem_code_amd.problem import Control, LT_BRICK_1GP, LT_BRICK_20, fcc_aluminium, fcc_tension, hcp_magnesium
+from cpfem_code_amd.problem import (Control, LT_BRICK_1GP, LT_BRICK_20, bcc_iron, fcc_aluminium, fcc_tension,
+                                    hcp_magnesium)
 from oracle import Oracle, OracleModel
 import refio
 
@@ -22,7 +23,9 @@ def _run_batch(o, mat, z):
                           z["in_ds"], z["in_sv"], z["in_dsv"])
 
 
-@pytest.mark.parametrize("name,mat", [("dstrss_fcc.npz", fcc_aluminium()), ("dstrss_hcp.npz", hcp_magnesium())])
+@pytest.mark.parametrize("name,mat", [("dstrss_fcc.npz", fcc_aluminium()), ("dstrss_hcp.npz", hcp_magnesium()),
+                                      ("dstrss_bcc.npz", bcc_iron()),
+                                      ("dstrss_hcp24.npz", hcp_magnesium(pyramidal_a=True))])
 def test_dstrss_float_oracle_bitexact(name, mat):
     z = golden(name)
     out = _run_batch(O32, mat, z)
