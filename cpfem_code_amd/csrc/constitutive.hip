@@ -208,7 +208,12 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
     }
     bool want_cont = false, want_ovf = false;
     if (live) {
-        const DevMaterial<N>& M = *Mp;
+        // material record of the point: |material code| (checked on the host).
+        // The kernels of a fixed count serve one material set, whose record is
+        // read through wave-uniform (scalar) loads; the generic ones index it
+        // per lane.
+        const int iel = A.mesh ? A.elem_le[A.gp_elem[g]] : A.iel[g];
+        const DevMaterial<N>& M = N == kNGeneric ? Mp[abs(iel) - 1] : Mp[0];
         const int NS = nsys(M);
         const size_t n = (size_t)A.n;
         double* el = A.el + g;
@@ -219,7 +224,6 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
         // mode) or the caller (batch mode); in mesh mode that kernel's error
         // code (920, 905, 906) seeds the point's code
         int kerr = 0;
-        const int iel = A.mesh ? A.elem_le[A.gp_elem[g]] : A.iel[g];
 #pragma unroll
         for (int i = 0; i < 6; ++i) de[i] = A.de[i * n + g];
 #pragma unroll

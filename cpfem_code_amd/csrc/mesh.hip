@@ -1111,10 +1111,15 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
                             cpfem_ctx** out) {
     if (!m || !mats || nmat < 1 || !ctl || !out) return CPFEM_E_ARG;
     *out = nullptr;
-    const int nss = mats[0].nss;
-    for (int k = 1; k < nmat; ++k)
-        if (mats[k].nss != nss) return CPFEM_E_ARG;
-    if (material_bytes(nss) == 0) return CPFEM_E_NSS;
+    // one material set of 12 or 18 systems runs in the kernels of its own count,
+    // anything else in the generic ones (record and count read per point)
+    int nss = 0;
+    for (int k = 0; k < nmat; ++k) {
+        if (material_bytes(mats[k].nss) == 0) return CPFEM_E_NSS;
+        nss = std::max(nss, (int)mats[k].nss);
+    }
+    int knss = mats[0].nss;
+    if (nmat > 1 || (knss != 12 && knss != 18)) knss = kNGeneric;
     // sv(1:3) Euler angles, sv(4:3+nss) slip resistances; FS3PRE may declare
     // more state variables (fs3p-gps.f90:30-31), which are carried unchanged
     if (m->nsv < 3 + nss || m->nsv > 63) return CPFEM_E_ARG;
@@ -1143,7 +1148,7 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
         } else {
             return CPFEM_E_ARG;
         }
-        if (kind[e] != kTie && (m->le[e] == 0 || abs(m->le[e]) > 1)) return CPFEM_E_ARG;  // one material set
+        if (kind[e] != kTie && (m->le[e] == 0 || abs(m->le[e]) > nmat)) return CPFEM_E_ARG;
         lt[e] = t;
         le[e] = m->le[e];
         np[e] = ng;
@@ -1171,6 +1176,7 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
     c->ngps = m->ngps;
     c->nsv = m->nsv;
     c->nss = nss;
+    c->knss = knss;
     c->ninc1 = m->ninc1;
     c->nmat = nmat;
     c->ctl = *ctl;
@@ -1259,13 +1265,14 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
         TRY(hipHostMalloc((void**)&c->red.host, 8 * sizeof(double), 0));
         // material
         {
-            const size_t mb = material_bytes(nss);
-            std::vector<char> hm(mb);
-            build_material(nss, hm.data(), mats[0], ctl->toler);
+            // one device record per material, indexed by |le| - 1
+            const size_t mb = material_bytes(knss);
+            std::vector<char> hm(mb * nmat);
+            for (int k = 0; k < nmat; ++k) build_material(knss, hm.data() + k * mb, mats[k], ctl->toler);
             char* dm = nullptr;
-            TRY(dalloc(c, &dm, mb));
+            TRY(dalloc(c, &dm, hm.size()));
             c->dmat = dm;
-            TRY(hipMemcpyAsync(dm, hm.data(), mb, hipMemcpyHostToDevice, c->stream));
+            TRY(hipMemcpyAsync(dm, hm.data(), hm.size(), hipMemcpyHostToDevice, c->stream));
             TRY(hipStreamSynchronize(c->stream));
         }
         TRY(alloc_schedule(c->sched, c->ngps));
@@ -1422,7 +1429,7 @@ static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
     const DevControl C = make_control(c->ctl);
     int counts[4] = {0, 0, 0, 0};
     {
-        const hipError_t he = run_constitutive(c->nss, c->dmat, C, dt, A, c->sched, s, counts);
+        const hipError_t he = run_constitutive(c->knss, c->dmat, C, dt, A, c->sched, s, counts);
         if (he == hipErrorNotSupported) return CPFEM_E_WIDTH;
         CPFEM_HIP(he);
     }
@@ -2034,7 +2041,7 @@ static int twin_tables(Ctx* c) {
     static_assert(sizeof(TwinTables) == 60 * 8 * 7, "layout");
     TwinTables h;
     memset(&h, 0, sizeof(h));
-    for (int i = 0; i < c->nss; ++i) {
+    for (int i = 0; i < c->mat0.nss; ++i) {
         h.ssang[i] = c->mat0.ssang[i];
         for (int k = 0; k < 3; ++k) {
             h.ab[i][k] = c->mat0.ab[i][k];

@@ -74,8 +74,9 @@ struct Matrix {
 };
 
 struct Ctx {
-    int nlmnt = 0, nnod = 0, ngps = 0, nsv = 0, nss = 0, ninc1 = 0, nmat = 0;
+    int nlmnt = 0, nnod = 0, ngps = 0, nsv = 0, nss = 0, ninc1 = 0, nmat = 0;   // nss: largest of the materials
     int precision = 0;
+    int knss = 0;                // slip-system count the constitutive kernels are compiled for
     cpfem_control ctl;
     cpfem_material mat0;         // host copy of material 1
     void* dmat = nullptr;

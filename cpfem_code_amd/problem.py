@@ -441,6 +441,25 @@ def brick_problem(n, length=1.0, *, lt=LT_BRICK_8GP, material: Optional[Material
 # the benchmark configurations
 # ---------------------------------------------------------------------------
 
+def with_second_phase(prob: Problem, material: Material, fraction: float = 0.5, seed: int = 0) -> Problem:
+    """The mesh with a random fraction of its elements given material code 2
+    (a second material set, as FS3PRE's material fractions assign them); their
+    points start from that material's slip resistances."""
+    rng = np.random.default_rng(seed)
+    le = prob.le.copy()
+    pick = (rng.random(prob.nlmnt) < fraction) & (prob.np_ > 0)
+    le[pick] = 2 * np.sign(le[pick])
+    sv = prob.sv.copy()
+    nss_max = max(prob.materials[0].nss, material.nss)
+    for e in np.nonzero(pick)[0]:
+        for k in range(prob.np_[e]):
+            gpt = prob.lg[e, k] - 1
+            sv[gpt, 3:3 + nss_max] = 0.0
+            sv[gpt, 3:3 + material.nss] = material.crss
+    return dataclasses.replace(prob, le=le, sv=f32(sv), nsv=3 + nss_max,
+                               materials=[prob.materials[0], material])
+
+
 def fcc_tension(n, ninc=50, seed=0, lt=LT_BRICK_8GP, **kw) -> Problem:
     """FCC aluminium, uniaxial tension along z, 1e-3 per increment, symmetry planes."""
     return brick_problem(n, lt=lt, material=fcc_aluminium(), ninc=ninc, seed=seed,

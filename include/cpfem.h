@@ -140,7 +140,8 @@ typedef struct {
 
 /* Upload mesh, materials and control; build the block sparsity, colourings
  * and element scatter map.  Replaces SETPTA (Stif_sol.f90:98), which the
- * reference calls once after RDDATA (Fasolt3.f90:24). */
+ * reference calls once after RDDATA (Fasolt3.f90:24).  mats[0..nmat-1] are the
+ * .fsd material sets; a point uses set |le| - 1 of its element (Sig_eps.f90:98). */
 int cpfem_create(const cpfem_mesh *mesh, const cpfem_material *mats, int32_t nmat,
                  const cpfem_control *ctl, cpfem_ctx **out);
 int cpfem_destroy(cpfem_ctx *ctx);

@@ -15,7 +15,7 @@ import scipy.sparse.linalg as spla
 
 from cpfem_code_amd import capi, prepro
 from cpfem_code_amd.problem import (Control, LT_BRICK_1GP, LT_BRICK_8GP, LT_BRICK_20, bcc_iron, brick_problem,
-                                    fcc_plane_strain, fcc_tension, hcp_compression)
+                                    fcc_aluminium, fcc_plane_strain, fcc_tension, hcp_compression, with_second_phase)
 from gpbatch import mesh_state
 from oracle import Oracle, OracleModel
 
@@ -182,6 +182,32 @@ def test_bcc_increments_match_oracle():
         assert np.median(e) < 1e-7 and e.max() < 2e-3
         assert rel_max(g["sv"][:, 3:prob.nsv], om.sv[:, 3:prob.nsv]) < 1e-4
     assert np.abs(om.sv[:, 3:prob.nsv] - prob.sv[:, 3:prob.nsv]).max() > 0   # the increments harden
+    sv.close()
+
+
+@pytest.mark.parametrize("second", ["fcc_soft", "bcc"])
+def test_two_material_increments_match_oracle(second):
+    """Two material sets on one mesh (material codes 1 and 2, the reference's
+    mat_code = |iel|, Sig_eps.f90:98): a softer FCC set (same kernels as one
+    material), or a 24-system BCC set (the generic kernels, mixed counts)."""
+    ctl = Control(sor_kind=1, sor_tol=1e-12, sor_omega=1.6, sor_maxit=50000)
+    base = fcc_tension(3, ninc=3, seed=17, control=ctl)
+    mat2 = fcc_aluminium(tau0=12.0, taus=50.0, h0=150.0) if second == "fcc_soft" else bcc_iron(tau0=30.0, taus=90.0)
+    prob = with_second_phase(base, mat2, fraction=0.5, seed=4)
+    assert set(np.abs(prob.le).tolist()) == {1, 2}
+    om = OracleModel(O64, prob)
+    om.ctl.keep_going = 1
+    sv = capi.Solver(prob)
+    for inc in (1, 2):
+        e_o = om.increment(inc)
+        st = sv.increment(inc)
+        assert st["code"] == e_o
+        assert st["nit"] == om.m.nit
+        g = sv.get_state()
+        assert rel_max(g["x"], om.x) < 1e-7
+        e = np.abs(g["st"] - om.st) / np.abs(om.st).max()
+        assert np.median(e) < 1e-7 and e.max() < 2e-3
+        assert rel_max(g["sv"][:, 3:prob.nsv], om.sv[:, 3:prob.nsv]) < 1e-4
     sv.close()
 
 
