@@ -37,6 +37,8 @@ NTAB = 99    # x_spec(1:15, 1:99, inc)
 LT_BRICK_1GP = 12
 LT_BRICK_8GP = 13
 LT_BRICK_20 = 22     # 20-node brick, 2x2x2 points (SHAPE_22, Shapef.f90:210)
+LT_TET10 = 21        # 10-node tetrahedron, 4 points (SHAPE_21, Shapef.f90:104)
+LT_WEDGE15 = 23      # 15-node pentahedron, 6 points (SHAPE_23, Shapef.f90:360)
 
 # local coordinates of the 20 nodes of SHAPE_22 (Shapef.f90:191-200): bottom
 # corners 1,3,5,7 and mid-sides 2,4,6,8 anticlockwise, 9-12 halfway up the
@@ -301,6 +303,43 @@ def basal_fibre_euler(rng: np.random.Generator, n: int, spread_deg: float = 15.0
     return np.stack([2 * math.pi * u[:, 0], phi, 2 * math.pi * u[:, 1]], 1)
 
 
+def _tet10_cells():
+    """The 6 tetrahedra of a cell around its (0,0,0)-(2,2,2) diagonal, nodes in
+    SHAPE_21 order (corners 1,3,5,10; 2,4,6 between 1-3, 3-5, 1-5; 7,8,9
+    between 10 and 1, 3, 5), oriented so that SHAPE_21's weight is positive."""
+    out = []
+    for perm in itertools.permutations(range(3)):
+        v = [np.zeros(3, int)]
+        for ax in perm:
+            nxt = v[-1].copy()
+            nxt[ax] += 2
+            v.append(nxt)
+        c1, c3, c5, c10 = v[0], v[1], v[2], v[3]
+        # SHAPE_21's local axes are (L10, L3, L1) with L5 = 1 - sum: weight =
+        # -det/24 with det = (x10 - x5) . ((x3 - x5) x (x1 - x5))
+        if np.dot(c10 - c5, np.cross(c3 - c5, c1 - c5)) > 0:
+            c1, c3 = c3, c1
+        mid = lambda a, b: (a + b) // 2
+        out.append([tuple(x) for x in (c1, mid(c1, c3), c3, mid(c3, c5), c5, mid(c1, c5),
+                                       mid(c1, c10), mid(c3, c10), mid(c5, c10), c10)])
+    return out
+
+
+def _wedge15_cells():
+    """The 2 prisms of a cell on its xy diagonal, nodes in SHAPE_23 order (base
+    corners 1,3,5 anticlockwise seen from above, top corners 10,12,14; 2,4,6 on
+    the base edges, 7,8,9 on the vertical ones, 11,13,15 on the top edges)."""
+    out = []
+    for tri in (((0, 0), (2, 0), (2, 2)), ((0, 0), (2, 2), (0, 2))):
+        b = [np.array([x, y, 0]) for x, y in tri]
+        t = [p + np.array([0, 0, 2]) for p in b]
+        mid = lambda a, c: (a + c) // 2
+        out.append([tuple(x) for x in (b[0], mid(b[0], b[1]), b[1], mid(b[1], b[2]), b[2], mid(b[2], b[0]),
+                                       mid(b[0], t[0]), mid(b[1], t[1]), mid(b[2], t[2]),
+                                       t[0], mid(t[0], t[1]), t[1], mid(t[1], t[2]), t[2], mid(t[2], t[0]))])
+    return out
+
+
 def brick_problem(n, length=1.0, *, lt=LT_BRICK_8GP, material: Optional[Material] = None,
                   faces: Optional[Dict[str, object]] = None, eps_inc=(0.0, 0.0, 1.0e-3),
                   ninc=50, seed=0, texture="random", control: Optional[Control] = None,
@@ -327,7 +366,33 @@ def brick_problem(n, length=1.0, *, lt=LT_BRICK_8GP, material: Optional[Material
     base = np.zeros((nel, 3), int)
     base[eidx] = np.stack([ei.ravel(), ej.ravel(), ek.ravel()], 1)
     ln = np.zeros((nel, 20), np.int32)
-    if lt == LT_BRICK_20:
+    if lt in (LT_TET10, LT_WEDGE15):
+        # each cell of the doubled grid's 3x3x3 points split into 6 tetrahedra
+        # around its (0,0,0)-(1,1,1) diagonal or 2 prisms on its xy diagonal;
+        # every edge midpoint is a grid point, and the splits conform
+        gx, gy, gz = 2 * nx + 1, 2 * ny + 1, 2 * nz + 1
+        cells = []
+        for e in range(nel):
+            b = 2 * base[e]
+            for loc in (_tet10_cells() if lt == LT_TET10 else _wedge15_cells()):
+                cells.append([(b[0] + p[0], b[1] + p[1], b[2] + p[2]) for p in loc])
+        cells = np.array(cells)                                 # (nel * k, nn, 3) grid indices
+        flat = cells[..., 0] + gx * (cells[..., 1] + gy * cells[..., 2])
+        order = np.unique(flat)                                 # x fastest over the used points
+        num = np.full(gx * gy * gz, -1, np.int64)
+        num[order] = np.arange(len(order))
+        nnod = len(order)
+        coords = np.zeros((nnod, 3))
+        coords[:, 0] = (order % gx) * lx / (2 * nx)
+        coords[:, 1] = ((order // gx) % gy) * ly / (2 * ny)
+        coords[:, 2] = (order // (gx * gy)) * lz / (2 * nz)
+        nd_el = flat.shape[1]
+        per = flat.shape[0] // nel
+        nel = flat.shape[0]
+        ln = np.zeros((nel, 20), np.int32)
+        ln[:, :nd_el] = 1 + num[flat]
+        base = np.repeat(base, per, axis=0)
+    elif lt == LT_BRICK_20:
         # grid of half cells; nodes are the points with at most one odd index
         gx, gy, gz = 2 * nx + 1, 2 * ny + 1, 2 * nz + 1
         I, J, K = np.meshgrid(np.arange(gx), np.arange(gy), np.arange(gz), indexing="ij")
@@ -362,7 +427,7 @@ def brick_problem(n, length=1.0, *, lt=LT_BRICK_8GP, material: Optional[Material
             ln[:, a] = 1 + (base[:, 0] + di) + nnx * ((base[:, 1] + dj) + nny * (base[:, 2] + dk))
         nd_el = 8
     coords = f32(coords)
-    npe = 1 if lt == LT_BRICK_1GP else 8
+    npe = {LT_BRICK_1GP: 1, LT_TET10: 4, LT_WEDGE15: 6}.get(lt, 8)
     ngps = nel * npe
     lg = np.zeros((nel, 8), np.int32)
     lg[:, :npe] = 1 + np.arange(ngps, dtype=np.int32).reshape(nel, npe)

@@ -896,6 +896,114 @@ static void shape20(int ig, const real *xnode, real *sf, real *sfd, real *weight
     *weight = R(1.0) * det;
 }
 
+/* SHAPE_21 (Shapef.f90:104-189) and SHAPE_23 (:360-485) also tabulate their
+ * functions and local derivatives to 6 decimals.  Generated here:
+ *  - 10-node tetrahedron: volume coordinates of the corners 1,3,5,10 with
+ *    local axes xi1 = L10, xi2 = L3, xi3 = L1 and L5 = 1 - xi1 - xi2 - xi3;
+ *    corner N = L(2L - 1), mid-edge N = 4 La Lb (2: 1-3, 4: 3-5, 6: 1-5,
+ *    7: 1-10, 8: 3-10, 9: 5-10); point k has L = a at the k-th corner (order
+ *    1,3,5,10) and b at the others, a = (5 + 3 sqrt 5)/20, b = (5 - sqrt 5)/20.
+ *    Rounded to 6 decimals this reproduces every tabulated entry.
+ *  - 15-node pentahedron: area coordinates L1 = xi1, L3 = xi2, L5 = 1 - xi1 -
+ *    xi2 of the base corners 1,3,5 (top corners 10,12,14 above them) and
+ *    zeta = xi3; corner N = L((2L - 1)(1 + zeta zc) - (1 - zeta^2))/2,
+ *    triangle-edge N = 2 La Lb (1 + zeta zc), vertical-edge N = L (1 - zeta^2);
+ *    points 1-3 at zeta = -1/sqrt 3 nearest corners 1, 3, 5 (area coordinates
+ *    2/3, 1/6, 1/6), points 4-6 at +1/sqrt 3.  This reproduces the tabulated
+ *    functions; 60 of the 270 tabulated derivatives differ from it by 1e-6 or
+ *    2e-6 (the table's own rounding noise), so on 15-node elements the
+ *    restatement agrees with the reference to that precision, not bit for bit. */
+static int tabq_ready = 0;
+static double tab21_n[4][10], tab21_d[4][3][10], tab23_n[6][15], tab23_d[6][3][15];
+
+static void tet10_tables(void) {
+    static const int corner_pos[4] = {0, 2, 4, 9};                 /* nodes 1,3,5,10 */
+    static const int mids[6][3] = {{1, 0, 1}, {3, 1, 2}, {5, 0, 2}, {6, 0, 3}, {7, 1, 3}, {8, 2, 3}};
+    /* d L_c / d xi for corners (1,3,5,10) */
+    static const double dl[4][3] = {{0, 0, 1}, {0, 1, 0}, {-1, -1, -1}, {1, 0, 0}};
+    const double a = (5.0 + 3.0 * sqrt(5.0)) / 20.0, b = (5.0 - sqrt(5.0)) / 20.0;
+    for (int ig = 0; ig < 4; ++ig) {
+        double L[4];
+        for (int c = 0; c < 4; ++c) L[c] = (c == ig) ? a : b;
+        for (int c = 0; c < 4; ++c) {
+            const int node = corner_pos[c];
+            tab21_n[ig][node] = round6(L[c] * (2.0 * L[c] - 1.0));
+            for (int j = 0; j < 3; ++j) tab21_d[ig][j][node] = round6((4.0 * L[c] - 1.0) * dl[c][j]);
+        }
+        for (int m = 0; m < 6; ++m) {
+            const int node = mids[m][0], p = mids[m][1], q = mids[m][2];
+            tab21_n[ig][node] = round6(4.0 * L[p] * L[q]);
+            for (int j = 0; j < 3; ++j) tab21_d[ig][j][node] = round6(4.0 * (L[q] * dl[p][j] + L[p] * dl[q][j]));
+        }
+    }
+}
+
+static void wedge15_tables(void) {
+    /* node -> (kind, corner a, corner b, zc): kind 0 corner, 1 triangle edge, 2 vertical edge;
+     * corners 0,1,2 = base corners 1,3,5 (area coordinates L1, L3, L5) */
+    static const int nodes[15][4] = {{0, 0, 0, -1}, {1, 0, 1, -1}, {0, 1, 1, -1}, {1, 1, 2, -1}, {0, 2, 2, -1},
+                                     {1, 2, 0, -1}, {2, 0, 0, 0},   {2, 1, 1, 0},   {2, 2, 2, 0},   {0, 0, 0, 1},
+                                     {1, 0, 1, 1},  {0, 1, 1, 1},   {1, 1, 2, 1},   {0, 2, 2, 1},   {1, 2, 0, 1}};
+    static const double dl[3][2] = {{1, 0}, {0, 1}, {-1, -1}};
+    const double z0 = 1.0 / sqrt(3.0);
+    for (int ig = 0; ig < 6; ++ig) {
+        double L[3];
+        for (int c = 0; c < 3; ++c) L[c] = (c == ig % 3) ? 2.0 / 3.0 : 1.0 / 6.0;
+        const double z = ig < 3 ? -z0 : z0;
+        for (int a = 0; a < 15; ++a) {
+            const int kind = nodes[a][0], p = nodes[a][1], q = nodes[a][2];
+            const double zc = nodes[a][3];
+            double n, d[3];
+            if (kind == 0) {
+                const double l = L[p], f = (2.0 * l - 1.0) * (1.0 + z * zc) - (1.0 - z * z);
+                n = 0.5 * l * f;
+                const double dldL = 0.5 * ((4.0 * l - 1.0) * (1.0 + z * zc) - (1.0 - z * z));
+                d[0] = dldL * dl[p][0];
+                d[1] = dldL * dl[p][1];
+                d[2] = 0.5 * l * ((2.0 * l - 1.0) * zc + 2.0 * z);
+            } else if (kind == 1) {
+                const double f = 1.0 + z * zc;
+                n = 2.0 * L[p] * L[q] * f;
+                for (int j = 0; j < 2; ++j) d[j] = 2.0 * f * (L[q] * dl[p][j] + L[p] * dl[q][j]);
+                d[2] = 2.0 * L[p] * L[q] * zc;
+            } else {
+                n = L[p] * (1.0 - z * z);
+                for (int j = 0; j < 2; ++j) d[j] = (1.0 - z * z) * dl[p][j];
+                d[2] = -2.0 * z * L[p];
+            }
+            tab23_n[ig][a] = round6(n);
+            for (int j = 0; j < 3; ++j) tab23_d[ig][j][a] = round6(d[j]);
+        }
+    }
+}
+
+/* The Jacobian, its inverse and the global derivatives of a tabulated
+ * element at one point (the loops of Shapef.f90:163-187 and 459-483);
+ * returns detJ. */
+static real shape_tab(int nn, const double *tn, const double *td, const real *xnode, real *sf, real *sfd) {
+    real dndn[20][3];
+    for (int a = 0; a < nn; ++a) {
+        sf[a] = R(tn[a]);
+        for (int j = 0; j < 3; ++j) dndn[a][j] = R(td[j * nn + a]);
+    }
+    real jac[9];
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 3; ++j) {
+            real sum = R(0.0);
+            for (int a = 0; a < nn; ++a) sum = sum + dndn[a][j] * xnode[3 * a + k];
+            jac[j + 3 * k] = sum;
+        }
+    real det;
+    invert_3(jac, &det);
+    for (int a = 0; a < nn; ++a)
+        for (int k = 0; k < 3; ++k) {
+            real sum = R(0.0);
+            for (int j = 0; j < 3; ++j) sum = sum + jac[k + 3 * j] * dndn[a][j];
+            sfd[3 * a + k] = sum;
+        }
+    return det;
+}
+
 /* Shape functions of element type lt at point ig (0-based); returns the
  * node count, 0 for a type this restatement does not cover. */
 static int shape_el(int lt, int ig, const real *xnode, real *sf, real *sfd, real *weight) {
@@ -906,6 +1014,21 @@ static int shape_el(int lt, int ig, const real *xnode, real *sf, real *sfd, real
     if (lt == 22) {
         shape20(ig, xnode, sf, sfd, weight);
         return 20;
+    }
+    if (lt == 21 || lt == 23) {
+        if (!tabq_ready) {
+            tet10_tables();
+            wedge15_tables();
+            tabq_ready = 1;
+        }
+        if (lt == 21) {
+            /* SHAPE_21: weight = -0.0416667 detJ (Shapef.f90:187) */
+            *weight = -(R(0.0416667) * shape_tab(10, tab21_n[ig], &tab21_d[ig][0][0], xnode, sf, sfd));
+            return 10;
+        }
+        /* SHAPE_23: weight = 0.16666667 detJ (Shapef.f90:483) */
+        *weight = R(0.16666667) * shape_tab(15, tab23_n[ig], &tab23_d[ig][0][0], xnode, sf, sfd);
+        return 15;
     }
     return 0;
 }
@@ -1489,7 +1612,7 @@ int orc_invert_3(double *mm, double *det) {
     return d != R(0.0);
 }
 void orc_shape(int lt, int ig, const double *xnode, double *sf, double *sfd, double *weight) {
-    const int nn = (lt == 22) ? 20 : 8;
+    const int nn = (lt == 22) ? 20 : (lt == 21) ? 10 : (lt == 23) ? 15 : 8;
     real x[60], f[20], d[60], w = R(0.0);
     for (int i = 0; i < 3 * nn; ++i) x[i] = R(xnode[i]);
     shape_el(lt, ig, x, f, d, &w);

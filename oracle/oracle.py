@@ -172,8 +172,8 @@ class Oracle:
         return m, det[0]
 
     def shape(self, lt, ig, xnode):
-        """Shape functions of element type lt (12, 13: 8 nodes; 22: 20 nodes) at point ig (0-based)."""
-        nn = 20 if lt == 22 else 8
+        """Shape functions of element type lt (12, 13: 8 nodes; 21: 10; 22: 20; 23: 15) at point ig (0-based)."""
+        nn = {22: 20, 21: 10, 23: 15}.get(lt, 8)
         xnode = np.ascontiguousarray(xnode, np.float64).reshape(nn, 3)
         sf = np.zeros(nn)
         sfd = np.zeros((nn, 3))

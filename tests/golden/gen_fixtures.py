@@ -14,6 +14,9 @@ Fixtures:
   dstrss_hcp24.npz  DSTRSS on 128 HCP points, 18 slip + 6 twin systems (seed 14)
   nrstep_fcc.npz  ELVALS+SUVALS then FACSLS on a 3^3 one-point brick mesh
   nrstep20_fcc.npz  the same step on a 2^3 mesh of 20-node bricks (SHAPE_22)
+  nrstep_tet10.npz, nrstep_wedge15.npz  the same step on 2^3 cells split into
+                  10-node tetrahedra (SHAPE_21) and 15-node pentahedra (SHAPE_23,
+                  elastic elements)
   incs_fcc.npz    two increments of the reference loop on a 2^3 one-point mesh
   fs3pre.npz      .fsd files the reference FS3PRE wrote for five answer sets
                   (8- and 20-node bricks, the four boundary options, ties),
@@ -41,8 +44,8 @@ sys.path.insert(0, HERE)
 import refio  # noqa: E402
 from gpbatch import make_gp_batch  # noqa: E402
 from cpfem_code_amd import fsd  # noqa: E402
-from cpfem_code_amd.problem import (LT_BRICK_1GP, LT_BRICK_20, Control, bcc_iron, fcc_aluminium,  # noqa: E402
-                                    fcc_tension, hcp_magnesium, f32, random_euler)
+from cpfem_code_amd.problem import (LT_BRICK_1GP, LT_BRICK_20, LT_TET10, LT_WEDGE15, Control,  # noqa: E402
+                                    bcc_iron, fcc_aluminium, fcc_tension, hcp_magnesium, f32, random_euler)
 
 DT = 1.0e-3
 
@@ -69,12 +72,30 @@ def nrstep_state(p, seed):
     return s
 
 
+def nrstep_fixture(name, p, seed):
+    """One NRITER step (ELVALS, SUVALS, FACSLS) of the reference on mesh p."""
+    s = nrstep_state(p, seed)
+    r = refio.nrstep(p, s, inc=1)
+    np.savez_compressed(os.path.join(HERE, name),
+                        in_dx=s["dx"].astype(np.float32), in_st=s["st"].astype(np.float32),
+                        in_ds=s["ds"].astype(np.float32),
+                        err=r["err"], fc=r["fc"].astype(np.float32), rhs=r["rhs"].astype(np.float32),
+                        is_=r["is"], ss=r["ss"].astype(np.float32),
+                        ds=r["state"]["ds"].astype(np.float32), dsv=r["state"]["dsv"].astype(np.float32),
+                        de=r["state"]["de"].astype(np.float32), dep=r["state"]["dep"].astype(np.float32),
+                        err_sor=r["err_sor"], ddx=r["ddx"].astype(np.float32))
+
+
 def main():
     assert refio.available(), "build the reference first: make -C oracle ref"
     dstrss_fixture("dstrss_fcc.npz", fcc_aluminium(), 256, 11, elastic_frac=0.1, stress=50.0)
     dstrss_fixture("dstrss_hcp.npz", hcp_magnesium(), 128, 12, elastic_frac=0.1, stress=25.0)
     dstrss_fixture("dstrss_bcc.npz", bcc_iron(), 256, 13, elastic_frac=0.1, stress=120.0)
     dstrss_fixture("dstrss_hcp24.npz", hcp_magnesium(pyramidal_a=True), 128, 14, elastic_frac=0.1, stress=25.0)
+    # 10-node tetrahedra (SHAPE_21) and 15-node pentahedra (SHAPE_23): one step on 2^3 cells
+    nrstep_fixture("nrstep_tet10.npz", fcc_tension(2, ninc=4, lt=LT_TET10, seed=5), 6)
+    nrstep_fixture("nrstep_wedge15.npz", fcc_tension(2, ninc=4, lt=LT_WEDGE15, seed=5,
+                                                     elastic_elements=np.arange(16)), 6)
 
     p = fcc_tension(3, ninc=4, lt=LT_BRICK_1GP, seed=5)
     s = nrstep_state(p, 6)

@@ -9,8 +9,8 @@ import numpy as np
 import pytest
 
 from conftest import golden
-from cpfem_code_amd.problem import (Control, LT_BRICK_1GP, LT_BRICK_20, bcc_iron, fcc_aluminium, fcc_tension,
-                                    hcp_magnesium)
+from cpfem_code_amd.problem import (Control, LT_BRICK_1GP, LT_BRICK_20, LT_TET10, LT_WEDGE15, bcc_iron,
+                                    fcc_aluminium, fcc_tension, hcp_magnesium)
 from oracle import Oracle, OracleModel
 import refio
 
@@ -99,6 +99,58 @@ def test_elvals_suvals_facsls_brick20_float_bitexact():
     e = om.facsls()
     assert max(e, int(z["err"])) == z["err_sor"]
     assert np.array_equal(om.ddx.astype(np.float32), z["ddx"])
+
+
+def _tab_model(lt, name, **kw):
+    p = fcc_tension(2, ninc=4, lt=lt, seed=5, **kw)
+    z = golden(name)
+    om = OracleModel(O32, p, maxofd=300)
+    om.dx[:] = z["in_dx"]
+    om.st[:] = z["in_st"]
+    om.ds[:] = z["in_ds"]
+    om.set_inc(1)
+    return p, z, om
+
+
+def test_elvals_suvals_facsls_tet10_float_bitexact():
+    """SHAPE_21 (10-node tetrahedron, Shapef.f90:104-189) through ELVALS, SUVALS
+    and FACSLS: its generated tables equal the reference's, so the step is
+    reproduced bit for bit."""
+    p, z, om = _tab_model(LT_TET10, "nrstep_tet10.npz")
+    assert om.nrstep() == z["err"]
+    assert np.array_equal(om.fc.astype(np.float32), z["fc"])
+    assert np.array_equal(om.ddx.astype(np.float32), z["rhs"])
+    assert np.array_equal(om.is_[:, :81], z["is_"])
+    assert np.array_equal(om.ss[:, :81].astype(np.float32), z["ss"])
+    for k in ("ds", "dsv", "de", "dep"):
+        assert np.array_equal(getattr(om, k).astype(np.float32), z[k]), k
+    e = om.facsls()
+    assert max(e, int(z["err"])) == z["err_sor"]
+    assert np.array_equal(om.ddx.astype(np.float32), z["ddx"])
+
+
+def test_elvals_suvals_facsls_wedge15_float_close():
+    """SHAPE_23 (15-node pentahedron, Shapef.f90:360-485): 60 of the
+    reference's 270 tabulated derivatives differ from the generated ones by
+    1e-6 or 2e-6, so the step agrees to that precision.  The elements are
+    elastic, whose update is linear (a plastic point's stiff slip law turns a
+    1e-6 change of its strain increment into 1e-3 of its stress): forces,
+    strain and stress increments, stiffness and solution to 1e-5 of their
+    largest values (measured 3e-6 at most)."""
+    p, z, om = _tab_model(LT_WEDGE15, "nrstep_wedge15.npz", elastic_elements=np.arange(16))
+    assert om.nrstep() == z["err"]
+    assert np.array_equal(om.is_[:, :81], z["is_"])
+    free = np.abs(z["ss"]).max(1) < 1e10        # rows without the efix penalty
+
+    def close(a, b, tol=1e-5):
+        a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+        return np.abs(a - b).max() <= tol * np.abs(b).max()
+    assert close(om.fc, z["fc"])
+    assert close(om.ss[:, :81][free], z["ss"][free])
+    for k in ("de", "ds", "dep"):
+        assert close(getattr(om, k), z[k]), k
+    om.facsls()
+    assert close(om.ddx, z["ddx"])
 
 
 def test_increments_float_bitexact():

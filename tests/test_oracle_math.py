@@ -7,6 +7,9 @@
   constant strain of a linear displacement field), weights summing to the
   volume of an affinely distorted cube, and the point order of SHAPE_22
   (Shapef.f90:203-204).
+* The generated tables of the 10-node tetrahedron and 15-node pentahedron
+  (SHAPE_21, SHAPE_23) on affinely mapped elements: partition of unity, patch
+  test and volume, to the tables' 6 decimals.
 * The Jacobi-scaled residual stopping rule of the SOR solve (sor_kind 1) is
   checked on the assembled system against a direct sparse solve.
 * DoReorient (Fasolt3.f90:79-178) is checked against an independent rotation:
@@ -19,7 +22,7 @@ import pytest
 import scipy.sparse.linalg as spla
 from scipy.spatial.transform import Rotation
 
-from cpfem_code_amd.problem import Control, LT_BRICK_8GP, fcc_tension, hcp_compression
+from cpfem_code_amd.problem import Control, LT_BRICK_8GP, LT_TET10, LT_WEDGE15, fcc_tension, hcp_compression
 from oracle import Oracle, OracleModel
 
 O64 = Oracle(64)
@@ -65,6 +68,32 @@ def test_brick8_weights_sum_to_volume(seed):
     x, a = _distorted(rng, affine_only=True)
     ws = [O64.shape(LT_BRICK_8GP, ig, x)[2] for ig in range(8)]
     assert abs(sum(ws) - abs(np.linalg.det(a))) < 1e-13
+
+
+@pytest.mark.parametrize("lt", [LT_TET10, LT_WEDGE15])
+@pytest.mark.parametrize("seed", range(3))
+def test_tabulated_elements_on_affine_images(lt, seed):
+    """10-node tetrahedra and 15-node pentahedra of the generated mesh, mapped
+    by a random affine map: the tables carry 6 decimals, so partition of unity
+    and the patch test hold to ~1e-6, and the weights (SHAPE_21's 0.0416667,
+    SHAPE_23's 0.16666667 and the tabulated values) sum to the mapped volume
+    to ~1e-6."""
+    rng = np.random.default_rng(seed)
+    p = fcc_tension(1, ninc=2, lt=lt)
+    a = np.eye(3) + 0.2 * rng.standard_normal((3, 3))
+    if np.linalg.det(a) < 0.3:
+        a = np.eye(3)
+    x = p.x.reshape(-1, 3) @ a.T + rng.standard_normal(3)
+    vol = 0.0
+    for e in range(p.nlmnt):
+        xl = x[p.ln[e, :p.nd[e]] - 1]
+        for ig in range(p.np_[e]):
+            sf, sfd, w = O64.shape(lt, ig, xl)
+            assert w > 0
+            assert abs(sf.sum() - 1.0) < 5e-6
+            assert np.abs(sfd.T @ xl - np.eye(3)).max() < 2e-5
+            vol += w
+    assert abs(vol - np.linalg.det(a)) < 3e-6 * np.linalg.det(a)
 
 
 def test_brick8_point_order_and_values():
