@@ -271,8 +271,98 @@ static hipError_t upload_brick20_tables() {
     return e;
 }
 
-// Global derivatives and weight (detJ) of the 20-node brick at point ig.
-__device__ __forceinline__ double brick20_shape(int ig, const double xl[20][3], double sfd[20][3]) {
+// SHAPE_21 (Shapef.f90:104-189) and SHAPE_23 (:360-485) tabulate their local
+// derivatives at the points to 6 decimals as well.  Generated here from the
+// quadratic functions:
+//  - 10-node tetrahedron: volume coordinates of corners 1,3,5,10 with local
+//    axes xi1 = L10, xi2 = L3, xi3 = L1, L5 = 1 - xi1 - xi2 - xi3; corner
+//    N = L(2L - 1), mid-edge N = 4 La Lb (2: 1-3, 4: 3-5, 6: 1-5, 7: 1-10,
+//    8: 3-10, 9: 5-10); point k has L = (5 + 3 sqrt 5)/20 at the k-th corner
+//    and (5 - sqrt 5)/20 at the others (this reproduces the whole table);
+//  - 15-node pentahedron: area coordinates L1 = xi1, L3 = xi2, L5 = 1 - xi1 -
+//    xi2 of base corners 1,3,5 (top corners 10,12,14 above them), zeta = xi3;
+//    corner N = L((2L - 1)(1 + zeta zc) - (1 - zeta^2))/2, triangle-edge N =
+//    2 La Lb (1 + zeta zc), vertical-edge N = L(1 - zeta^2); points 1-3 at
+//    zeta = -1/sqrt 3 nearest corners 1,3,5 (area coordinates 2/3, 1/6, 1/6),
+//    4-6 at +1/sqrt 3 (60 of the 270 tabulated values differ from this by
+//    1e-6 or 2e-6, the table's own rounding).
+__constant__ double kTab21D[4][3][10];
+__constant__ double kTab23D[6][3][15];
+
+static hipError_t upload_tet_wedge_tables() {
+    auto r6 = [](double v) { return round(v * 1.0e6) / 1.0e6; };
+    double t21[4][3][10], t23[6][3][15];
+    {
+        static const int corner_pos[4] = {0, 2, 4, 9};
+        static const int mids[6][3] = {{1, 0, 1}, {3, 1, 2}, {5, 0, 2}, {6, 0, 3}, {7, 1, 3}, {8, 2, 3}};
+        static const double dl[4][3] = {{0, 0, 1}, {0, 1, 0}, {-1, -1, -1}, {1, 0, 0}};
+        const double a = (5.0 + 3.0 * sqrt(5.0)) / 20.0, b = (5.0 - sqrt(5.0)) / 20.0;
+        for (int ig = 0; ig < 4; ++ig) {
+            double L[4];
+            for (int c = 0; c < 4; ++c) L[c] = (c == ig) ? a : b;
+            for (int c = 0; c < 4; ++c)
+                for (int j = 0; j < 3; ++j) t21[ig][j][corner_pos[c]] = r6((4.0 * L[c] - 1.0) * dl[c][j]);
+            for (int m = 0; m < 6; ++m) {
+                const int p = mids[m][1], q = mids[m][2];
+                for (int j = 0; j < 3; ++j) t21[ig][j][mids[m][0]] = r6(4.0 * (L[q] * dl[p][j] + L[p] * dl[q][j]));
+            }
+        }
+    }
+    {
+        // node -> (kind: 0 corner, 1 triangle edge, 2 vertical edge; corners p, q of the base; zc)
+        static const int nodes[15][4] = {{0, 0, 0, -1}, {1, 0, 1, -1}, {0, 1, 1, -1}, {1, 1, 2, -1}, {0, 2, 2, -1},
+                                         {1, 2, 0, -1}, {2, 0, 0, 0},   {2, 1, 1, 0},   {2, 2, 2, 0},   {0, 0, 0, 1},
+                                         {1, 0, 1, 1},  {0, 1, 1, 1},   {1, 1, 2, 1},   {0, 2, 2, 1},   {1, 2, 0, 1}};
+        static const double dl[3][2] = {{1, 0}, {0, 1}, {-1, -1}};
+        const double z0 = 1.0 / sqrt(3.0);
+        for (int ig = 0; ig < 6; ++ig) {
+            double L[3];
+            for (int c = 0; c < 3; ++c) L[c] = (c == ig % 3) ? 2.0 / 3.0 : 1.0 / 6.0;
+            const double z = ig < 3 ? -z0 : z0;
+            for (int a = 0; a < 15; ++a) {
+                const int kind = nodes[a][0], p = nodes[a][1], q = nodes[a][2];
+                const double zc = nodes[a][3];
+                double d[3];
+                if (kind == 0) {
+                    const double l = L[p];
+                    const double dn = 0.5 * ((4.0 * l - 1.0) * (1.0 + z * zc) - (1.0 - z * z));
+                    d[0] = dn * dl[p][0];
+                    d[1] = dn * dl[p][1];
+                    d[2] = 0.5 * l * ((2.0 * l - 1.0) * zc + 2.0 * z);
+                } else if (kind == 1) {
+                    const double f = 1.0 + z * zc;
+                    for (int j = 0; j < 2; ++j) d[j] = 2.0 * f * (L[q] * dl[p][j] + L[p] * dl[q][j]);
+                    d[2] = 2.0 * L[p] * L[q] * zc;
+                } else {
+                    for (int j = 0; j < 2; ++j) d[j] = (1.0 - z * z) * dl[p][j];
+                    d[2] = -2.0 * z * L[p];
+                }
+                for (int j = 0; j < 3; ++j) t23[ig][j][a] = r6(d[j]);
+            }
+        }
+    }
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(kTab21D), t21, sizeof(t21));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(kTab23D), t23, sizeof(t23));
+    return e;
+}
+
+// Local derivative table entry d N_a / d xi_j at point ig of the NN-node
+// tabulated element.
+template <int NN>
+__device__ __forceinline__ double tab_dn(int ig, int j, int a) {
+    if constexpr (NN == 20)
+        return kTab22D[ig][j][a];
+    else if constexpr (NN == 10)
+        return kTab21D[ig][j][a];
+    else
+        return kTab23D[ig][j][a];
+}
+
+// Global derivatives and weight of a tabulated element at point ig: detJ
+// (SHAPE_22), -0.0416667 detJ (SHAPE_21, Shapef.f90:187) or 0.16666667 detJ
+// (SHAPE_23, Shapef.f90:483).
+template <int NN>
+__device__ __forceinline__ double tab_shape(int ig, const double xl[NN][3], double sfd[NN][3]) {
     double jac[3][3];  // jac[j][k] = sum_a dndn(j,a) x(k,a)
 #pragma unroll
     for (int k = 0; k < 3; ++k)
@@ -280,20 +370,25 @@ __device__ __forceinline__ double brick20_shape(int ig, const double xl[20][3], 
         for (int j = 0; j < 3; ++j) {
             double s = 0.0;
 #pragma unroll
-            for (int a = 0; a < 20; ++a) s = s + kTab22D[ig][j][a] * xl[a][k];
+            for (int a = 0; a < NN; ++a) s = s + tab_dn<NN>(ig, j, a) * xl[a][k];
             jac[j][k] = s;
         }
     const double det = invert_3(jac);
 #pragma unroll
-    for (int a = 0; a < 20; ++a)
+    for (int a = 0; a < NN; ++a)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             double s = 0.0;
 #pragma unroll
-            for (int j = 0; j < 3; ++j) s = s + jac[k][j] * kTab22D[ig][j][a];
+            for (int j = 0; j < 3; ++j) s = s + jac[k][j] * tab_dn<NN>(ig, j, a);
             sfd[a][k] = s;
         }
-    return 1.0 * det;
+    if constexpr (NN == 20)
+        return 1.0 * det;
+    else if constexpr (NN == 10)
+        return -(0.0416667 * det);
+    else
+        return 0.16666667 * det;
 }
 
 // ---------------------------------------------------------------------------
@@ -335,7 +430,7 @@ __global__ __launch_bounds__(kBlock) void k_kin(const KinArgs K) {
     if constexpr (NN == 8)
         w = brick_shape(K.elem_lt[e], K.gp_local[g], xl, sfd);
     else
-        w = brick20_shape(K.gp_local[g], xl, sfd);
+        w = tab_shape<NN>(K.gp_local[g], xl, sfd);
     int kerr = (w < 1.e-7) ? 920 : 0;
     double de[6], r[3], deff;
     const int e2 = dstrns<NN>(sfd, dxl, K.smale, K.smalr, de, r, deff);
@@ -354,7 +449,22 @@ __global__ __launch_bounds__(kBlock) void k_kin(const KinArgs K) {
 // group 0: (weight, weight*eps, weight*st11, weight*st22), 1: (st33, st23,
 // st31, st12), 2: (ep11, ep22, ep33, ep23), 3: (ep31, ep12, 0, 0)
 // ---------------------------------------------------------------------------
+// Integration weight of point g (element e, NN nodes) at the current coordinates.
 template <int NN>
+__device__ __noinline__ double point_weight(const KinArgs& K, int g, int e) {
+    double xl[NN][3], sfd[NN][3];
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+        const int nd = K.elem_nodes[kMaxNodes * e + a];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) xl[a][k] = K.x[3 * nd + k];
+    }
+    if constexpr (NN == 8)
+        return brick_shape(K.elem_lt[e], K.gp_local[g], xl, sfd);
+    else
+        return tab_shape<NN>(K.gp_local[g], xl, sfd);
+}
+
 __global__ __launch_bounds__(kBlock) void k_vol_sums(const KinArgs K, const double* __restrict__ st,
                                                      const double* __restrict__ ep, const double* __restrict__ eps,
                                                      int group, double* partial, int slot0) {
@@ -362,19 +472,12 @@ __global__ __launch_bounds__(kBlock) void k_vol_sums(const KinArgs K, const doub
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < K.n) {
         const int e = K.gp_elem[g];
-        if (K.elem_nd[e] == NN) {
-            double xl[NN][3], sfd[NN][3];
-#pragma unroll
-            for (int a = 0; a < NN; ++a) {
-                const int nd = K.elem_nodes[kMaxNodes * e + a];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) xl[a][k] = K.x[3 * nd + k];
-            }
-            double w;
-            if constexpr (NN == 8)
-                w = brick_shape(K.elem_lt[e], K.gp_local[g], xl, sfd);
-            else
-                w = brick20_shape(K.gp_local[g], xl, sfd);
+        const int nn = K.elem_nd[e];
+        if (nn == 8 || nn == 10 || nn == 15 || nn == 20) {
+            const double w = nn == 8    ? point_weight<8>(K, g, e)
+                             : nn == 10 ? point_weight<10>(K, g, e)
+                             : nn == 15 ? point_weight<15>(K, g, e)
+                                        : point_weight<20>(K, g, e);
             const size_t n = (size_t)K.n;
             if (group == 0) {
                 v[0] = w;
@@ -397,34 +500,35 @@ __global__ __launch_bounds__(kBlock) void k_vol_sums(const KinArgs K, const doub
 }
 
 // ---------------------------------------------------------------------------
-// 20-node element assembly: one 256-thread block per element, thread = node
-// pair (i, j) of the 400 (two per thread for the first 144)
+// Assembly of the tabulated quadratic elements (20-node bricks, 10-node
+// tetrahedra, 15-node pentahedra): one 256-thread block per element, thread
+// = node pair (i, j) of the NN^2 (several per thread above 256)
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void k_assemble20(const AsmArgs A, T* __restrict__ vals) {
-    __shared__ double sh_x[20][3];
-    __shared__ double sh_dn64[8][20][3];
+template <typename T, int NN>
+__global__ __launch_bounds__(256) void k_assemble_tab(const AsmArgs A, T* __restrict__ vals) {
+    __shared__ double sh_x[NN][3];
+    __shared__ double sh_dn64[8][NN][3];
     __shared__ double sh_w64[8];
     __shared__ double sh_s64[8][6];
-    __shared__ T sh_dn[8][20][3];
+    __shared__ T sh_dn[8][NN][3];
     __shared__ T sh_w[8];
     __shared__ T sh_d[8][36];
     __shared__ T sh_s[8][6];
     const int e = A.elems[blockIdx.x];
     const int tid = threadIdx.x;
     const int ng = A.elem_np[e];
-    if (tid < 60) sh_x[tid / 3][tid % 3] = A.x[3 * A.elem_nodes[kMaxNodes * e + tid / 3] + tid % 3];
+    if (tid < 3 * NN) sh_x[tid / 3][tid % 3] = A.x[3 * A.elem_nodes[kMaxNodes * e + tid / 3] + tid % 3];
     __syncthreads();
     const size_t n = (size_t)A.ngps;
     if (tid < ng) {
-        double xl[20][3], sfd[20][3];
+        double xl[NN][3], sfd[NN][3];
 #pragma unroll
-        for (int a = 0; a < 20; ++a)
+        for (int a = 0; a < NN; ++a)
 #pragma unroll
             for (int c = 0; c < 3; ++c) xl[a][c] = sh_x[a][c];
-        const double w = brick20_shape(tid, xl, sfd);
+        const double w = tab_shape<NN>(tid, xl, sfd);
 #pragma unroll
-        for (int a = 0; a < 20; ++a)
+        for (int a = 0; a < NN; ++a)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 sh_dn64[tid][a][c] = sfd[a][c];
@@ -444,8 +548,8 @@ __global__ __launch_bounds__(256) void k_assemble20(const AsmArgs A, T* __restri
     }
     __syncthreads();
     const int base = A.scat_start[e];
-    for (int p = tid; p < 400; p += 256) {
-        const int i = p / 20, j = p - 20 * (p / 20);
+    for (int p = tid; p < NN * NN; p += 256) {
+        const int i = p / NN, j = p - NN * (p / NN);
         T acc[3][3];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
@@ -477,7 +581,7 @@ __global__ __launch_bounds__(256) void k_assemble20(const AsmArgs A, T* __restri
             }
     }
     // internal forces of node tid (Elvals.f90:122-126)
-    if (tid < 20) {
+    if (tid < NN) {
         double fl[3] = {0.0, 0.0, 0.0};
         for (int g = 0; g < ng; ++g) {
             const double w = sh_w64[g];
@@ -1070,13 +1174,13 @@ static int build_structure(Ctx* c, const cpfem_mesh* m, const std::vector<int>& 
         necol = std::max(necol, k + 1);
     }
     // elements ordered by colour, then kind
-    c->ekstart.assign(3 * necol + 1, 0);
-    for (int e = 0; e < nel; ++e) c->ekstart[3 * ecol[e] + kind[e] + 1]++;
-    for (int k = 0; k < 3 * necol; ++k) c->ekstart[k + 1] += c->ekstart[k];
+    c->ekstart.assign(kKinds * necol + 1, 0);
+    for (int e = 0; e < nel; ++e) c->ekstart[kKinds * ecol[e] + kind[e] + 1]++;
+    for (int k = 0; k < kKinds * necol; ++k) c->ekstart[k + 1] += c->ekstart[k];
     std::vector<int> ebc(nel);
     {
         std::vector<int> pos(c->ekstart.begin(), c->ekstart.end() - 1);
-        for (int e = 0; e < nel; ++e) ebc[pos[3 * ecol[e] + kind[e]]++] = e;
+        for (int e = 0; e < nel; ++e) ebc[pos[kKinds * ecol[e] + kind[e]]++] = e;
     }
     c->A.R = R;
     c->A.S = S;
@@ -1134,7 +1238,7 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
     std::vector<int> en((size_t)nel * kMaxNodes, 0), eg((size_t)nel * 8, -1), lt(nel), le(nel), np(nel), nde(nel),
         kind(nel);
     std::vector<int> gpe(m->ngps, -1), gpl(m->ngps, -1);
-    int npts8 = 0, npts20 = 0;
+    int npts8 = 0, npts20 = 0, npts10 = 0, npts15 = 0;
     for (int e = 0; e < nel; ++e) {
         const int t = m->lt[e], nd = m->nd[e], ng = m->np[e];
         if (t == 12 && nd == 8 && ng == 1) {
@@ -1143,6 +1247,10 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
             kind[e] = kElem8;
         } else if (t == 22 && nd == 20 && ng == 8) {
             kind[e] = kElem20;
+        } else if (t == 21 && nd == 10 && ng == 4) {
+            kind[e] = kElem10;
+        } else if (t == 23 && nd == 15 && ng == 6) {
+            kind[e] = kElem15;
         } else if (t >= 1 && t <= 6 && nd == 2 && ng == 0) {
             kind[e] = kTie;
         } else {
@@ -1167,6 +1275,8 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
         }
         if (kind[e] == kElem8) npts8 += ng;
         if (kind[e] == kElem20) npts20 += ng;
+        if (kind[e] == kElem10) npts10 += ng;
+        if (kind[e] == kElem15) npts15 += ng;
     }
     for (int g = 0; g < m->ngps; ++g)
         if (gpe[g] < 0) return CPFEM_E_ARG;
@@ -1187,6 +1297,8 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
     c->tiem.assign(m->tiem, m->tiem + m->ninc1);
     c->npts8 = npts8;
     c->npts20 = npts20;
+    c->npts10 = npts10;
+    c->npts15 = npts15;
     memset(&c->stats, 0, sizeof(c->stats));
     int rc = 0;
 #define TRY(x)                      \
@@ -1218,6 +1330,7 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
         TRY(hipMemcpyAsync(c->elem_np, np.data(), nel * 4, hipMemcpyHostToDevice, c->stream));
         TRY(hipMemcpyAsync(c->elem_nd, nde.data(), nel * 4, hipMemcpyHostToDevice, c->stream));
         if (npts20 > 0) TRY(upload_brick20_tables());
+        if (npts10 + npts15 > 0) TRY(upload_tet_wedge_tables());
         TRY(hipMemcpyAsync(c->gp_elem, gpe.data(), G * 4, hipMemcpyHostToDevice, c->stream));
         TRY(hipMemcpyAsync(c->gp_local, gpl.data(), G * 4, hipMemcpyHostToDevice, c->stream));
         TRY(hipMemcpyAsync(c->ib, m->ib, c->nnod * 4, hipMemcpyHostToDevice, c->stream));
@@ -1401,6 +1514,8 @@ static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
     K.kerr = c->sched.ck_kerr;
     if (c->npts8 > 0) hipLaunchKernelGGL(k_kin<8>, dim3(nblk(c->ngps)), dim3(kBlock), 0, s, K);
     if (c->npts20 > 0) hipLaunchKernelGGL(k_kin<20>, dim3(nblk(c->ngps)), dim3(kBlock), 0, s, K);
+    if (c->npts10 > 0) hipLaunchKernelGGL(k_kin<10>, dim3(nblk(c->ngps)), dim3(kBlock), 0, s, K);
+    if (c->npts15 > 0) hipLaunchKernelGGL(k_kin<15>, dim3(nblk(c->ngps)), dim3(kBlock), 0, s, K);
     ConstArgs A;
     memset(&A, 0, sizeof(A));
     A.n = c->ngps;
@@ -1459,10 +1574,10 @@ static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
     AA.ngps = c->ngps;
     AA.fc = c->fc;
     const bool f32 = c->precision == 1;
-    const int ncol_el = ((int)c->ekstart.size() - 1) / 3;
+    const int ncol_el = ((int)c->ekstart.size() - 1) / kKinds;
     for (int k = 0; k < ncol_el; ++k) {
-        for (int q = 0; q < 3; ++q) {
-            const int b0 = c->ekstart[3 * k + q], cnt = c->ekstart[3 * k + q + 1] - b0;
+        for (int q = 0; q < kKinds; ++q) {
+            const int b0 = c->ekstart[kKinds * k + q], cnt = c->ekstart[kKinds * k + q + 1] - b0;
             if (cnt == 0) continue;
             AA.nel_color = cnt;
             AA.elems = c->elem_by_color + b0;
@@ -1474,9 +1589,22 @@ static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
                     hipLaunchKernelGGL(k_assemble<double>, dim3(nb), dim3(256), 0, s, AA, (double*)c->A.vals);
             } else if (q == kElem20) {
                 if (f32)
-                    hipLaunchKernelGGL(k_assemble20<float>, dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
+                    hipLaunchKernelGGL((k_assemble_tab<float, 20>), dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
                 else
-                    hipLaunchKernelGGL(k_assemble20<double>, dim3(cnt), dim3(256), 0, s, AA, (double*)c->A.vals);
+                    hipLaunchKernelGGL((k_assemble_tab<double, 20>), dim3(cnt), dim3(256), 0, s, AA,
+                                       (double*)c->A.vals);
+            } else if (q == kElem10) {
+                if (f32)
+                    hipLaunchKernelGGL((k_assemble_tab<float, 10>), dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
+                else
+                    hipLaunchKernelGGL((k_assemble_tab<double, 10>), dim3(cnt), dim3(256), 0, s, AA,
+                                       (double*)c->A.vals);
+            } else if (q == kElem15) {
+                if (f32)
+                    hipLaunchKernelGGL((k_assemble_tab<float, 15>), dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
+                else
+                    hipLaunchKernelGGL((k_assemble_tab<double, 15>), dim3(cnt), dim3(256), 0, s, AA,
+                                       (double*)c->A.vals);
             } else {
                 if (f32)
                     hipLaunchKernelGGL(k_ties<float>, dim3(nblk(cnt)), dim3(kBlock), 0, s, cnt, AA.elems, c->elem_lt,
@@ -2092,23 +2220,13 @@ extern "C" int cpfem_volume_average(cpfem_ctx* c, double* out) {
     K.elem_nd = c->elem_nd;
     K.x = c->x;
     const int nb = nblk(c->ngps);
-    if (2 * nb > c->red.nblocks) return CPFEM_E_ARG;
+    if (nb > c->red.nblocks) return CPFEM_E_ARG;
     double sums[16];
     for (int group = 0; group < 4; ++group) {
-        if (c->npts8 > 0)
-            hipLaunchKernelGGL(k_vol_sums<8>, dim3(nb), dim3(kBlock), 0, c->stream, K, (const double*)c->st,
-                               (const double*)c->ep, (const double*)c->eps, group, c->red.partial, 0);
-        else
-            hipLaunchKernelGGL(k_fill<double>, dim3(nblk(4 * (size_t)nb)), dim3(kBlock), 0, c->stream,
-                               c->red.partial, 4 * (size_t)nb, 0.0);
-        if (c->npts20 > 0)
-            hipLaunchKernelGGL(k_vol_sums<20>, dim3(nb), dim3(kBlock), 0, c->stream, K, (const double*)c->st,
-                               (const double*)c->ep, (const double*)c->eps, group, c->red.partial, nb);
-        else
-            hipLaunchKernelGGL(k_fill<double>, dim3(nblk(4 * (size_t)nb)), dim3(kBlock), 0, c->stream,
-                               c->red.partial + 4 * (size_t)nb, 4 * (size_t)nb, 0.0);
+        hipLaunchKernelGGL(k_vol_sums, dim3(nb), dim3(kBlock), 0, c->stream, K, (const double*)c->st,
+                           (const double*)c->ep, (const double*)c->eps, group, c->red.partial, 0);
         CPFEM_HIP(hipGetLastError());
-        reduce_now(c, 2 * nb, sums + 4 * group);
+        reduce_now(c, nb, sums + 4 * group);
     }
     const double vol = sums[0];
     out[0] = vol;

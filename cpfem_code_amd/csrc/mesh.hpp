@@ -39,7 +39,10 @@ struct Reducer {
 };
 
 constexpr int kMaxNodes = 20;                    // nodes per element (20-node brick)
-enum ElemKind { kElem8 = 0, kElem20 = 1, kTie = 2 };
+// Element kinds: 8-node bricks, the tabulated quadratic elements (20-node
+// bricks, 10-node tetrahedra, 15-node pentahedra) and two-node ties.
+enum ElemKind { kElem8 = 0, kElem20 = 1, kTie = 2, kElem10 = 3, kElem15 = 4 };
+constexpr int kKinds = 5;
 
 constexpr int kRowLanes = 9;                     // lanes per matrix row
 constexpr int kGroupRows = 64 / kRowLanes;       // rows per group (= per wave)
@@ -91,10 +94,11 @@ struct Ctx {
     int* gp_elem = nullptr;      // [ngps]
     int* gp_local = nullptr;     // [ngps]
     int* ib = nullptr;           // [nnod]
-    // elements by colour and, within a colour, by kind (kElem8, kElem20, kTie):
-    // kind q of colour k is elem_by_color[ekstart[3k+q] .. ekstart[3k+q+1])
+    // elements by colour and, within a colour, by kind (ElemKind):
+    // kind q of colour k is elem_by_color[ekstart[kKinds*k+q] .. ekstart[kKinds*k+q+1])
     std::vector<int> ekstart;
     int npts8 = 0, npts20 = 0;   // points of 8-node and of 20-node elements
+    int npts10 = 0, npts15 = 0;  // points of 10-node and of 15-node elements
     int* elem_by_color = nullptr;
     double* x_spec = nullptr;    // [ninc1][99][15]
     double* f_spec = nullptr;    // [ninc1][99][3]

@@ -111,10 +111,73 @@ def _brick20_tables():
 _TAB20 = None
 
 
+def _tet10_tables():
+    """WEIGHT_1's tables (data_out.f90:396, the same as SHAPE_21's): quadratic
+    tetrahedron in volume coordinates of corners 1,3,5,10 with local axes
+    (L10, L3, L1), L5 = 1 - their sum; point k has (5 + 3 sqrt 5)/20 at the
+    k-th corner and (5 - sqrt 5)/20 at the others; 6 decimals."""
+    corners = {0: 0, 2: 1, 4: 2, 9: 3}
+    mids = {1: (0, 1), 3: (1, 2), 5: (0, 2), 6: (0, 3), 7: (1, 3), 8: (2, 3)}
+    dl = np.array([[0, 0, 1], [0, 1, 0], [-1, -1, -1], [1, 0, 0]], float)
+    a, b = (5 + 3 * math.sqrt(5)) / 20, (5 - math.sqrt(5)) / 20
+    tn, td = np.zeros((4, 10)), np.zeros((4, 3, 10))
+    for ig in range(4):
+        L = [a if c == ig else b for c in range(4)]
+        for node, c in corners.items():
+            tn[ig, node] = L[c] * (2 * L[c] - 1)
+            td[ig, :, node] = (4 * L[c] - 1) * dl[c]
+        for node, (p, q) in mids.items():
+            tn[ig, node] = 4 * L[p] * L[q]
+            td[ig, :, node] = 4 * (L[q] * dl[p] + L[p] * dl[q])
+    return np.round(tn * 1e6) / 1e6, np.round(td * 1e6) / 1e6
+
+
+def _wedge15_tables():
+    """WEIGHT_3's tables (data_out.f90:629, the same as SHAPE_23's): quadratic
+    pentahedron in area coordinates L1 = xi1, L3 = xi2, L5 = 1 - xi1 - xi2 of
+    base corners 1,3,5 and zeta = xi3; points 1-3 at zeta = -1/sqrt 3 nearest
+    corners 1,3,5 (2/3, 1/6, 1/6), 4-6 at +1/sqrt 3; 6 decimals (the tabulated
+    derivatives differ from these by up to 2e-6 in 60 of 270 entries)."""
+    kinds = [(0, 0, 0, -1), (1, 0, 1, -1), (0, 1, 1, -1), (1, 1, 2, -1), (0, 2, 2, -1), (1, 2, 0, -1), (2, 0, 0, 0),
+             (2, 1, 1, 0), (2, 2, 2, 0), (0, 0, 0, 1), (1, 0, 1, 1), (0, 1, 1, 1), (1, 1, 2, 1), (0, 2, 2, 1),
+             (1, 2, 0, 1)]
+    dl = np.array([[1, 0, 0], [0, 1, 0], [-1, -1, 0]], float)
+    ez = np.array([0.0, 0.0, 1.0])
+    tn, td = np.zeros((6, 15)), np.zeros((6, 3, 15))
+    for ig in range(6):
+        L = [2 / 3 if c == ig % 3 else 1 / 6 for c in range(3)]
+        z = (-1 if ig < 3 else 1) / math.sqrt(3)
+        for a, (kind, p, q, zc) in enumerate(kinds):
+            if kind == 0:
+                l = L[p]
+                tn[ig, a] = 0.5 * l * ((2 * l - 1) * (1 + z * zc) - (1 - z * z))
+                td[ig, :, a] = 0.5 * ((4 * l - 1) * (1 + z * zc) - (1 - z * z)) * dl[p] \
+                    + 0.5 * l * ((2 * l - 1) * zc + 2 * z) * ez
+            elif kind == 1:
+                f = 1 + z * zc
+                tn[ig, a] = 2 * L[p] * L[q] * f
+                td[ig, :, a] = 2 * f * (L[q] * dl[p] + L[p] * dl[q]) + 2 * L[p] * L[q] * zc * ez
+            else:
+                tn[ig, a] = L[p] * (1 - z * z)
+                td[ig, :, a] = (1 - z * z) * dl[p] - 2 * z * L[p] * ez
+    return np.round(tn * 1e6) / 1e6, np.round(td * 1e6) / 1e6
+
+
+_TABQ = {}
+
+
 def point_weight(lt: int, ig: int, xnode: np.ndarray):
     """(weight, point coordinates) of point ig (0-based) of an element with node
-    coordinates xnode[nn][3] (WEIGHT_4 for lt 12, WEIGHT_2 for lt 22)."""
+    coordinates xnode[nn][3] (WEIGHT_4 for lt 12, WEIGHT_1 for 21, WEIGHT_2 for
+    22, WEIGHT_3 for 23)."""
     global _TAB20
+    if lt in (21, 23):
+        if lt not in _TABQ:
+            _TABQ[lt] = _tet10_tables() if lt == 21 else _wedge15_tables()
+        n, d = _TABQ[lt][0][ig], _TABQ[lt][1][ig]
+        det = float(np.linalg.det(d @ xnode))
+        # data_out.f90: weight = -0.0416667 det (WEIGHT_1), 0.16666667 det (WEIGHT_3)
+        return (-0.0416667 * det if lt == 21 else 0.16666667 * det), n @ xnode
     if lt == 12:
         n, d = _brick8(np.zeros(3))
         scale = 8.0

@@ -96,8 +96,10 @@ typedef struct cpfem_ctx cpfem_ctx;
 
 /* Mesh, boundary codes and load tables as read from the .fsd file
  * (Faslt_io_nowin.f90:29-128).  Element types: 12 (8-node brick, one point,
- * SHAPE_12 Shapef.f90:32), 13 (8-node brick, 2x2x2 points), 22 (20-node brick,
- * SHAPE_22 Shapef.f90:210) and the two-node ties 1-6 (Elvals.f90:62-77).
+ * SHAPE_12 Shapef.f90:32), 13 (8-node brick, 2x2x2 points), 21 (10-node
+ * tetrahedron, 4 points, SHAPE_21 Shapef.f90:104), 22 (20-node brick,
+ * SHAPE_22 Shapef.f90:210), 23 (15-node pentahedron, 6 points, SHAPE_23
+ * Shapef.f90:360) and the two-node ties 1-6 (Elvals.f90:62-77).
  * nsv >= 3 + nss state variables per point (at most 63); those past 3 + nss
  * are carried unchanged. */
 typedef struct {

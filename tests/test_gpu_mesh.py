@@ -14,7 +14,8 @@ import pytest
 import scipy.sparse.linalg as spla
 
 from cpfem_code_amd import capi, prepro
-from cpfem_code_amd.problem import (Control, LT_BRICK_1GP, LT_BRICK_8GP, LT_BRICK_20, bcc_iron, brick_problem,
+from cpfem_code_amd.problem import (Control, LT_BRICK_1GP, LT_BRICK_8GP, LT_BRICK_20, LT_TET10, LT_WEDGE15,
+                                    bcc_iron, brick_problem,
                                     fcc_aluminium, fcc_plane_strain, fcc_tension, hcp_compression, with_second_phase)
 from gpbatch import mesh_state
 from oracle import Oracle, OracleModel
@@ -28,7 +29,15 @@ def rel_max(a, b):
 
 
 def _maxofd(prob):
-    # DOF columns per row: 81 for 8-node bricks, up to 243 for 20-node ones
+    # DOF columns per row: 81 for 8-node bricks, up to 243 for 20-node ones;
+    # 3 x the largest node neighbourhood for the tetrahedra and pentahedra
+    if np.any((prob.lt == LT_TET10) | (prob.lt == LT_WEDGE15)):
+        nb = {}
+        for e in range(prob.nlmnt):
+            nodes = prob.ln[e, :prob.nd[e]]
+            for a in nodes:
+                nb.setdefault(int(a), set()).update(int(b) for b in nodes)
+        return 3 * max(len(v) for v in nb.values())
     return 243 if np.any(prob.lt == LT_BRICK_20) else 120
 
 
@@ -61,9 +70,9 @@ def _pair(prob, state, precision=0):
 
 
 @pytest.mark.parametrize("lt,ties", [(LT_BRICK_8GP, False), (LT_BRICK_1GP, False), (LT_BRICK_20, False),
-                                     (LT_BRICK_8GP, True)])
+                                     (LT_BRICK_8GP, True), (LT_TET10, False), (LT_WEDGE15, False)])
 def test_elvals_suvals_match_oracle(lt, ties):
-    prob = fcc_tension(2 if lt == LT_BRICK_20 else 3, ninc=3, lt=lt, seed=5)
+    prob = fcc_tension(3 if lt in (LT_BRICK_8GP, LT_BRICK_1GP) else 2, ninc=3, lt=lt, seed=5)
     if ties:
         prob = with_ties(prob, lt=2)
     _check_elvals(prob)
@@ -208,6 +217,37 @@ def test_two_material_increments_match_oracle(second):
         e = np.abs(g["st"] - om.st) / np.abs(om.st).max()
         assert np.median(e) < 1e-7 and e.max() < 2e-3
         assert rel_max(g["sv"][:, 3:prob.nsv], om.sv[:, 3:prob.nsv]) < 1e-4
+    sv.close()
+
+
+@pytest.mark.parametrize("lt", [LT_TET10, LT_WEDGE15])
+def test_tet_wedge_increments_match_oracle(lt):
+    """10-node tetrahedra (SHAPE_21) and 15-node pentahedra (SHAPE_23) through
+    whole increments: same Newton path, fields at the solver / constitutive
+    tolerance."""
+    ctl = Control(sor_kind=1, sor_tol=1e-12, sor_omega=1.6, sor_maxit=50000)
+    prob = fcc_tension(2, ninc=3, lt=lt, seed=11, control=ctl)
+    om = OracleModel(O64, prob, maxofd=_maxofd(prob))
+    om.ctl.keep_going = 1
+    sv = capi.Solver(prob)
+    e_o = om.increment(1)
+    st = sv.increment(1)
+    assert st["code"] == e_o and st["nit"] == om.m.nit
+    g = sv.get_state()
+    assert rel_max(g["x"] - prob.x, om.x - prob.x) < 1e-6
+    e = np.abs(g["st"] - om.st) / np.abs(om.st).max()
+    assert np.median(e) < 1e-7 and e.max() < 2e-3
+    # increment 2 has points whose slip Newton does not converge (987 on both
+    # sides): the bar is the Newton tolerance, as for the 20-node bricks
+    e_o = om.increment(2)
+    st = sv.increment(2)
+    assert st["code"] == e_o
+    g = sv.get_state()
+    assert rel_max(g["x"] - prob.x, om.x - prob.x) < 1e-3
+    e = np.abs(g["st"] - om.st) / np.abs(om.st).max()
+    assert np.median(e) < 1e-3
+    v = sv.volume_average()
+    assert abs(v["volume"] - 1.0) < 1e-2
     sv.close()
 
 

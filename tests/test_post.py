@@ -113,6 +113,30 @@ def test_point_weights_integrate_affine_volume(lt):
     np.testing.assert_allclose(sum(v * x for v, x in w), vol * centre, rtol=tol, atol=tol)
 
 
+@pytest.mark.parametrize("lt", [21, 23])
+def test_tet_wedge_weights_integrate_affine_cell(lt):
+    """The 6 tetrahedra (or 2 pentahedra) of a cell of the mesh generator,
+    mapped affinely: their point weights sum to the mapped cell's volume and
+    the weighted point coordinates to volume times its centre (tables of 6
+    decimals, the weight constants 0.0416667 and 0.16666667)."""
+    from cpfem_code_amd.problem import _tet10_cells, _wedge15_cells
+    rng = np.random.default_rng(lt)
+    a = np.eye(3) + 0.2 * rng.normal(size=(3, 3))
+    b = rng.normal(size=3)
+    vol = 8.0 * np.linalg.det(a)                  # cell of side 2
+    centre = np.ones(3) @ a.T + b
+    tot, mom = 0.0, np.zeros(3)
+    for cell in (_tet10_cells() if lt == 21 else _wedge15_cells()):
+        xl = np.asarray(cell, float) @ a.T + b
+        for ig in range(4 if lt == 21 else 6):
+            v, x = post.point_weight(lt, ig, xl)
+            assert v > 0
+            tot += v
+            mom += v * x
+    assert abs(tot - vol) < 3e-6 * vol
+    np.testing.assert_allclose(mom, vol * centre, rtol=1e-5, atol=1e-5)
+
+
 def _bunge(ph1, phi, ph2):
     """Crystal-to-specimen rotation of Bunge angles (columns are crystal axes)."""
     def rz(a):
