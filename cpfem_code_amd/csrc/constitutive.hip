@@ -116,9 +116,8 @@ __device__ __forceinline__ void scale_it_c(const DevMaterial<N>& M, Compact<AMAX
 
 // SLPFTA on the active set; leaves ah in X form and gpre = slips of the last PLASTC.
 template <int N, int AMAX>
-__device__ __forceinline__ void slpfta_c(const DevMaterial<N>& M, const DevControl& C, Compact<AMAX>& S, double dt,
-                                         double* gpre, size_t gstride, int& err, int64_t& flops,
-                                         int64_t& transc) {
+__device__ __forceinline__ int slpfta_c(const DevMaterial<N>& M, const DevControl& C, Compact<AMAX>& S, double dt,
+                                        double* gpre, size_t gstride, int& err, int64_t& flops, int64_t& transc) {
     const int nsa = S.nsa;
     double th[AMAX];
     int it = 0;
@@ -182,6 +181,7 @@ __device__ __forceinline__ void slpfta_c(const DevMaterial<N>& M, const DevContr
         }
     }
     flops += (int64_t)nsa * nsa + 3 * nsa;
+    return it;
 }
 
 // The per-point kernel.  One lane per point, 64 lanes per block.
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
         elastc<N>(M, ph[0], ph[1], ph[2], el, (int)n);
         tens_rot(r, st, t);
         int64_t flops = 0, transc = 0;
-        int iter = 0;
+        int iter = 0, inner = 0;   // outer and slip-Newton iterations so far
         bool elastic = false;
         if (first) {
             flops = 1575 + 12 + 120 + 28;
@@ -251,6 +251,7 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
 #pragma unroll
             for (int i = 0; i < 6; ++i) s[i] = A.ck_s[i * n + g];
             iter = A.ck_iter[g];
+            inner = A.ck_inner[g];
             flops = A.flops[g];
             transc = A.transc[g];
         }
@@ -389,7 +390,7 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
                     for (int q = 0; q < 6; ++q) bc[(6 * k + q) * 64] = b[q];
                 }
                 scale_it_c<N, AMAX>(M, S, s, gamcap, C.afactor, flops, transc);
-                slpfta_c<N, AMAX>(M, C, S, dt, gpre, n, err, flops, transc);
+                inner += slpfta_c<N, AMAX>(M, C, S, dt, gpre, n, err, flops, transc);
                 scale_it_c<N, AMAX>(M, S, s, gamcap, C.afactor, flops, transc);
                 // plastic strain increment and strain-increment error (Cpstrs.f90:96-98)
 #pragma unroll
@@ -549,6 +550,7 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
 #pragma unroll
                 for (int i = 0; i < 6; ++i) A.ck_s[i * n + g] = s[i];
                 A.ck_iter[g] = iter;
+                A.ck_inner[g] = inner;
                 A.ck_kerr[g] = err;
                 A.flops[g] = flops;
                 A.transc[g] = transc;
@@ -639,12 +641,102 @@ constexpr int kAmaxWide = 12;
 constexpr int kAmaxGeneric = kConstMaxWidth;   // fourth tier, generic system counts only
 constexpr int kFirstPassIters = 8;
 
+__global__ void k_zero_n(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 __global__ void k_zero4(int* p) {
     if (threadIdx.x < 4) p[threadIdx.x] = 0;
 }
 
 __global__ void k_set1(int* p, int v) {
     if (threadIdx.x == 0) *p = v;
+}
+
+// Ordering of a list of unfinished points before the pass that resumes them:
+// a point's slip-Newton iteration count per outer iteration barely changes
+// from one outer iteration to the next, so points are grouped by the mean so
+// far (descending, in eighths of an iteration): the lanes of a wave then run
+// inner loops of similar length, and the longest waves start first.  A
+// counting sort over kSortBins bins; the order within a bin is arbitrary and
+// changes no result (every point is updated independently).
+constexpr int kSortBins = 1024;
+
+__device__ __forceinline__ int sort_bin(const int* ck_inner, const int* ck_iter, int g) {
+    const int it = ck_iter[g] > 1 ? ck_iter[g] : 1;
+    const int m = (8 * ck_inner[g]) / it;
+    return kSortBins - 1 - (m < kSortBins - 1 ? m : kSortBins - 1);
+}
+
+// Bin counts: per block in LDS, then one device atomic per occupied bin
+// (most points share a few bins, whose device counters would otherwise take
+// one atomic per point).
+__global__ __launch_bounds__(256) void k_sort_count(const int* __restrict__ list, const int* __restrict__ count,
+                                                    const int* __restrict__ ck_inner,
+                                                    const int* __restrict__ ck_iter, int* bins) {
+    __shared__ int sh[kSortBins];
+    for (int b = threadIdx.x; b < kSortBins; b += blockDim.x) sh[b] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < *count) atomicAdd(&sh[sort_bin(ck_inner, ck_iter, list[i])], 1);
+    __syncthreads();
+    for (int b = threadIdx.x; b < kSortBins; b += blockDim.x)
+        if (sh[b]) atomicAdd(&bins[b], sh[b]);
+}
+
+// exclusive prefix sum of the kSortBins counts, one block of kSortBins threads
+__global__ __launch_bounds__(kSortBins) void k_sort_scan(int* bins) {
+    __shared__ int sh[kSortBins];
+    const int t = threadIdx.x;
+    const int v = bins[t];
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kSortBins; off <<= 1) {
+        const int add = t >= off ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    bins[t] = sh[t] - v;
+}
+
+// Scatter into bin order: each block reserves its run of every occupied bin
+// with one device atomic, then ranks its points within the run in LDS.
+__global__ __launch_bounds__(256) void k_sort_scatter(const int* __restrict__ list, const int* __restrict__ count,
+                                                      const int* __restrict__ ck_inner,
+                                                      const int* __restrict__ ck_iter, int* bins,
+                                                      int* __restrict__ out) {
+    __shared__ int cnt[kSortBins];
+    __shared__ int base[kSortBins];
+    for (int b = threadIdx.x; b < kSortBins; b += blockDim.x) cnt[b] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < *count;
+    int g = 0, key = 0;
+    if (live) {
+        g = list[i];
+        key = sort_bin(ck_inner, ck_iter, g);
+        atomicAdd(&cnt[key], 1);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kSortBins; b += blockDim.x) {
+        if (cnt[b]) base[b] = atomicAdd(&bins[b], cnt[b]);
+        cnt[b] = 0;
+    }
+    __syncthreads();
+    if (live) out[base[key] + atomicAdd(&cnt[key], 1)] = g;
+}
+
+// Sorted copy of the n-point list into S.sorted (device count *count == n).
+static hipError_t sort_points(ConstSchedule& S, const int* list, const int* count, int n, hipStream_t stream) {
+    const int nb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_zero_n, dim3(kSortBins / 256), dim3(256), 0, stream, S.bins, kSortBins);
+    hipLaunchKernelGGL(k_sort_count, dim3(nb), dim3(256), 0, stream, list, count, S.ck_inner, S.ck_iter, S.bins);
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kSortBins), 0, stream, S.bins);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(nb), dim3(256), 0, stream, list, count, S.ck_inner, S.ck_iter, S.bins,
+                       S.sorted);
+    return hipGetLastError();
 }
 
 template <int N, int AMAX>
@@ -673,6 +765,7 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
     if (e != hipSuccess) return e;
     A.ck_s = S.ck_s;
     A.ck_iter = S.ck_iter;
+    A.ck_inner = S.ck_inner;
     A.ck_kerr = S.ck_kerr;
     A.el = S.el;
     A.gp = S.gp;
@@ -689,7 +782,8 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
     counts[0] = S.host_counts[0];
     // tier 1, pass 2: the unfinished points, compacted, up to maxit
     if (S.host_counts[0] > 0) {
-        A.in_list = S.list_a;
+        if ((e = sort_points(S, S.list_a, S.counts + 0, S.host_counts[0], stream)) != hipSuccess) return e;
+        A.in_list = S.sorted;
         A.in_count = S.counts + 0;
         A.cont_list = S.list_b;
         A.cont_count = S.counts + 1;
@@ -703,7 +797,8 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
         hipLaunchKernelGGL(k_zero4, dim3(1), dim3(64), 0, stream, S.counts);  // list_a reused below
         // the overflow count was read on the host; keep it on the device in slot 3
         hipLaunchKernelGGL(k_set1, dim3(1), dim3(64), 0, stream, S.counts + 3, S.host_counts[2]);
-        A.in_list = S.list_ovf;
+        if ((e = sort_points(S, S.list_ovf, S.counts + 3, S.host_counts[2], stream)) != hipSuccess) return e;
+        A.in_list = S.sorted;
         A.in_count = S.counts + 3;
         A.cont_list = S.list_b;
         A.cont_count = S.counts + 1;
@@ -766,6 +861,9 @@ hipError_t alloc_schedule(ConstSchedule& S, int n) {
     if ((e = hipMalloc((void**)&S.ck_s, sizeof(double) * 6 * (size_t)n)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&S.ck_iter, sizeof(int) * (size_t)n)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&S.ck_kerr, sizeof(int) * (size_t)n)) != hipSuccess) return e;
+    if ((e = hipMalloc((void**)&S.ck_inner, sizeof(int) * (size_t)n)) != hipSuccess) return e;
+    if ((e = hipMalloc((void**)&S.sorted, sizeof(int) * (size_t)n)) != hipSuccess) return e;
+    if ((e = hipMalloc((void**)&S.bins, sizeof(int) * kSortBins)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&S.list_a, sizeof(int) * (size_t)n)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&S.list_b, sizeof(int) * (size_t)n)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&S.list_ovf, sizeof(int) * (size_t)n)) != hipSuccess) return e;
@@ -777,7 +875,8 @@ hipError_t alloc_schedule(ConstSchedule& S, int n) {
 }
 
 void free_schedule(ConstSchedule& S) {
-    void* p[] = {S.ck_s, S.ck_iter, S.ck_kerr, S.list_a, S.list_b, S.list_ovf, S.el, S.gp, S.counts};
+    void* p[] = {S.ck_s, S.ck_iter, S.ck_kerr, S.ck_inner, S.sorted, S.bins, S.list_a,
+                 S.list_b, S.list_ovf, S.el, S.gp, S.counts};
     for (void* q : p)
         if (q) (void)hipFree(q);
     if (S.host_counts) (void)hipHostFree(S.host_counts);

@@ -316,6 +316,7 @@ struct ConstArgs {
     double* ck_s;              // [6][n] stress at the next outer iteration
     int* ck_iter;              // [n] outer iterations done
     int* ck_kerr;              // [n] error code of the kinematics / earlier iterations
+    int* ck_inner;             // [n] slip-Newton iterations so far
     const int* in_list;        // points of this launch (null: all points, first pass)
     const int* in_count;       // device count of in_list
     int* cont_list;            // unfinished points

@@ -18,6 +18,9 @@ struct ConstSchedule {
     double* ck_s = nullptr;    // [6][n]
     int* ck_iter = nullptr;    // [n]
     int* ck_kerr = nullptr;    // [n]
+    int* ck_inner = nullptr;   // [n] slip-Newton iterations so far (ordering key)
+    int* sorted = nullptr;     // [n] a point list in resume order
+    int* bins = nullptr;       // [1024] counting-sort bins
     int* list_a = nullptr;     // [n] unfinished after the first pass
     int* list_b = nullptr;     // [n] spare
     int* list_ovf = nullptr;   // [n] active set wider than the first kernel
