@@ -196,12 +196,13 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
     __shared__ int sh_isa[AMAX][64];
     const int lane = threadIdx.x;
     const int idx = blockIdx.x * 64 + lane;
-    const bool first = (A.in_list == nullptr);
+    const bool first = A.first != 0;
     bool live;
     int g = 0;
     if (first) {
+        // every point, in the order of in_list when one is given
         live = idx < A.n;
-        g = idx;
+        if (live) g = A.in_list ? A.in_list[idx] : idx;
     } else {
         live = idx < *A.in_count;
         if (live) g = A.in_list[idx];
@@ -663,23 +664,39 @@ __global__ void k_set1(int* p, int v) {
 // changes no result (every point is updated independently).
 constexpr int kSortBins = 1024;
 
-__device__ __forceinline__ int sort_bin(const int* ck_inner, const int* ck_iter, int g) {
-    const int it = ck_iter[g] > 1 ? ck_iter[g] : 1;
-    const int m = (8 * ck_inner[g]) / it;
-    return kSortBins - 1 - (m < kSortBins - 1 ? m : kSortBins - 1);
-}
+// Sort key of a point: with flops (the work of the point's previous update,
+// for the pass over every point) an eighth-octave bin of it, otherwise the
+// mean slip-Newton iterations per outer iteration so far in eighths.  Larger
+// work sorts first.
+struct SortKey {
+    const int* ck_inner;
+    const int* ck_iter;
+    const long long* flops;
+    __device__ __forceinline__ int bin(int g) const {
+        int m;
+        if (flops) {
+            const unsigned long long f = flops[g] > 0 ? (unsigned long long)flops[g] : 1ull;
+            const int e = 63 - __clzll(f);
+            const int frac = e >= 3 ? (int)((f >> (e - 3)) & 7ull) : (int)((f << (3 - e)) & 7ull);
+            m = 8 * e + frac;
+        } else {
+            const int it = ck_iter[g] > 1 ? ck_iter[g] : 1;
+            m = (8 * ck_inner[g]) / it;
+        }
+        return kSortBins - 1 - (m < kSortBins - 1 ? m : kSortBins - 1);
+    }
+};
 
 // Bin counts: per block in LDS, then one device atomic per occupied bin
 // (most points share a few bins, whose device counters would otherwise take
 // one atomic per point).
-__global__ __launch_bounds__(256) void k_sort_count(const int* __restrict__ list, const int* __restrict__ count,
-                                                    const int* __restrict__ ck_inner,
-                                                    const int* __restrict__ ck_iter, int* bins) {
+__global__ __launch_bounds__(256) void k_sort_count(const int* __restrict__ list, int count, const SortKey K,
+                                                    int* bins) {
     __shared__ int sh[kSortBins];
     for (int b = threadIdx.x; b < kSortBins; b += blockDim.x) sh[b] = 0;
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < *count) atomicAdd(&sh[sort_bin(ck_inner, ck_iter, list[i])], 1);
+    if (i < count) atomicAdd(&sh[K.bin(list ? list[i] : i)], 1);
     __syncthreads();
     for (int b = threadIdx.x; b < kSortBins; b += blockDim.x)
         if (sh[b]) atomicAdd(&bins[b], sh[b]);
@@ -703,20 +720,18 @@ __global__ __launch_bounds__(kSortBins) void k_sort_scan(int* bins) {
 
 // Scatter into bin order: each block reserves its run of every occupied bin
 // with one device atomic, then ranks its points within the run in LDS.
-__global__ __launch_bounds__(256) void k_sort_scatter(const int* __restrict__ list, const int* __restrict__ count,
-                                                      const int* __restrict__ ck_inner,
-                                                      const int* __restrict__ ck_iter, int* bins,
-                                                      int* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_sort_scatter(const int* __restrict__ list, int count, const SortKey K,
+                                                      int* bins, int* __restrict__ out) {
     __shared__ int cnt[kSortBins];
     __shared__ int base[kSortBins];
     for (int b = threadIdx.x; b < kSortBins; b += blockDim.x) cnt[b] = 0;
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < *count;
+    const bool live = i < count;
     int g = 0, key = 0;
     if (live) {
-        g = list[i];
-        key = sort_bin(ck_inner, ck_iter, g);
+        g = list ? list[i] : i;
+        key = K.bin(g);
         atomicAdd(&cnt[key], 1);
     }
     __syncthreads();
@@ -728,14 +743,15 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const int* __restrict__ li
     if (live) out[base[key] + atomicAdd(&cnt[key], 1)] = g;
 }
 
-// Sorted copy of the n-point list into S.sorted (device count *count == n).
-static hipError_t sort_points(ConstSchedule& S, const int* list, const int* count, int n, hipStream_t stream) {
+// Sorted copy of the n-point list (null: points 0..n-1) into S.sorted, keyed
+// by the previous updates' work (flops non-null) or by the checkpoints.
+static hipError_t sort_points(ConstSchedule& S, const int* list, int n, const long long* flops, hipStream_t stream) {
     const int nb = (n + 255) / 256;
+    const SortKey K{S.ck_inner, S.ck_iter, flops};
     hipLaunchKernelGGL(k_zero_n, dim3(kSortBins / 256), dim3(256), 0, stream, S.bins, kSortBins);
-    hipLaunchKernelGGL(k_sort_count, dim3(nb), dim3(256), 0, stream, list, count, S.ck_inner, S.ck_iter, S.bins);
+    hipLaunchKernelGGL(k_sort_count, dim3(nb), dim3(256), 0, stream, list, n, K, S.bins);
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kSortBins), 0, stream, S.bins);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(nb), dim3(256), 0, stream, list, count, S.ck_inner, S.ck_iter, S.bins,
-                       S.sorted);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(nb), dim3(256), 0, stream, list, n, K, S.bins, S.sorted);
     return hipGetLastError();
 }
 
@@ -769,9 +785,15 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
     A.ck_kerr = S.ck_kerr;
     A.el = S.el;
     A.gp = S.gp;
-    // tier 1, pass 1: every point, at most kFirstPassIters outer iterations
+    // tier 1, pass 1: every point, at most kFirstPassIters outer iterations;
+    // on a mesh, in the order of the work of each point's previous update
+    A.first = 1;
     A.in_list = nullptr;
     A.in_count = nullptr;
+    if (A.mesh) {
+        if ((e = sort_points(S, nullptr, A.n, A.flops, stream)) != hipSuccess) return e;
+        A.in_list = S.sorted;
+    }
     A.cont_list = S.list_a;
     A.cont_count = S.counts + 0;
     A.ovf_list = S.list_ovf;
@@ -780,9 +802,10 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
     launch_one<N, kAmax>(dmat, C, dt, A, A.n, stream);
     if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
     counts[0] = S.host_counts[0];
+    A.first = 0;
     // tier 1, pass 2: the unfinished points, compacted, up to maxit
     if (S.host_counts[0] > 0) {
-        if ((e = sort_points(S, S.list_a, S.counts + 0, S.host_counts[0], stream)) != hipSuccess) return e;
+        if ((e = sort_points(S, S.list_a, S.host_counts[0], nullptr, stream)) != hipSuccess) return e;
         A.in_list = S.sorted;
         A.in_count = S.counts + 0;
         A.cont_list = S.list_b;
@@ -797,7 +820,7 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
         hipLaunchKernelGGL(k_zero4, dim3(1), dim3(64), 0, stream, S.counts);  // list_a reused below
         // the overflow count was read on the host; keep it on the device in slot 3
         hipLaunchKernelGGL(k_set1, dim3(1), dim3(64), 0, stream, S.counts + 3, S.host_counts[2]);
-        if ((e = sort_points(S, S.list_ovf, S.counts + 3, S.host_counts[2], stream)) != hipSuccess) return e;
+        if ((e = sort_points(S, S.list_ovf, S.host_counts[2], nullptr, stream)) != hipSuccess) return e;
         A.in_list = S.sorted;
         A.in_count = S.counts + 3;
         A.cont_list = S.list_b;

@@ -317,7 +317,8 @@ struct ConstArgs {
     int* ck_iter;              // [n] outer iterations done
     int* ck_kerr;              // [n] error code of the kinematics / earlier iterations
     int* ck_inner;             // [n] slip-Newton iterations so far
-    const int* in_list;        // points of this launch (null: all points, first pass)
+    int first;                 // 1: the first pass (every point, from the increment's inputs)
+    const int* in_list;        // points of this launch (first pass: their order, null: 0..n-1)
     const int* in_count;       // device count of in_list
     int* cont_list;            // unfinished points
     int* cont_count;
