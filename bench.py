@@ -168,10 +168,10 @@ def main():
                          "uniaxial tension (32^3 FP64 drop-in case); hcp_compression: Mg-like HCP with twins (48^3)")
     ap.add_argument("--paths", default="fp64,mixed")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--omega-variants", default="auto,1.9",
-                    help="comma list of SOR omegas (a number, or 'auto' for the omega and launch shape tuned at "
-                         "the first solve) to run the mixed path with as well, reported beside the headline, "
-                         "which uses the configured 1.6; empty skips them")
+    ap.add_argument("--omega-variants", default="mixed:auto,fp64:auto,mixed:1.9",
+                    help="comma list of path:omega (path fp64 or mixed; omega a number, or 'auto' for the omega "
+                         "and launch shape tuned at the first solve) to run as well, reported beside the "
+                         "headline, which uses the configured 1.6; empty skips them")
     ap.add_argument("--trace-iterations", action="store_true",
                     help="drive each increment through the per-step calls and print every Newton iteration")
     args = ap.parse_args()
@@ -212,19 +212,22 @@ def main():
         if prec == 0:
             last_state = state
     omega_runs = []
-    variants = [v for v in args.omega_variants.split(",") if v.strip()] if "mixed" in results else []
+    variants = [v.strip() for v in args.omega_variants.split(",") if v.strip()]
     for v in variants:
         import dataclasses
-        om = 0.0 if v.strip() == "auto" else float(v)
+        path, _, oms = v.partition(":") if ":" in v else ("mixed", ":", v)
+        om = 0.0 if oms == "auto" else float(oms)
         pv = dataclasses.replace(prob, control=dataclasses.replace(ctl, sor_omega=om))
-        recs_o, wall_o, _, warm_o = run_path(pv, 1, args.warmup, args.steps, capi)
+        recs_o, wall_o, _, warm_o = run_path(pv, 0 if path == "fp64" else 1, args.warmup, args.steps, capi)
         run = {
-            "omega": recs_o[-1]["sor_omega"], "set_as": v.strip(), "precision": "mixed",
+            "omega": recs_o[-1]["sor_omega"], "set_as": oms, "precision": path,
             "waves_per_workgroup": recs_o[-1]["sor_wpb"],
             "s_per_increment": wall_o / args.steps,
             "sweeps_per_increment": sum(r["sor_sweeps_total"] for r in recs_o) / args.steps,
             "nr_iterations_per_increment": sum(r["nit"] for r in recs_o) / args.steps,
             "gp_updates_per_s": sum(r["gp_updates"] for r in recs_o) / wall_o,
+            "constitutive_s_per_increment": sum(r["t_const_ms"] for r in recs_o) * 1e-3 / args.steps,
+            "sor_s_per_increment": sum(r["t_sor_ms"] for r in recs_o) * 1e-3 / args.steps,
             "note": "same workload and 1e-8 relative residual with the SOR relaxation factor set in cpfem.dat; "
                     "not the headline value, which keeps the configured omega 1.6"}
         if om == 0.0:
