@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libcpfem.so")
 
-SOURCES = ["constitutive.hip", "mesh.hip"]
+SOURCES = ["constitutive.hip", "mesh.hip", "calib.hip"]
 
 HIPFLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -51,7 +51,21 @@ def build(verbose: bool = False) -> str:
     for o in objs:
         os.remove(o)
     build_host()
+    build_test_helpers()
     return LIB
+
+
+TEST_NATIVE = os.path.join(os.path.dirname(HERE), "tests", "native")
+
+
+def build_test_helpers() -> str:
+    """Host build of the constitutive adjoint's per-point step for the CPU tests
+    (tests/native/libadjoint_host.so); no product path loads it."""
+    src = os.path.join(TEST_NATIVE, "adjoint_host.hip")
+    out = os.path.join(TEST_NATIVE, "libadjoint_host.so")
+    subprocess.check_call([hipcc()] + HIPFLAGS + ["-shared", "-o", out, src],
+                          stderr=subprocess.DEVNULL)
+    return out
 
 
 FLANG = "/opt/rocm/lib/llvm/bin/flang"

@@ -499,6 +499,18 @@ __global__ __launch_bounds__(kBlock) void k_vol_sums(const KinArgs K, const doub
     block_partials<4>(v, partial, slot0 + blockIdx.x);
 }
 
+__global__ __launch_bounds__(kBlock) void k_point_weights(const KinArgs K, double* __restrict__ w) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= K.n) return;
+    const int e = K.gp_elem[g];
+    const int nn = K.elem_nd[e];
+    w[g] = nn == 8    ? point_weight<8>(K, g, e)
+           : nn == 10 ? point_weight<10>(K, g, e)
+           : nn == 15 ? point_weight<15>(K, g, e)
+           : nn == 20 ? point_weight<20>(K, g, e)
+                      : 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // Assembly of the tabulated quadratic elements (20-node bricks, 10-node
 // tetrahedra, 15-node pentahedra): one 256-thread block per element, thread
@@ -1403,6 +1415,7 @@ fail:
 extern "C" int cpfem_destroy(cpfem_ctx* c) {
     if (!c) return 0;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    calib_free(c);
     free_schedule(c->sched);
     for (int i = 0; i < 3; ++i)
         if (c->graphs[i]) (void)hipGraphExecDestroy(c->graphs[i]);
@@ -2234,6 +2247,20 @@ extern "C" int cpfem_volume_average(cpfem_ctx* c, double* out) {
     for (int i = 0; i < 6; ++i) out[7 + i] = vol > 0.0 ? sums[8 + i] / vol : 0.0;
     out[13] = vol > 0.0 ? sums[1] / vol : 0.0;
     return 0;
+}
+
+hipError_t point_weights(Ctx* c, double* w) {
+    KinArgs K;
+    memset(&K, 0, sizeof(K));
+    K.n = c->ngps;
+    K.gp_elem = c->gp_elem;
+    K.gp_local = c->gp_local;
+    K.elem_nodes = c->elem_nodes;
+    K.elem_lt = c->elem_lt;
+    K.elem_nd = c->elem_nd;
+    K.x = c->x;
+    hipLaunchKernelGGL(k_point_weights, dim3(nblk(c->ngps)), dim3(kBlock), 0, c->stream, K, w);
+    return hipGetLastError();
 }
 
 extern "C" int cpfem_set_tflag(cpfem_ctx* c, int32_t tflag) {

@@ -129,6 +129,13 @@ struct Ctx {
     hipEvent_t ev[kEvents] = {};
     cpfem_stats stats;
     std::vector<void*> allocs;
+    void* calib = nullptr;       // replay and adjoint state (calib.hip)
 };
+
+// Integration weight of every point at the current coordinates, w[ngps]
+// (mesh.hip), enqueued on the context's stream.
+hipError_t point_weights(Ctx* c, double* w);
+// Releases the replay and adjoint state (calib.hip).
+void calib_free(Ctx* c);
 
 }  // namespace cpfem

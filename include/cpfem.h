@@ -28,6 +28,7 @@ extern "C" {
 #define CPFEM_E_HIP (-2)       /* HIP runtime error */
 #define CPFEM_E_NSS (-3)       /* number of slip systems outside 1..CPFEM_MAXSS */
 #define CPFEM_E_NOGPU (-4)     /* no gfx950 device */
+#define CPFEM_E_SINGULAR (-6)  /* calibration: a reverse step met a singular Jacobian of the converged equations */
 #define CPFEM_E_WIDTH (-5)     /* a point's active set is wider than the compiled kernels: 12 systems
                                   for materials of 12 or 18 systems, 24 for other counts */
 
@@ -216,6 +217,50 @@ int cpfem_get_stats(cpfem_ctx *ctx, cpfem_stats *stats);
 #define CPFEM_TUNE_LAUNCH 1
 #define CPFEM_TUNE_OMEGA 2
 int cpfem_autotune(cpfem_ctx *ctx, int32_t flags, double *omega, int32_t *waves_per_block);
+
+/* Calibration: replay of a recorded load path and the adjoint of its
+ * constitutive updates.  No counterpart in the reference, which has no
+ * derivatives; for fitting hardening and rate parameters to a measured
+ * macroscopic stress-strain curve.
+ *
+ * cpfem_calib_begin starts a record of ninc increments from the current state
+ * (stress and angles kept as the start of the path).  One material set only,
+ * without twin systems, with the same ampar(1:6) on every system and a latent
+ * matrix of one diagonal and one off-diagonal value (else CPFEM_E_ARG), and at
+ * most 24 systems.  toler and maxit are the CPSTRS/SLPFTA tolerance and
+ * iteration limit of the replay: the gradient is that of the converged
+ * equations, so the replay should converge far below the reference's toler
+ * (1e-10 for a 1e-6 finite-difference check).  ckpt_every: increments per
+ * checkpoint of the reverse sweep.
+ *
+ * cpfem_calib_record(inc) keeps, for every point, the inputs of the last
+ * DSTRSS call of increment inc (de, dr, dsv(1:3), ds) and its integration
+ * weight; call it after cpfem_nriter and before cpfem_update_gptv.
+ *
+ * cpfem_calib_run replays the path with params = {tau0, taus, h0, q, m}
+ * (tau0: every initial slip resistance; taus = ampar(5), h0 = ampar(2),
+ * q = off-diagonal latent ratio, m = ampar(1) of every system): curve[inc-1]
+ * = weight-averaged stress component comp (0..5: 11 22 33 23 31 12) after
+ * increment inc, loss = sum (curve - target)^2, and with grad != NULL the
+ * derivative of the loss with respect to the five parameters.  Returns 0,
+ * CPFEM_E_SINGULAR (grad unreliable), or another error code. */
+typedef struct {
+    int32_t points, increments;
+    int32_t unconverged;     /* replayed point updates stopped at maxit (forward pass) */
+    int32_t singular;        /* reverse steps with a singular Jacobian */
+    int32_t elastic;         /* reverse steps on the elastic branch */
+    int32_t pad_;
+    double t_forward_ms;     /* forward replay */
+    double t_reverse_ms;     /* segment recomputation and reverse steps */
+} cpfem_calib_stats;
+int cpfem_calib_begin(cpfem_ctx *ctx, int32_t ninc, int32_t ckpt_every, double toler, int32_t maxit);
+int cpfem_calib_record(cpfem_ctx *ctx, int32_t inc);
+/* Recorded inputs of increment inc per point, rec[ngps][19]: de(6), dr(3),
+ * dsv(1:3) (6..), ds(6) (12..), weight (18). */
+int cpfem_calib_history(cpfem_ctx *ctx, int32_t inc, double *rec);
+int cpfem_calib_run(cpfem_ctx *ctx, const double *params, const double *target, int32_t comp, double *curve,
+                    double *loss, double *grad, cpfem_calib_stats *stats);
+int cpfem_calib_end(cpfem_ctx *ctx);
 
 /* Device name and compute units of the current device (diagnostics). */
 int cpfem_device_info(char *name, int32_t len, int32_t *cus);

@@ -1,0 +1,71 @@
+// Host build of the per-point reverse step of the constitutive adjoint
+// (cpfem_code_amd/csrc/adjoint.hpp), for CPU tests only: the same template
+// code that the device kernel k_adjoint runs, called on one point at a time
+// so that tests can hold it to central finite differences of the FP64 oracle.
+#include <string.h>
+
+#include <memory>
+
+#include "../../cpfem_code_amd/csrc/adjoint.hpp"
+#include "../../include/cpfem.h"
+
+using namespace cpfem;
+
+namespace {
+constexpr int kNM = 24;
+
+void fill(DevMaterial<kNGeneric>& d, const cpfem_material& m) {
+    memset(&d, 0, sizeof(d));
+    d.nss = m.nss;
+    for (int i = 0; i < 9; ++i) d.e[i] = m.e[i];
+    for (int i = 0; i < m.nss; ++i) {
+        d.ssang[i] = m.ssang[i];
+        for (int k = 0; k < 3; ++k) {
+            d.ab[i][k] = m.ab[i][k];
+            d.an[i][k] = m.an[i][k];
+        }
+        d.am[i] = m.ampar[i][0];
+        d.th0[i] = m.ampar[i][1];
+        d.th4[i] = m.ampar[i][2];
+        d.alp[i] = m.ampar[i][3];
+        d.taus[i] = m.ampar[i][4];
+        d.nsat[i] = m.ampar[i][5];
+        for (int k = 0; k < m.nss; ++k) d.smatrx[i][k] = m.smatrx[k][i];
+    }
+}
+}  // namespace
+
+// One reverse step; array sizes as adj::point_reverse (s0, srate, s0b: nss).
+// Returns point_reverse's code (1 plastic, 0 elastic, -1 singular), -2 for
+// nss > 24.
+extern "C" int adj_point_reverse_host(const cpfem_material* m, double dt, int iel, const double* st,
+                                      const double* php, const double* s0, const double* de, const double* r,
+                                      const double* dg, const double* dsg, const double* so, const double* srate,
+                                      const double* sb, double* phb, double* s0b, double* stb, double* pb) {
+    if (m->nss > kNM) return -2;
+    std::unique_ptr<DevMaterial<kNGeneric>> M(new DevMaterial<kNGeneric>());
+    fill(*M, *m);
+    adj::PointIn<kNM> P;
+    memset(&P, 0, sizeof(P));
+    for (int i = 0; i < 6; ++i) {
+        P.st[i] = st[i];
+        P.de[i] = de[i];
+        P.dsg[i] = dsg[i];
+        P.so[i] = so[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+        P.php[i] = php[i];
+        P.r[i] = r[i];
+        P.dg[i] = dg[i];
+    }
+    for (int i = 0; i < m->nss; ++i) {
+        P.s0[i] = s0[i];
+        P.srate[i] = srate[i];
+    }
+    P.iel = iel;
+    double s0bl[kNM];
+    for (int i = 0; i < m->nss; ++i) s0bl[i] = s0b[i];
+    const int rc = adj::point_reverse<kNGeneric, kNM>(*M, m->nss, dt, P, sb, phb, s0bl, stb, pb);
+    for (int i = 0; i < m->nss; ++i) s0b[i] = s0bl[i];
+    return rc;
+}
