@@ -1,0 +1,112 @@
+"""Calibration harness: gradients of the macroscopic stress-strain curve with
+respect to hardening and rate parameters.
+
+The load path is run once on the GPU (UPDATE_NODE, NRITER, UPDATE_GPTV,
+DoReorient; Fasolt3.f90:38-46) with the inputs of every point's last DSTRSS
+call recorded (cpfem_calib_record).  The curve is then a function of the
+parameters through the constitutive updates alone: each point replays its
+recorded strain and rotation increments (cpfem_calib_run), and the loss
+    L(p) = sum_n (curve_n(p) - target_n)^2,   p = (tau0, taus, h0, q, m)
+is differentiated by the hand-written adjoint of the constitutive update
+(reverse sweep with checkpoints).  The recorded kinematics are held fixed, so
+the gradient is exact for the replay; the equilibrium correction a full
+finite-element adjoint would add (the response of the displacement field to
+the parameters) is not included.
+
+``CalibLoss`` wraps the replay and its adjoint in a torch.autograd.Function,
+so that torch optimisers can drive a fit; torch is used only here.
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import Optional
+
+import numpy as np
+
+from . import capi
+from .problem import Problem
+
+PARAM_NAMES = ("tau0", "taus", "h0", "q", "m")
+
+
+def material_params(mat) -> np.ndarray:
+    """(tau0, taus, h0, q, m) of a material with uniform parameters."""
+    return np.array([float(mat.crss[0]), float(mat.ampar[0, 4]), float(mat.ampar[0, 1]),
+                     float(mat.smatrx[0, 1]), float(mat.ampar[0, 0])])
+
+
+@dataclasses.dataclass
+class RecordedPath:
+    solver: "capi.Solver"
+    ninc: int
+    fe_curve: np.ndarray          # volume-average stress component after each increment (finite-element run)
+    comp: int
+    codes: list
+
+
+def record_path(prob: Problem, ninc: Optional[int] = None, comp: int = 2, ckpt_every: int = 10,
+                toler: float = 1e-10, maxit: int = 200, precision: int = 0, control=None) -> RecordedPath:
+    """Run ninc increments of prob on the GPU, recording every point's last
+    constitutive inputs; returns the solver holding the record."""
+    ninc = ninc or prob.ninc1 - 1
+    s = capi.Solver(prob, precision=precision, control=control)
+    s.calib_begin(ninc, ckpt_every=ckpt_every, toler=toler, maxit=maxit)
+    curve, codes = np.zeros(ninc), []
+    for inc in range(1, ninc + 1):
+        s.update_node()
+        r = s.nriter(inc)
+        codes.append(r["code"])
+        s.calib_record(inc)
+        s.update_gptv()
+        s.reorient()
+        curve[inc - 1] = s.volume_average()["st"][comp]
+    return RecordedPath(solver=s, ninc=ninc, fe_curve=curve, comp=comp, codes=codes)
+
+
+def seeded_target(path: RecordedPath, p0: np.ndarray, seed: int = 4, spread: float = 0.1):
+    """Target curve: the replay at parameters drawn around p0 (factor 1 +-
+    spread, seeded).  Returns (target, p_target)."""
+    rng = np.random.default_rng(seed)
+    p_t = p0 * (1.0 + spread * rng.uniform(-1.0, 1.0, size=5))
+    curve, _, _, _ = path.solver.calib_run(p_t, np.zeros(path.ninc), comp=path.comp, grad=False)
+    return curve, p_t
+
+
+def loss_and_grad(path: RecordedPath, params, target, grad: bool = True):
+    curve, loss, g, st = path.solver.calib_run(params, target, comp=path.comp, grad=grad)
+    return loss, g, curve, st
+
+
+def central_differences(path: RecordedPath, params, target, rel: float = 1e-6) -> np.ndarray:
+    """Central differences of the replay loss (relative step rel per parameter)."""
+    p = np.asarray(params, np.float64)
+    g = np.zeros(5)
+    for k in range(5):
+        h = rel * abs(p[k])
+        up, dn = p.copy(), p.copy()
+        up[k] += h
+        dn[k] -= h
+        lu = path.solver.calib_run(up, target, comp=path.comp, grad=False)[1]
+        ld = path.solver.calib_run(dn, target, comp=path.comp, grad=False)[1]
+        g[k] = (lu - ld) / (2 * h)
+    return g
+
+
+def make_torch_loss():
+    """torch.autograd.Function over the replay loss (imports torch lazily)."""
+    import torch
+
+    class CalibLoss(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, params, path, target):
+            p = params.detach().cpu().double().numpy()
+            loss, g, _, _ = loss_and_grad(path, p, target, grad=True)
+            ctx.save_for_backward(torch.as_tensor(g, dtype=params.dtype))
+            return params.new_tensor(loss)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            (g,) = ctx.saved_tensors
+            return grad_out * g.to(grad_out.device), None, None
+
+    return CalibLoss

@@ -188,6 +188,21 @@ def _bind_mesh_api(L):
     L.cpfem_get_matrix.argtypes = [_CTX, _P, _P, _P, _P]
     L.cpfem_get_stats.argtypes = [_CTX, C.POINTER(CpfemStats)]
     L.cpfem_autotune.argtypes = [_CTX, C.c_int32, _P, _P]
+    L.cpfem_calib_begin.argtypes = [_CTX, C.c_int32, C.c_int32, C.c_double, C.c_int32]
+    L.cpfem_calib_record.argtypes = [_CTX, C.c_int32]
+    L.cpfem_calib_history.argtypes = [_CTX, C.c_int32, _P]
+    L.cpfem_calib_run.argtypes = [_CTX, _P, _P, C.c_int32, _P, _P, _P, C.POINTER(CpfemCalibStats)]
+    L.cpfem_calib_exclude.argtypes = [_CTX, _P, _P]
+    L.cpfem_calib_end.argtypes = [_CTX]
+
+
+class CpfemCalibStats(C.Structure):
+    _fields_ = [("points", C.c_int32), ("increments", C.c_int32), ("unconverged", C.c_int32),
+                ("singular", C.c_int32), ("elastic", C.c_int32), ("pad_", C.c_int32),
+                ("t_forward_ms", C.c_double), ("t_reverse_ms", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad_"}
 
 
 # cpfem_autotune flags
@@ -366,6 +381,49 @@ class Solver:
         d = st.as_dict()
         d["code"] = code
         return d
+
+    # --- calibration: replay of a recorded load path and its adjoint -------
+    def calib_begin(self, ninc, ckpt_every=10, toler=1e-10, maxit=200):
+        """Start recording ninc increments from the current state."""
+        _check(self.L.cpfem_calib_begin(self.h, int(ninc), int(ckpt_every), float(toler), int(maxit)),
+               "cpfem_calib_begin")
+
+    def calib_record(self, inc):
+        """Keep the inputs of increment inc's last constitutive call (after
+        nriter, before update_gptv)."""
+        _check(self.L.cpfem_calib_record(self.h, int(inc)), "cpfem_calib_record")
+
+    def calib_history(self, inc):
+        """Recorded inputs of increment inc: de (6), dr (3), dsv(1:3), ds (6), weight."""
+        rec = np.zeros((self.prob.ngps, 19))
+        _check(self.L.cpfem_calib_history(self.h, int(inc), _ptr(rec)), "cpfem_calib_history")
+        return dict(de=rec[:, 0:6], r=rec[:, 6:9], dg=rec[:, 9:12], ds=rec[:, 12:18], w=rec[:, 18])
+
+    def calib_run(self, params, target, comp=2, grad=True):
+        """Replay with params (tau0, taus, h0, q, m): curve, loss, gradient (or None), stats."""
+        params = np.ascontiguousarray(params, np.float64)
+        target = np.ascontiguousarray(target, np.float64)
+        curve = np.zeros(len(target))
+        loss = np.zeros(1)
+        g = np.zeros(5) if grad else None
+        st = CpfemCalibStats()
+        code = self.L.cpfem_calib_run(self.h, _ptr(params), _ptr(target), int(comp), _ptr(curve), _ptr(loss),
+                                      _ptr(g) if grad else None, C.byref(st))
+        if code == -6:
+            raise RuntimeError("cpfem_calib_run: singular Jacobian in a reverse step")
+        _check(code, "cpfem_calib_run")
+        return curve, float(loss[0]), g, st.as_dict()
+
+    def calib_exclude(self, params):
+        """Zero the weight of points whose replay at params does not converge;
+        returns their count."""
+        params = np.ascontiguousarray(params, np.float64)
+        n = np.zeros(1, np.int32)
+        _check(self.L.cpfem_calib_exclude(self.h, _ptr(params), _ptr(n)), "cpfem_calib_exclude")
+        return int(n[0])
+
+    def calib_end(self):
+        _check(self.L.cpfem_calib_end(self.h), "cpfem_calib_end")
 
     def stats(self):
         """Counters and timers accumulated since the last NRITER (or since creation)."""

@@ -59,6 +59,7 @@ struct Calib {
     double* seg_out = nullptr;   // [every][6 + nss][n] new stress, slip rates
     double *stb = nullptr, *phb = nullptr, *s0b = nullptr, *pb = nullptr;  // cotangents
     int* counters = nullptr;     // [4] device: unconverged, singular, elastic
+    int* bad = nullptr;          // [n] 1: the replay stopped at maxit in some increment
     double* partial = nullptr;   // [nblocks][4]
     std::vector<double> hpartial;
     int nblocks = 0;
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(kCalBlock) void k_replay_commit(int n, int nfs, int
                                                              const double* __restrict__ t,
                                                              const double* __restrict__ srate,
                                                              const double* __restrict__ erepmd, double* out,
-                                                             double* partial) {
+                                                             int* bad, double* partial) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     if (g < n) {
@@ -189,6 +190,7 @@ __global__ __launch_bounds__(kCalBlock) void k_replay_commit(int n, int nfs, int
         const double w = rec[kRecW * N + g];
         v[0] = w * st[comp * N + g];
         v[1] = erepmd[g] > toler ? 1.0 : 0.0;
+        if (bad && erepmd[g] > toler) bad[g] = 1;
     }
     block_sum4(v, partial);
 }
@@ -229,6 +231,13 @@ __global__ __launch_bounds__(64) void k_adjoint(int n, int nss, double dt, int c
     for (int i = 0; i < 3; ++i) phb[i * Nn + g] = ph[i];
     for (int i = 0; i < nss; ++i) s0b[i * Nn + g] = s0[i];
     for (int i = 1; i < adj::kParams; ++i) pb[i * Nn + g] += p[i];
+}
+
+// zero weight for the points in bad, every increment
+__global__ void k_exclude(int n, int ninc, const int* __restrict__ bad, double* __restrict__ hist) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n || !bad[g]) return;
+    for (int k = 0; k < ninc; ++k) hist[((size_t)k * kRec + kRecW) * n + g] = 0.0;
 }
 
 // final sums: [0] tau0 (all slip-resistance cotangents at the start), [1..4]
@@ -280,7 +289,7 @@ int set_params(Ctx* c, Calib* K, const double p[5]) {
 }
 
 // One replayed increment n (1-based): load, constitutive update, commit.
-int replay_step(Ctx* c, Calib* K, int n, int comp, double* out, double sums[4]) {
+int replay_step(Ctx* c, Calib* K, int n, int comp, double* out, double sums[4], int* bad = nullptr) {
     hipStream_t s = c->stream;
     const int nb = nb_of(K->n);
     const double* rec = K->hist + (size_t)(n - 1) * kRec * K->n;
@@ -322,7 +331,7 @@ int replay_step(Ctx* c, Calib* K, int n, int comp, double* out, double sums[4]) 
     CPFEM_HIP(he);
     hipLaunchKernelGGL(k_replay_commit, dim3(nb), dim3(kCalBlock), 0, s, K->n, K->nfs, K->nss, comp, K->toler, rec,
                        K->st, (const double*)K->ds, K->sv, (const double*)K->dsv, (const double*)K->t,
-                       (const double*)K->srate, (const double*)K->erepmd, out, K->partial);
+                       (const double*)K->srate, (const double*)K->erepmd, out, bad, K->partial);
     CPFEM_HIP(hipGetLastError());
     return sum_partials(c, K, nb, sums);
 }
@@ -421,6 +430,7 @@ extern "C" int cpfem_calib_begin(cpfem_ctx* c, int32_t ninc, int32_t ckpt_every,
     CAL_ALLOC(K->s0b, (size_t)K->nss * n);
     CAL_ALLOC(K->pb, (size_t)adj::kParams * n);
     CAL_ALLOC(K->counters, 4);
+    CAL_ALLOC(K->bad, n);
     CAL_ALLOC(K->partial, (size_t)4 * K->nblocks);
 #undef CAL_ALLOC
     if (e == hipSuccess) {
@@ -579,6 +589,44 @@ extern "C" int cpfem_calib_run(cpfem_ctx* c, const double* params, const double*
     (void)hipEventDestroy(e1);
     (void)hipEventDestroy(e2);
     return cnt[1] > 0 ? CPFEM_E_SINGULAR : 0;
+}
+
+extern "C" int cpfem_calib_exclude(cpfem_ctx* c, const double* params, int32_t* excluded) {
+    if (!c || !c->calib || !params) return CPFEM_E_ARG;
+    Calib* K = (Calib*)c->calib;
+    for (int i = 0; i < K->ninc; ++i)
+        if (!K->have[i]) return CPFEM_E_ARG;
+    if (!(params[0] > 0.0) || !(params[4] > 0.0)) return CPFEM_E_ARG;
+    int rc = set_params(c, K, params);
+    if (rc) return rc;
+    hipStream_t s = c->stream;
+    const int nb = nb_of(K->n);
+    CPFEM_HIP(hipMemsetAsync(K->bad, 0, sizeof(int) * (size_t)K->n, s));
+    hipLaunchKernelGGL(k_replay_init, dim3(nb), dim3(kCalBlock), 0, s, K->n, K->nss, (const double*)K->st0,
+                       (const double*)K->ph0, params[0], K->st, K->sv);
+    CPFEM_HIP(hipGetLastError());
+    for (int inc = 1; inc <= K->ninc; ++inc) {
+        double sums[4];
+        if ((rc = replay_step(c, K, inc, 2, nullptr, sums, K->bad))) return rc;
+    }
+    hipLaunchKernelGGL(k_exclude, dim3(nb), dim3(kCalBlock), 0, s, K->n, K->ninc, (const int*)K->bad, K->hist);
+    CPFEM_HIP(hipGetLastError());
+    for (int inc = 1; inc <= K->ninc; ++inc) {
+        const double* rec = K->hist + (size_t)(inc - 1) * kRec * K->n;
+        hipLaunchKernelGGL(k_wsum, dim3(nb), dim3(kCalBlock), 0, s, K->n, rec + (size_t)kRecW * K->n, K->partial);
+        CPFEM_HIP(hipGetLastError());
+        double sums[4];
+        if ((rc = sum_partials(c, K, nb, sums))) return rc;
+        K->wsum[inc - 1] = sums[0];
+    }
+    if (excluded) {
+        std::vector<int> b(K->n);
+        CPFEM_HIP(hipMemcpy(b.data(), K->bad, sizeof(int) * b.size(), hipMemcpyDeviceToHost));
+        int cnt = 0;
+        for (int v : b) cnt += v != 0;
+        *excluded = cnt;
+    }
+    return 0;
 }
 
 extern "C" int cpfem_calib_end(cpfem_ctx* c) {

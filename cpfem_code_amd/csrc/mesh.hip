@@ -2249,7 +2249,7 @@ extern "C" int cpfem_volume_average(cpfem_ctx* c, double* out) {
     return 0;
 }
 
-hipError_t point_weights(Ctx* c, double* w) {
+hipError_t cpfem::point_weights(Ctx* c, double* w) {
     KinArgs K;
     memset(&K, 0, sizeof(K));
     K.n = c->ngps;

@@ -260,6 +260,13 @@ int cpfem_calib_record(cpfem_ctx *ctx, int32_t inc);
 int cpfem_calib_history(cpfem_ctx *ctx, int32_t inc, double *rec);
 int cpfem_calib_run(cpfem_ctx *ctx, const double *params, const double *target, int32_t comp, double *curve,
                     double *loss, double *grad, cpfem_calib_stats *stats);
+/* Points whose replay at params stops at maxit in some increment (CPSTRS
+ * held at a fixed point of SCALE_IT's cap, Cpstrs.f90:59,75,187-230, which
+ * solves no constitutive equation and has no derivative of the converged
+ * equations) get zero weight in every increment of the record: the curve is
+ * then the average over the other points and its gradient is exact.
+ * *excluded returns their count. */
+int cpfem_calib_exclude(cpfem_ctx *ctx, const double *params, int32_t *excluded);
 int cpfem_calib_end(cpfem_ctx *ctx);
 
 /* Device name and compute units of the current device (diagnostics). */

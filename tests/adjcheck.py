@@ -136,3 +136,100 @@ def fd_gradients(oracle, mat, b, dt, cot, rel=1e-6, ctl=TIGHT):
 def random_cotangents(rng, n, nss):
     return dict(sb=rng.normal(size=(n, 6)), phb=rng.normal(scale=100.0, size=(n, 3)),
                 s0b=rng.normal(size=(n, nss)))
+
+
+def point_materials(prob):
+    """Material code of each point's element (1-based point numbers in lg)."""
+    iel = np.zeros(prob.ngps, np.int32)
+    for e in range(prob.nlmnt):
+        for gp in prob.lg[e]:
+            if gp > 0:
+                iel[gp - 1] = prob.le[e]
+    return iel
+
+
+def replay_oracle(oracle, prob, params, hist, comp=2, ctl=TIGHT):
+    """The replay of cpfem_calib_run restated with the FP64 oracle: every
+    point's recorded increments through DSTRSS, then UPDATE_GPTV's commit."""
+    from cpfem_code_amd.problem import MAXSV
+    tau0, taus, h0, q, m = (float(v) for v in params)
+    mat = with_params(prob.materials[0], taus=taus, h0=h0, q=q, m=m)
+    n, nss = prob.ngps, mat.nss
+    iel = point_materials(prob)
+    st = np.zeros((n, 6))
+    sv = np.array(prob.sv, np.float64)
+    sv[:, 3:3 + nss] = tau0
+    curve = np.zeros(len(hist))
+    for k, h in enumerate(hist):
+        dsv = np.zeros((n, MAXSV))
+        dsv[:, :3] = h["dg"]
+        o = oracle.dstrss_batch([mat], ctl, float(prob.tiem[k]), iel, h["de"], h["r"], st, h["ds"], sv, dsv)
+        st = st + o["ds"]
+        sv[:, :3 + nss] = sv[:, :3 + nss] + o["dsv"][:, :3 + nss]
+        curve[k] = (h["w"] * st[:, comp]).sum() / h["w"].sum()
+    return curve, st, sv
+
+
+def record_oracle(oracle, prob, ninc):
+    """The load path run by the oracle (UPDATE_NODE, NRITER, UPDATE_GPTV), with
+    the inputs of each increment's last DSTRSS call kept as cpfem_calib_record
+    keeps them (unit weights)."""
+    from oracle import OracleModel
+    om = OracleModel(oracle, prob)
+    lib, m = om.o.lib, om.m
+    hist = []
+    for inc in range(1, ninc + 1):
+        m.inc = inc
+        lib.orc_update_node(C.byref(m))
+        lib.orc_nriter(C.byref(m))
+        hist.append(dict(de=om.de.copy(), r=om.dr.copy(), dg=om.dsv[:, :3].copy(), ds=om.ds.copy(),
+                         w=np.ones(prob.ngps)))
+        lib.orc_update_gptv(C.byref(m))
+    return hist
+
+
+def adjoint_replay_host(L, oracle, prob, params, hist, target, comp=2, ctl=TIGHT):
+    """The reverse sweep of cpfem_calib_run on the CPU: the oracle's replay
+    forward, then the host build of the per-point reverse step, increment by
+    increment backwards.  Returns (loss, gradient, curve)."""
+    from cpfem_code_amd.problem import MAXSV
+    tau0, taus, h0, q, m = (float(v) for v in params)
+    mat = with_params(prob.materials[0], taus=taus, h0=h0, q=q, m=m)
+    ms = capi.material_struct(mat)
+    n, nss = prob.ngps, mat.nss
+    iel = point_materials(prob)
+    st = np.zeros((n, 6))
+    sv = np.array(prob.sv, np.float64)
+    sv[:, 3:3 + nss] = tau0
+    states, outs = [], []
+    curve = np.zeros(len(hist))
+    for k, h in enumerate(hist):
+        states.append((st.copy(), sv.copy()))
+        dsv = np.zeros((n, MAXSV))
+        dsv[:, :3] = h["dg"]
+        o = oracle.dstrss_batch([mat], ctl, float(prob.tiem[k]), iel, h["de"], h["r"], st, h["ds"], sv, dsv)
+        outs.append((o["t"].copy(), o["srate"][:, :nss].copy()))
+        st = st + o["ds"]
+        sv[:, :3 + nss] = sv[:, :3 + nss] + o["dsv"][:, :3 + nss]
+        curve[k] = (h["w"] * st[:, comp]).sum() / h["w"].sum()
+    loss = float(((curve - target) ** 2).sum())
+    stb, phb, s0b, pb = np.zeros((n, 6)), np.zeros((n, 3)), np.zeros((n, nss)), np.zeros(5)
+    f = lambda a: np.ascontiguousarray(a, np.float64)
+    for k in range(len(hist) - 1, -1, -1):
+        h = hist[k]
+        coef = 2.0 * (curve[k] - target[k]) / h["w"].sum()
+        st0, sv0 = states[k]
+        so_all, sr_all = outs[k]
+        dt = float(prob.tiem[k])
+        for g in range(n):
+            sb = f(stb[g]).copy()
+            sb[comp] += coef * h["w"][g]
+            ph, s0v, stn, p = f(phb[g]).copy(), f(s0b[g]).copy(), np.zeros(6), np.zeros(5)
+            L.adj_point_reverse_host(C.byref(ms), dt, int(iel[g]), _p(f(st0[g])), _p(f(sv0[g, :3])),
+                                     _p(f(sv0[g, 3:3 + nss])), _p(f(h["de"][g])), _p(f(h["r"][g])),
+                                     _p(f(h["dg"][g])), _p(f(h["ds"][g])), _p(f(so_all[g])), _p(f(sr_all[g])),
+                                     _p(sb), _p(ph), _p(s0v), _p(stn), _p(p))
+            stb[g], phb[g], s0b[g] = stn, ph, s0v
+            pb += p
+    grad = np.array([s0b.sum(), pb[1], pb[2], pb[3], pb[4]])
+    return loss, grad, curve

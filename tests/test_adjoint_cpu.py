@@ -16,8 +16,10 @@ of magnitude (checked by the planted-mistake test below).
 import numpy as np
 import pytest
 
-from adjcheck import adjoint, fd_gradients, host_lib, random_cotangents, with_params
-from cpfem_code_amd.problem import bcc_iron, fcc_aluminium
+from adjcheck import (adjoint, adjoint_replay_host, fd_gradients, host_lib, point_materials, random_cotangents,
+                      record_oracle, replay_oracle, with_params)
+from cpfem_code_amd import calibrate
+from cpfem_code_amd.problem import MAXSV, Control, bcc_iron, fcc_aluminium, fcc_tension
 from gpbatch import make_gp_batch
 from oracle import Oracle
 
@@ -75,3 +77,38 @@ def test_planted_mistake_is_caught():
     wrong["p"][:, 2] += a["p"][:, 3] * 1.4 / 200.0   # h0 picking up a q term
     with pytest.raises(AssertionError):
         _compare(wrong, f, {"p": 1e-5})
+
+
+def test_replay_adjoint_over_increments_matches_finite_differences():
+    """The reverse sweep over a recorded load path (2x2x2 bricks, 6
+    increments of tension, oracle forward and host reverse steps) against
+    central differences of the oracle's replay, with the points that stop at
+    the iteration limit given zero weight as cpfem_calib_exclude does."""
+    prob = fcc_tension(2, ninc=6, seed=4)
+    rc = Control(toler=1e-10, maxit=200)
+    hist = record_oracle(O64, prob, 6)
+    mat, iel, n = prob.materials[0], point_materials(prob), prob.ngps
+    st, sv, bad = np.zeros((n, 6)), np.array(prob.sv, float), np.zeros(n, bool)
+    for k, h in enumerate(hist):
+        dsv = np.zeros((n, MAXSV))
+        dsv[:, :3] = h["dg"]
+        o = O64.dstrss_batch([mat], rc, float(prob.tiem[k]), iel, h["de"], h["r"], st, h["ds"], sv, dsv)
+        bad |= o["outer"] >= rc.maxit
+        st, sv[:, :15] = st + o["ds"], sv[:, :15] + o["dsv"][:, :15]
+    assert bad.sum() < n // 2
+    for h in hist:
+        h["w"] = np.where(bad, 0.0, 1.0)
+    p0 = calibrate.material_params(mat)
+    pt = p0 * (1 + 0.1 * np.random.default_rng(4).uniform(-1, 1, 5))
+    target, _, _ = replay_oracle(O64, prob, pt, hist, ctl=rc)
+    _, g, _ = adjoint_replay_host(host_lib(), O64, prob, p0, hist, target, ctl=rc)
+    fd = np.zeros(5)
+    for k in range(5):
+        h = 1e-6 * abs(p0[k])
+        up, dn = p0.copy(), p0.copy()
+        up[k] += h
+        dn[k] -= h
+        cu, _, _ = replay_oracle(O64, prob, up, hist, ctl=rc)
+        cd, _, _ = replay_oracle(O64, prob, dn, hist, ctl=rc)
+        fd[k] = (((cu - target) ** 2).sum() - ((cd - target) ** 2).sum()) / (2 * h)
+    assert np.all(np.abs(g - fd) <= 1e-5 * np.abs(fd) + 1e-7 * np.abs(fd).max()), (g, fd)

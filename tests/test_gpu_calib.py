@@ -1,0 +1,136 @@
+"""Calibration path on the GPU: recorded load path, replay and adjoint.
+
+* The replay (cpfem_calib_run, product constitutive kernels) reproduces the
+  oracle's replay of the same recorded increments (FP64, tolerance 1e-10):
+  curves and final stresses within 1e-9 relative.
+* The replay at the material's own parameters reproduces the finite-element
+  run's curve to the finite-element run's constitutive tolerance.
+* With the points excluded whose replay stops at the iteration limit (the
+  reference's SCALE_IT cap holds them at a fixed point that solves no
+  constitutive equation, Cpstrs.f90:59,75,187-230), the adjoint gradient
+  equals central differences of the replay loss (relative step 1e-6) on the
+  GPU and of the oracle's replay on the CPU, within 1e-5 of each component's
+  magnitude plus 1e-7 of the gradient's largest entry.
+* Gradients do not depend on the checkpoint spacing (bitwise).
+"""
+import numpy as np
+import pytest
+
+from adjcheck import replay_oracle
+from cpfem_code_amd import calibrate
+from cpfem_code_amd.problem import Control, fcc_tension
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+O64 = Oracle(64)
+RCTL = Control(toler=1e-10, maxit=200)   # the replay's tolerance (record_path's default)
+
+
+@pytest.fixture(scope="module")
+def path():
+    prob = fcc_tension(4, ninc=12, seed=4)
+    print("calib: recording 4^3 FCC tension, 12 increments", flush=True)
+    p = calibrate.record_path(prob, ninc=12, comp=2, ckpt_every=5)
+    yield prob, p
+    p.solver.close()
+
+
+@pytest.fixture(scope="module")
+def path_excluded():
+    prob = fcc_tension(4, ninc=12, seed=4)
+    print("calib: recording 4^3 FCC tension, 12 increments (non-converging points excluded)", flush=True)
+    p = calibrate.record_path(prob, ninc=12, comp=2, ckpt_every=5)
+    nx = p.solver.calib_exclude(calibrate.material_params(prob.materials[0]))
+    print(f"calib: {nx} of {prob.ngps} points excluded", flush=True)
+    assert nx < prob.ngps // 2
+    yield prob, p
+    p.solver.close()
+
+
+def _grad_bar(g, ref):
+    return np.abs(g - ref) <= 1e-5 * np.abs(ref) + 1e-7 * np.abs(ref).max()
+
+
+def test_replay_matches_oracle_replay(path):
+    prob, p = path
+    p0 = calibrate.material_params(prob.materials[0])
+    hist = [p.solver.calib_history(i) for i in range(1, p.ninc + 1)]
+    tgt = np.zeros(p.ninc)
+    curve, loss, _, st = p.solver.calib_run(p0, tgt, comp=2, grad=False)
+    ocurve, _, _ = replay_oracle(O64, prob, p0, hist, comp=2, ctl=RCTL)
+    rel = np.abs(curve - ocurve).max() / np.abs(ocurve).max()
+    print(f"calib: replay vs oracle replay {rel:.2e}, unconverged {st['unconverged']}", flush=True)
+    assert rel < 1e-9
+    assert abs(loss - (curve ** 2).sum()) <= 1e-12 * loss
+
+
+def test_replay_reproduces_finite_element_curve(path):
+    prob, p = path
+    p0 = calibrate.material_params(prob.materials[0])
+    curve, _, _, _ = p.solver.calib_run(p0, np.zeros(p.ninc), comp=2, grad=False)
+    rel = np.abs(curve - p.fe_curve).max() / np.abs(p.fe_curve).max()
+    print(f"calib: replay vs finite-element curve {rel:.2e} (FE toler {prob.control.toler:g})", flush=True)
+    assert rel < 10 * prob.control.toler
+
+
+def test_gradient_matches_central_differences(path_excluded):
+    prob, p = path_excluded
+    p0 = calibrate.material_params(prob.materials[0])
+    target, _ = calibrate.seeded_target(p, p0, seed=4)
+    loss, g, _, st = calibrate.loss_and_grad(p, p0, target)
+    assert st["singular"] == 0
+    fd = calibrate.central_differences(p, p0, target)
+    print("calib: adjoint", g, "\ncalib: fd (gpu)", fd, flush=True)
+    assert _grad_bar(g, fd).all(), (g, fd)
+    # central differences of the oracle's replay (CPU)
+    hist = [p.solver.calib_history(i) for i in range(1, p.ninc + 1)]
+    fo = np.zeros(5)
+    for k in range(5):
+        h = 1e-6 * abs(p0[k])
+        up, dn = p0.copy(), p0.copy()
+        up[k] += h
+        dn[k] -= h
+        cu, _, _ = replay_oracle(O64, prob, up, hist, ctl=RCTL)
+        cd, _, _ = replay_oracle(O64, prob, dn, hist, ctl=RCTL)   # history carries the zeroed weights
+        fo[k] = (((cu - target) ** 2).sum() - ((cd - target) ** 2).sum()) / (2 * h)
+    print("calib: fd (oracle)", fo, flush=True)
+    assert _grad_bar(g, fo).all(), (g, fo)
+
+
+def test_gradient_without_exclusion_is_close(path):
+    """With the non-converging points kept, the adjoint still treats them as
+    converged: the gradient is near but not equal to central differences."""
+    prob, p = path
+    p0 = calibrate.material_params(prob.materials[0])
+    target, _ = calibrate.seeded_target(p, p0, seed=4)
+    _, g, _, _ = calibrate.loss_and_grad(p, p0, target)
+    fd = calibrate.central_differences(p, p0, target)
+    rel = np.abs(g - fd) / np.abs(fd)
+    print("calib: without exclusion, relative error per parameter", rel, flush=True)
+    assert np.all(np.sign(g[:3]) == np.sign(fd[:3]))
+
+
+def test_gradient_independent_of_checkpoint_spacing():
+    prob = fcc_tension(3, ninc=7, seed=5)
+    p0 = calibrate.material_params(prob.materials[0])
+    grads = []
+    for every in (1, 3, 7):
+        p = calibrate.record_path(prob, ninc=7, ckpt_every=every)
+        target = np.linspace(0.0, 80.0, 7)
+        _, g, _, _ = calibrate.loss_and_grad(p, p0, target)
+        grads.append(g)
+        p.solver.close()
+    assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2]), grads
+
+
+def test_torch_autograd_wrapper(path_excluded):
+    import torch
+    prob, p = path_excluded
+    p0 = calibrate.material_params(prob.materials[0])
+    target, _ = calibrate.seeded_target(p, p0, seed=4)
+    CalibLoss = calibrate.make_torch_loss()
+    x = torch.tensor(p0, dtype=torch.float64, requires_grad=True)
+    loss = CalibLoss.apply(x, p, target)
+    loss.backward()
+    _, g, _, _ = calibrate.loss_and_grad(p, p0, target)
+    assert np.array_equal(x.grad.numpy(), g)
