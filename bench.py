@@ -223,6 +223,75 @@ def cpu_baseline(prob, state, seconds, capi_mod):
                        f"{t_total:.1f} s including the harness file I/O")
 
 
+def calibration_leg(capi_mod, n, ctl, cpu=True):
+    """Gradient of the macroscopic stress-strain curve (16^3 FCC tension, 50
+    increments, FP64): the load path on the GPU with the inputs of every
+    point's last constitutive call recorded, then the replay loss
+    sum (curve - target)^2 and its adjoint gradient with respect to tau0,
+    taus, h0, latent ratio and m, next to central differences (relative step
+    1e-6) of the same replay."""
+    from cpfem_code_amd import calibrate
+    from cpfem_code_amd.problem import fcc_tension
+    prob = fcc_tension(n, ninc=50, seed=4, control=ctl)
+    t0 = time.time()
+    path = calibrate.record_path(prob, ninc=50, comp=2, ckpt_every=10)
+    t_record = time.time() - t0
+    p0 = calibrate.material_params(prob.materials[0])
+    c_all = path.solver.calib_run(p0, np.zeros(50), grad=False)[0]
+    fe_rel = float(np.abs(c_all - path.fe_curve).max() / np.abs(path.fe_curve).max())
+    nx = calibrate.exclude_nonconverging(path, p0, rel=1e-6)
+    target, p_t = calibrate.seeded_target(path, p0, seed=4)
+    calibrate.loss_and_grad(path, p0, target)          # warm-up
+    t0 = time.time()
+    loss, g, curve, st = calibrate.loss_and_grad(path, p0, target)
+    t_grad = time.time() - t0
+    t0 = time.time()
+    _, _, _, stf = calibrate.loss_and_grad(path, p0, target, grad=False)
+    t_fwd = time.time() - t0
+    t0 = time.time()
+    fd = calibrate.central_differences(path, p0, target, rel=1e-6)
+    t_fd = time.time() - t0
+    out = {
+        "workload": f"FCC Al uniaxial tension along z, {n}^3 8-node bricks x 8 points, 50 increments of 1e-3, FP64",
+        "points": prob.ngps, "increments": 50, "parameters": list(calibrate.PARAM_NAMES),
+        "params": p0.tolist(), "target_params_seed4": p_t.tolist(),
+        "points_excluded_nonconverging": nx,
+        "record_s": t_record, "replay_forward_s": t_fwd, "loss_and_gradient_s": t_grad,
+        "forward_ms_in_gradient": st["t_forward_ms"], "reverse_ms_in_gradient": st["t_reverse_ms"],
+        "central_differences_s": t_fd, "loss": loss, "gradient": g.tolist(), "central_differences": fd.tolist(),
+        "max_rel_diff_vs_central_differences": float(np.max(np.abs(g - fd) / np.abs(fd))),
+        "replay_vs_finite_element_curve_max_rel": fe_rel,
+        "checkpoint_every": 10,
+        "note": "replay tolerance 1e-10 (maxit 200); points that stop at the iteration limit at the parameters or "
+                "on their difference stencil get zero weight; the gradient holds the recorded kinematics fixed",
+    }
+    # the same gradient with the whole trajectory kept in HBM (no recomputation)
+    path.solver.calib_checkpoints(0)
+    calibrate.loss_and_grad(path, p0, target)
+    t0 = time.time()
+    _, g_keep, _, st_keep = calibrate.loss_and_grad(path, p0, target)
+    out["trajectory_kept"] = {"loss_and_gradient_s": time.time() - t0, "reverse_ms": st_keep["t_reverse_ms"],
+                              "gradient_equal_to_checkpointed": bool(np.array_equal(g_keep, g))}
+    if cpu:
+        # the oracle's replay of the first increments on one core
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle import Oracle
+        from adjcheck import replay_oracle
+        from cpfem_code_amd.problem import Control as Ctl
+        hist = [path.solver.calib_history(i) for i in range(1, 51)]
+        t0 = time.time()
+        k = 5
+        replay_oracle(Oracle(64), prob, p0, hist[:k], ctl=Ctl(toler=1e-10, maxit=200))
+        t1 = (time.time() - t0) / k
+        out["cpu_baseline_replay"] = {"kind": "port", "cores": 1,
+                                      "sample": f"oracle replay of increments 1-{k} of 50 (the yield transition)",
+                                      "s_per_increment": t1, "value": prob.ngps / t1, "unit": "GP-updates/s",
+                                      "gpu_replay_s_per_increment": t_fwd / 50}
+    path.solver.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,6 +311,9 @@ def main():
                     help="comma list of path:omega (path fp64 or mixed; omega a number, or 'auto' for the omega "
                          "and launch shape tuned at the first solve) to run as well, reported beside the "
                          "headline, which uses the configured 1.6; empty skips them")
+    ap.add_argument("--calibration", type=int, default=16,
+                    help="mesh size of the calibration leg (replay loss and adjoint gradient over 50 increments "
+                         "of tension); 0 skips it")
     ap.add_argument("--trace-iterations", action="store_true",
                     help="drive each increment through the per-step calls and print every Newton iteration")
     args = ap.parse_args()
@@ -388,6 +460,8 @@ def main():
         "device": name,
     }
     line.update(extra)
+    if args.calibration > 0:
+        line["calibration"] = calibration_leg(capi, args.calibration, ctl, cpu=args.cpu_seconds > 0)
     if cpu:
         line["gp_rate_vs_cpu_reference"] = value / cpu["value"]
     print(json.dumps(line))

@@ -77,8 +77,11 @@ def loss_and_grad(path: RecordedPath, params, target, grad: bool = True):
     return loss, g, curve, st
 
 
-def central_differences(path: RecordedPath, params, target, rel: float = 1e-6) -> np.ndarray:
-    """Central differences of the replay loss (relative step rel per parameter)."""
+def central_differences(path: RecordedPath, params, target, rel: float = 1e-6, counts: Optional[list] = None):
+    """Central differences of the replay loss (relative step rel per parameter).
+    counts, if given, receives (unconverged at +h, unconverged at -h) per
+    parameter: a change from the unperturbed count marks a point that crossed
+    into or out of the non-converging regime, a jump in the loss."""
     p = np.asarray(params, np.float64)
     g = np.zeros(5)
     for k in range(5):
@@ -86,10 +89,29 @@ def central_differences(path: RecordedPath, params, target, rel: float = 1e-6) -
         up, dn = p.copy(), p.copy()
         up[k] += h
         dn[k] -= h
-        lu = path.solver.calib_run(up, target, comp=path.comp, grad=False)[1]
-        ld = path.solver.calib_run(dn, target, comp=path.comp, grad=False)[1]
+        _, lu, _, su = path.solver.calib_run(up, target, comp=path.comp, grad=False)
+        _, ld, _, sd = path.solver.calib_run(dn, target, comp=path.comp, grad=False)
         g[k] = (lu - ld) / (2 * h)
+        if counts is not None:
+            counts.append((su["unconverged"], sd["unconverged"]))
     return g
+
+
+def exclude_nonconverging(path: RecordedPath, params, rel: float = 0.0) -> int:
+    """Give zero weight to the points whose replay stops at the iteration
+    limit at params and, with rel > 0, at each params +- rel*|p_k| e_k (the
+    stencil of central differences): points near the non-converging regime of
+    CPSTRS cross it under small changes of the parameters, which puts jumps in
+    the loss.  Exclusions accumulate; returns the number of points excluded."""
+    p = np.asarray(params, np.float64)
+    n = path.solver.calib_exclude(p)
+    if rel > 0.0:
+        for k in range(5):
+            for sgn in (1.0, -1.0):
+                q = p.copy()
+                q[k] += sgn * rel * abs(p[k])
+                n = path.solver.calib_exclude(q)
+    return n
 
 
 def make_torch_loss():

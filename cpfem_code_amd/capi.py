@@ -193,6 +193,7 @@ def _bind_mesh_api(L):
     L.cpfem_calib_history.argtypes = [_CTX, C.c_int32, _P]
     L.cpfem_calib_run.argtypes = [_CTX, _P, _P, C.c_int32, _P, _P, _P, C.POINTER(CpfemCalibStats)]
     L.cpfem_calib_exclude.argtypes = [_CTX, _P, _P]
+    L.cpfem_calib_checkpoints.argtypes = [_CTX, C.c_int32]
     L.cpfem_calib_end.argtypes = [_CTX]
 
 
@@ -416,11 +417,15 @@ class Solver:
 
     def calib_exclude(self, params):
         """Zero the weight of points whose replay at params does not converge;
-        returns their count."""
+        returns the number of points excluded so far."""
         params = np.ascontiguousarray(params, np.float64)
         n = np.zeros(1, np.int32)
         _check(self.L.cpfem_calib_exclude(self.h, _ptr(params), _ptr(n)), "cpfem_calib_exclude")
         return int(n[0])
+
+    def calib_checkpoints(self, ckpt_every):
+        """Checkpoint spacing of the reverse sweep (<= 0: keep the whole trajectory)."""
+        _check(self.L.cpfem_calib_checkpoints(self.h, int(ckpt_every)), "cpfem_calib_checkpoints")
 
     def calib_end(self):
         _check(self.L.cpfem_calib_end(self.h), "cpfem_calib_end")

@@ -60,6 +60,7 @@ struct Calib {
     double *stb = nullptr, *phb = nullptr, *s0b = nullptr, *pb = nullptr;  // cotangents
     int* counters = nullptr;     // [4] device: unconverged, singular, elastic
     int* bad = nullptr;          // [n] 1: the replay stopped at maxit in some increment
+    std::vector<int> excluded;   // [n] host: points given zero weight so far
     double* partial = nullptr;   // [nblocks][4]
     std::vector<double> hpartial;
     int nblocks = 0;
@@ -261,6 +262,25 @@ __global__ __launch_bounds__(kCalBlock) void k_grad_sums(int n, int nss, const d
     block_sum4(v, partial);
 }
 
+// Checkpoints and segment buffers for ckpt_every increments per segment
+// (<= 0: one segment, the whole trajectory kept by the forward pass).
+hipError_t alloc_segments(Calib* K, int ckpt_every, hipStream_t s) {
+    for (double* p : {K->ck, K->seg_state, K->seg_out}) {
+        if (!p) continue;
+        auto it = std::find(K->allocs.begin(), K->allocs.end(), (void*)p);
+        if (it != K->allocs.end()) K->allocs.erase(it);
+        (void)hipFree(p);
+    }
+    K->ck = K->seg_state = K->seg_out = nullptr;
+    K->every = ckpt_every <= 0 ? K->ninc : std::min(ckpt_every, K->ninc);
+    K->nck = (K->ninc + K->every - 1) / K->every;
+    const size_t n = (size_t)K->n;
+    hipError_t e = calloc_dev(K, &K->ck, (size_t)K->nck * (6 + K->nfs) * n, s);
+    if (e == hipSuccess) e = calloc_dev(K, &K->seg_state, (size_t)K->every * (6 + K->nfs) * n, s);
+    if (e == hipSuccess) e = calloc_dev(K, &K->seg_out, (size_t)K->every * (6 + K->nss) * n, s);
+    return e;
+}
+
 int sum_partials(Ctx* c, Calib* K, int nb, double out[4]) {
     CPFEM_HIP(hipMemcpyAsync(K->hpartial.data(), K->partial, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost,
                              c->stream));
@@ -368,7 +388,7 @@ void calib_free(Ctx* c) {
 using namespace cpfem;
 
 extern "C" int cpfem_calib_begin(cpfem_ctx* c, int32_t ninc, int32_t ckpt_every, double toler, int32_t maxit) {
-    if (!c || ninc < 1 || ninc > c->ninc1 || ckpt_every < 1 || !(toler > 0.0) || maxit < 1) return CPFEM_E_ARG;
+    if (!c || ninc < 1 || ninc > c->ninc1 || !(toler > 0.0) || maxit < 1) return CPFEM_E_ARG;
     // one material, no twin systems, uniform parameters over the systems and
     // a two-valued latent matrix: the five parameters then have one meaning
     if (c->nmat != 1) return CPFEM_E_ARG;
@@ -385,8 +405,6 @@ extern "C" int cpfem_calib_begin(cpfem_ctx* c, int32_t ninc, int32_t ckpt_every,
     Calib* K = new Calib();
     c->calib = K;
     K->ninc = ninc;
-    K->every = std::min((int)ckpt_every, (int)ninc);
-    K->nck = (ninc + K->every - 1) / K->every;
     K->n = c->ngps;
     K->nss = m.nss;
     K->nfs = 3 + c->nss;
@@ -422,9 +440,6 @@ extern "C" int cpfem_calib_begin(cpfem_ctx* c, int32_t ninc, int32_t ckpt_every,
     CAL_ALLOC(K->flags, 4);
     CAL_ALLOC(K->flops, n);
     CAL_ALLOC(K->transc, n);
-    CAL_ALLOC(K->ck, (size_t)K->nck * (6 + K->nfs) * n);
-    CAL_ALLOC(K->seg_state, (size_t)K->every * (6 + K->nfs) * n);
-    CAL_ALLOC(K->seg_out, (size_t)K->every * (6 + K->nss) * n);
     CAL_ALLOC(K->stb, 6 * n);
     CAL_ALLOC(K->phb, 3 * n);
     CAL_ALLOC(K->s0b, (size_t)K->nss * n);
@@ -441,6 +456,7 @@ extern "C" int cpfem_calib_begin(cpfem_ctx* c, int32_t ninc, int32_t ckpt_every,
             K->dmat = dm;
         }
     }
+    if (e == hipSuccess) e = alloc_segments(K, ckpt_every, s);
     if (e != hipSuccess) {
         calib_free(c);
         fprintf(stderr, "cpfem: calibration buffers: %s\n", hipGetErrorString(e));
@@ -514,14 +530,23 @@ extern "C" int cpfem_calib_run(cpfem_ctx* c, const double* params, const double*
     std::vector<double> cur(K->ninc), coef(K->ninc);
     double L = 0.0;
     long long unconv = 0;
+    // the last segment's states and outputs are kept by this pass, so the
+    // reverse sweep starts without recomputing it
+    const int last0 = 1 + (K->nck - 1) * K->every;
     for (int inc = 1; inc <= K->ninc; ++inc) {
         if (grad && (inc - 1) % K->every == 0) {
             const int k = (inc - 1) / K->every;
             hipLaunchKernelGGL(k_state_copy, dim3(nb), dim3(kCalBlock), 0, s, K->n, K->nfs, (const double*)K->st,
                                (const double*)K->sv, K->ck + (size_t)k * fs);
         }
+        double* keep = nullptr;
+        if (grad && inc >= last0) {
+            hipLaunchKernelGGL(k_state_copy, dim3(nb), dim3(kCalBlock), 0, s, K->n, K->nfs, (const double*)K->st,
+                               (const double*)K->sv, K->seg_state + (size_t)(inc - last0) * fs);
+            keep = K->seg_out + (size_t)(inc - last0) * fo;
+        }
         double sums[4];
-        if ((rc = replay_step(c, K, inc, comp, nullptr, sums))) return rc;
+        if ((rc = replay_step(c, K, inc, comp, keep, sums))) return rc;
         const double W = K->wsum[inc - 1];
         cur[inc - 1] = W > 0.0 ? sums[0] / W : 0.0;
         unconv += (long long)llround(sums[1]);
@@ -542,7 +567,7 @@ extern "C" int cpfem_calib_run(cpfem_ctx* c, const double* params, const double*
             const int n0 = 1 + k * K->every, n1 = std::min(K->ninc, n0 + K->every - 1);
             hipLaunchKernelGGL(k_state_restore, dim3(nb), dim3(kCalBlock), 0, s, K->n, K->nfs,
                                (const double*)(K->ck + (size_t)k * fs), K->st, K->sv);
-            for (int inc = n0; inc <= n1; ++inc) {
+            for (int inc = n0; inc <= n1 && k < K->nck - 1; ++inc) {
                 double* state = K->seg_state + (size_t)(inc - n0) * fs;
                 hipLaunchKernelGGL(k_state_copy, dim3(nb), dim3(kCalBlock), 0, s, K->n, K->nfs,
                                    (const double*)K->st, (const double*)K->sv, state);
@@ -619,13 +644,24 @@ extern "C" int cpfem_calib_exclude(cpfem_ctx* c, const double* params, int32_t* 
         if ((rc = sum_partials(c, K, nb, sums))) return rc;
         K->wsum[inc - 1] = sums[0];
     }
-    if (excluded) {
-        std::vector<int> b(K->n);
-        CPFEM_HIP(hipMemcpy(b.data(), K->bad, sizeof(int) * b.size(), hipMemcpyDeviceToHost));
-        int cnt = 0;
-        for (int v : b) cnt += v != 0;
-        *excluded = cnt;
+    std::vector<int> b(K->n);
+    CPFEM_HIP(hipMemcpy(b.data(), K->bad, sizeof(int) * b.size(), hipMemcpyDeviceToHost));
+    if (K->excluded.size() != b.size()) K->excluded.assign(b.size(), 0);
+    int cnt = 0;
+    for (size_t g = 0; g < b.size(); ++g) {
+        K->excluded[g] |= b[g] != 0;
+        cnt += K->excluded[g];
     }
+    if (excluded) *excluded = cnt;
+    return 0;
+}
+
+extern "C" int cpfem_calib_checkpoints(cpfem_ctx* c, int32_t ckpt_every) {
+    if (!c || !c->calib) return CPFEM_E_ARG;
+    Calib* K = (Calib*)c->calib;
+    CPFEM_HIP(hipStreamSynchronize(c->stream));
+    CPFEM_HIP(alloc_segments(K, ckpt_every, c->stream));
+    CPFEM_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
 

@@ -231,7 +231,10 @@ int cpfem_autotune(cpfem_ctx *ctx, int32_t flags, double *omega, int32_t *waves_
  * iteration limit of the replay: the gradient is that of the converged
  * equations, so the replay should converge far below the reference's toler
  * (1e-10 for a 1e-6 finite-difference check).  ckpt_every: increments per
- * checkpoint of the reverse sweep.
+ * checkpoint of the reverse sweep, whose segments except the last are run
+ * forward again from their checkpoints; ckpt_every <= 0 keeps the whole
+ * trajectory from the forward pass (ninc * (15 + 2 nss) doubles per point)
+ * and recomputes nothing.
  *
  * cpfem_calib_record(inc) keeps, for every point, the inputs of the last
  * DSTRSS call of increment inc (de, dr, dsv(1:3), ds) and its integration
@@ -265,8 +268,12 @@ int cpfem_calib_run(cpfem_ctx *ctx, const double *params, const double *target, 
  * solves no constitutive equation and has no derivative of the converged
  * equations) get zero weight in every increment of the record: the curve is
  * then the average over the other points and its gradient is exact.
- * *excluded returns their count. */
+ * Exclusions accumulate over calls (e.g. at each point of a finite-difference
+ * stencil); *excluded returns the number of points excluded so far. */
 int cpfem_calib_exclude(cpfem_ctx *ctx, const double *params, int32_t *excluded);
+/* Change the checkpoint spacing of the reverse sweep (as ckpt_every of
+ * cpfem_calib_begin); the record is kept. */
+int cpfem_calib_checkpoints(cpfem_ctx *ctx, int32_t ckpt_every);
 int cpfem_calib_end(cpfem_ctx *ctx);
 
 /* Device name and compute units of the current device (diagnostics). */
