@@ -45,9 +45,12 @@ class RecordedPath:
 
 
 def record_path(prob: Problem, ninc: Optional[int] = None, comp: int = 2, ckpt_every: int = 10,
-                toler: float = 1e-10, maxit: int = 200, precision: int = 0, control=None) -> RecordedPath:
+                toler: float = 1e-10, maxit: int = 60, precision: int = 0, control=None) -> RecordedPath:
     """Run ninc increments of prob on the GPU, recording every point's last
-    constitutive inputs; returns the solver holding the record."""
+    constitutive inputs; returns the solver holding the record.  toler and
+    maxit are the replay's CPSTRS settings: converging points need well under
+    60 outer iterations at 1e-10 (16^3 tension: the same loss and gradient,
+    bit for bit, as with 200), and the ones that stop there are excluded."""
     ninc = ninc or prob.ninc1 - 1
     s = capi.Solver(prob, precision=precision, control=control)
     s.calib_begin(ninc, ckpt_every=ckpt_every, toler=toler, maxit=maxit)

@@ -384,7 +384,7 @@ class Solver:
         return d
 
     # --- calibration: replay of a recorded load path and its adjoint -------
-    def calib_begin(self, ninc, ckpt_every=10, toler=1e-10, maxit=200):
+    def calib_begin(self, ninc, ckpt_every=10, toler=1e-10, maxit=60):
         """Start recording ninc increments from the current state."""
         _check(self.L.cpfem_calib_begin(self.h, int(ninc), int(ckpt_every), float(toler), int(maxit)),
                "cpfem_calib_begin")

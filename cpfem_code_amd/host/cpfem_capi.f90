@@ -165,5 +165,42 @@ MODULE CPFEM_CAPI
             REAL(C_DOUBLE), INTENT(OUT) :: omega
             INTEGER(C_INT32_T), INTENT(OUT) :: waves_per_block
         END FUNCTION
+
+        ! calibration: record of the load path, replay loss and its adjoint gradient
+        INTEGER(C_INT) FUNCTION cpfem_calib_begin(ctx, ninc, ckpt_every, toler, maxit) &
+                BIND(C, name='cpfem_calib_begin')
+            IMPORT :: C_INT, C_INT32_T, C_PTR, C_DOUBLE
+            TYPE(C_PTR), VALUE :: ctx
+            INTEGER(C_INT32_T), VALUE :: ninc, ckpt_every, maxit
+            REAL(C_DOUBLE), VALUE :: toler
+        END FUNCTION
+
+        INTEGER(C_INT) FUNCTION cpfem_calib_record(ctx, inc) BIND(C, name='cpfem_calib_record')
+            IMPORT :: C_INT, C_INT32_T, C_PTR
+            TYPE(C_PTR), VALUE :: ctx
+            INTEGER(C_INT32_T), VALUE :: inc
+        END FUNCTION
+
+        INTEGER(C_INT) FUNCTION cpfem_calib_exclude(ctx, params, excluded) BIND(C, name='cpfem_calib_exclude')
+            IMPORT :: C_INT, C_INT32_T, C_PTR, C_DOUBLE
+            TYPE(C_PTR), VALUE :: ctx
+            REAL(C_DOUBLE), INTENT(IN) :: params(5)
+            INTEGER(C_INT32_T), INTENT(OUT) :: excluded
+        END FUNCTION
+
+        INTEGER(C_INT) FUNCTION cpfem_calib_run(ctx, params, target, comp, curve, loss, grad, stats) &
+                BIND(C, name='cpfem_calib_run')
+            IMPORT :: C_INT, C_INT32_T, C_PTR, C_DOUBLE
+            TYPE(C_PTR), VALUE :: ctx
+            REAL(C_DOUBLE), INTENT(IN) :: params(5), target(*)
+            INTEGER(C_INT32_T), VALUE :: comp
+            REAL(C_DOUBLE), INTENT(OUT) :: curve(*), loss, grad(5)
+            TYPE(C_PTR), VALUE :: stats
+        END FUNCTION
+
+        INTEGER(C_INT) FUNCTION cpfem_calib_end(ctx) BIND(C, name='cpfem_calib_end')
+            IMPORT :: C_INT, C_PTR
+            TYPE(C_PTR), VALUE :: ctx
+        END FUNCTION
     END INTERFACE
 END MODULE CPFEM_CAPI

@@ -23,7 +23,7 @@ from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
 O64 = Oracle(64)
-RCTL = Control(toler=1e-10, maxit=200)   # the replay's tolerance (record_path's default)
+RCTL = Control(toler=1e-10, maxit=60)   # the replay's settings (record_path's defaults)
 
 
 @pytest.fixture(scope="module")
