@@ -234,13 +234,20 @@ def calibration_leg(capi_mod, n, ctl, cpu=True):
     from cpfem_code_amd.problem import fcc_tension
     prob = fcc_tension(n, ninc=50, seed=4, control=ctl)
     t0 = time.time()
-    path = calibrate.record_path(prob, ninc=50, comp=2, ckpt_every=10)
+    path = calibrate.record_path(prob, ninc=50, comp=2, ckpt_every=10, equilibrium=True)
     t_record = time.time() - t0
     p0 = calibrate.material_params(prob.materials[0])
     c_all = path.solver.calib_run(p0, np.zeros(50), grad=False)[0]
     fe_rel = float(np.abs(c_all - path.fe_curve).max() / np.abs(path.fe_curve).max())
     nx = calibrate.exclude_nonconverging(path, p0, rel=1e-6)
     target, p_t = calibrate.seeded_target(path, p0, seed=4)
+    # gradient of the finite-element path: replay adjoint plus the equilibrium correction
+    calibrate.loss_and_grad(path, p0, target)          # warm-up
+    t0 = time.time()
+    _, g_eq, _, st_eq = calibrate.loss_and_grad(path, p0, target)
+    t_eq = time.time() - t0
+    # gradient of the replay (kinematics held fixed), checked against its central differences
+    path.solver.calib_options(0)
     calibrate.loss_and_grad(path, p0, target)          # warm-up
     t0 = time.time()
     loss, g, curve, st = calibrate.loss_and_grad(path, p0, target)
@@ -259,6 +266,12 @@ def calibration_leg(capi_mod, n, ctl, cpu=True):
         "record_s": t_record, "replay_forward_s": t_fwd, "loss_and_gradient_s": t_grad,
         "forward_ms_in_gradient": st["t_forward_ms"], "reverse_ms_in_gradient": st["t_reverse_ms"],
         "central_differences_s": t_fd, "loss": loss, "gradient": g.tolist(), "central_differences": fd.tolist(),
+        "equilibrium_corrected": {
+            "gradient": g_eq.tolist(), "loss_and_gradient_s": t_eq, "reverse_ms": st_eq["t_reverse_ms"],
+            "solves": st_eq["equilibrium_solves"], "sor_sweeps": st_eq["solve_sweeps"],
+            "note": "adds the response of the converged displacement increments (one solve with each increment's "
+                    "recorded Newton matrix); the gradient of the finite-element path itself, checked against "
+                    "finite-element central differences in tests/test_gpu_calib.py"},
         "max_rel_diff_vs_central_differences": float(np.max(np.abs(g - fd) / np.abs(fd))),
         "replay_vs_finite_element_curve_max_rel": fe_rel,
         "checkpoint_every": 10,

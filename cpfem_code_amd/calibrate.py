@@ -8,16 +8,19 @@ parameters through the constitutive updates alone: each point replays its
 recorded strain and rotation increments (cpfem_calib_run), and the loss
     L(p) = sum_n (curve_n(p) - target_n)^2,   p = (tau0, taus, h0, q, m)
 is differentiated by the hand-written adjoint of the constitutive update
-(reverse sweep with checkpoints).  The recorded kinematics are held fixed, so
-the gradient is exact for the replay; the equilibrium correction a full
-finite-element adjoint would add (the response of the displacement field to
-the parameters) is not included.
+(reverse sweep with checkpoints).  With the recorded kinematics held fixed
+the gradient is exact for the replay; with ``equilibrium=True`` the record
+also keeps every increment's coordinates and Newton matrix, and the reverse
+sweep adds the response of the converged displacement increments (one solve
+per increment), which brings the gradient to that of the finite-element path
+itself (within 1-3 % at 6^3, where the replay-only gradient is 30 % low).
 
 ``CalibLoss`` wraps the replay and its adjoint in a torch.autograd.Function,
 so that torch optimisers can drive a fit; torch is used only here.
 """
 from __future__ import annotations
 
+import copy
 import dataclasses
 from typing import Optional
 
@@ -45,7 +48,8 @@ class RecordedPath:
 
 
 def record_path(prob: Problem, ninc: Optional[int] = None, comp: int = 2, ckpt_every: int = 10,
-                toler: float = 1e-10, maxit: int = 60, precision: int = 0, control=None) -> RecordedPath:
+                toler: float = 1e-10, maxit: int = 60, precision: int = 0, control=None,
+                equilibrium: bool = False) -> RecordedPath:
     """Run ninc increments of prob on the GPU, recording every point's last
     constitutive inputs; returns the solver holding the record.  toler and
     maxit are the replay's CPSTRS settings: converging points need well under
@@ -54,6 +58,8 @@ def record_path(prob: Problem, ninc: Optional[int] = None, comp: int = 2, ckpt_e
     ninc = ninc or prob.ninc1 - 1
     s = capi.Solver(prob, precision=precision, control=control)
     s.calib_begin(ninc, ckpt_every=ckpt_every, toler=toler, maxit=maxit)
+    if equilibrium:
+        s.calib_options(capi.CALIB_EQUILIBRIUM)
     curve, codes = np.zeros(ninc), []
     for inc in range(1, ninc + 1):
         s.update_node()
@@ -115,6 +121,45 @@ def exclude_nonconverging(path: RecordedPath, params, rel: float = 0.0) -> int:
                 q[k] += sgn * rel * abs(p[k])
                 n = path.solver.calib_exclude(q)
     return n
+
+
+def problem_with_params(prob: Problem, params) -> Problem:
+    """prob with material 1's five calibration parameters replaced (every
+    system; tau0 as the initial slip resistance of every point)."""
+    tau0, taus, h0, q, m = (float(v) for v in params)
+    mat = copy.deepcopy(prob.materials[0])
+    mat.ampar = np.array(mat.ampar, np.float64)
+    mat.ampar[:, 0], mat.ampar[:, 1], mat.ampar[:, 4] = m, h0, taus
+    sm = np.full((mat.nss, mat.nss), q)
+    np.fill_diagonal(sm, np.diag(np.asarray(mat.smatrx, np.float64)))
+    mat.smatrx = sm
+    mat.crss = np.full(mat.nss, tau0)
+    sv = np.array(prob.sv, np.float64)
+    sv[:, 3:3 + mat.nss] = tau0
+    return dataclasses.replace(prob, materials=[mat] + list(prob.materials[1:]), sv=sv)
+
+
+def fe_curve(prob: Problem, params, ninc: int, comp: int = 2, mask=None) -> np.ndarray:
+    """The finite-element path itself at params (not the replay): the stress
+    component comp averaged over the points with mask (all if None) with their
+    integration weights, after every increment."""
+    s = capi.Solver(problem_with_params(prob, params))
+    try:
+        s.calib_begin(ninc, ckpt_every=0)
+        c = np.zeros(ninc)
+        for inc in range(1, ninc + 1):
+            s.update_node()
+            s.nriter(inc)
+            s.calib_record(inc)
+            s.update_gptv()
+            s.reorient()
+            w = s.calib_history(inc)["w"]
+            if mask is not None:
+                w = w * mask
+            c[inc - 1] = (w * s.get_state()["st"][:, comp]).sum() / w.sum()
+        return c
+    finally:
+        s.close()
 
 
 def make_torch_loss():

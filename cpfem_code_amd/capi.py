@@ -194,17 +194,21 @@ def _bind_mesh_api(L):
     L.cpfem_calib_run.argtypes = [_CTX, _P, _P, C.c_int32, _P, _P, _P, C.POINTER(CpfemCalibStats)]
     L.cpfem_calib_exclude.argtypes = [_CTX, _P, _P]
     L.cpfem_calib_checkpoints.argtypes = [_CTX, C.c_int32]
+    L.cpfem_calib_options.argtypes = [_CTX, C.c_int32]
     L.cpfem_calib_end.argtypes = [_CTX]
 
 
 class CpfemCalibStats(C.Structure):
     _fields_ = [("points", C.c_int32), ("increments", C.c_int32), ("unconverged", C.c_int32),
-                ("singular", C.c_int32), ("elastic", C.c_int32), ("pad_", C.c_int32),
-                ("t_forward_ms", C.c_double), ("t_reverse_ms", C.c_double)]
+                ("singular", C.c_int32), ("elastic", C.c_int32), ("equilibrium_solves", C.c_int32),
+                ("t_forward_ms", C.c_double), ("t_reverse_ms", C.c_double), ("solve_sweeps", C.c_double)]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad_"}
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
+
+# cpfem_calib_options flags
+CALIB_EQUILIBRIUM = 1
 
 # cpfem_autotune flags
 TUNE_LAUNCH = 1
@@ -422,6 +426,11 @@ class Solver:
         n = np.zeros(1, np.int32)
         _check(self.L.cpfem_calib_exclude(self.h, _ptr(params), _ptr(n)), "cpfem_calib_exclude")
         return int(n[0])
+
+    def calib_options(self, flags):
+        """CALIB_EQUILIBRIUM: keep coordinates and Newton matrices for the
+        equilibrium correction (before the first calib_record)."""
+        _check(self.L.cpfem_calib_options(self.h, int(flags)), "cpfem_calib_options")
 
     def calib_checkpoints(self, ckpt_every):
         """Checkpoint spacing of the reverse sweep (<= 0: keep the whole trajectory)."""

@@ -64,6 +64,10 @@ __host__ __device__ inline D3 dsin(const D3& a) {
     const double c = cos(a.v);
     return D3{sin(a.v), {c * a.d[0], c * a.d[1], c * a.d[2]}};
 }
+__host__ __device__ inline D3 dsqrt(const D3& a) {
+    const double r = sqrt(a.v), h = 0.5 / r;
+    return D3{r, {h * a.d[0], h * a.d[1], h * a.d[2]}};
+}
 __host__ __device__ inline D3 dcos(const D3& a) {
     const double s = -sin(a.v);
     return D3{cos(a.v), {s * a.d[0], s * a.d[1], s * a.d[2]}};
@@ -179,6 +183,33 @@ __host__ __device__ inline void tens_rot_h(const double r[3], const double a[6],
     rm[2][1] = -r[0];
     const double o[3][3] = {{a[0], a[5], a[4]}, {a[5], a[1], a[3]}, {a[4], a[3], a[2]}};
     double w[3][3], nw[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) w[i][j] = o[i][0] * rm[0][j] + o[i][1] * rm[1][j] + o[i][2] * rm[2][j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) nw[i][j] = rm[0][i] * w[0][j] + rm[1][i] * w[1][j] + rm[2][i] * w[2][j];
+    b[0] = nw[0][0];
+    b[1] = nw[1][1];
+    b[2] = nw[2][2];
+    b[3] = nw[1][2];
+    b[4] = nw[2][0];
+    b[5] = nw[0][1];
+}
+
+// TENS_ROT with the rotation carried as dual numbers (derivative in r).
+__host__ __device__ inline void tens_rot_d(const D3 r[3], const double a[6], D3 b[6]) {
+    D3 rm[3][3];
+    const D3 one = dconst(1.0);
+    rm[0][0] = dsqrt(one - r[1] * r[1] - r[2] * r[2]);
+    rm[1][1] = dsqrt(one - r[0] * r[0] - r[2] * r[2]);
+    rm[2][2] = dsqrt(one - r[0] * r[0] - r[1] * r[1]);
+    rm[0][1] = r[2];
+    rm[1][0] = -r[2];
+    rm[0][2] = -r[1];
+    rm[2][0] = r[1];
+    rm[1][2] = r[0];
+    rm[2][1] = -r[0];
+    const double o[3][3] = {{a[0], a[5], a[4]}, {a[5], a[1], a[3]}, {a[4], a[3], a[2]}};
+    D3 w[3][3], nw[3][3];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) w[i][j] = o[i][0] * rm[0][j] + o[i][1] * rm[1][j] + o[i][2] * rm[2][j];
     for (int i = 0; i < 3; ++i)
@@ -326,13 +357,14 @@ struct PointIn {
 
 // One reverse step.  sb (cotangent of the new stress) is consumed; phb and s0b
 // hold the cotangents of the new angles / slip resistances on entry and of the
-// previous ones on exit; stb receives the cotangent of the start stress; pb
-// accumulates the parameters.  Returns 1 if the plastic branch ran, 0 for the
-// elastic branch, -1 on a singular Jacobian.
+// previous ones on exit; stb receives the cotangent of the start stress, deb
+// and rb those of the strain and rotation increments; pb accumulates the
+// parameters.  Returns 1 if the plastic branch ran, 0 for the elastic branch,
+// -1 on a singular Jacobian.
 template <int N, int NM>
 __host__ __device__ int point_reverse(const DevMaterial<N>& M, int nss, double dt, const PointIn<NM>& P,
                                       const double sb_in[6], double phb[3], double s0b[NM], double stb[6],
-                                      double pb[kParams]) {
+                                      double pb[kParams], double deb[6], double rb[3]) {
     // geometry at the angles of the call, with their Euler-angle derivatives
     const double phi[3] = {P.php[0] + P.dg[0], P.php[1] + P.dg[1], P.php[2] + P.dg[2]};
     const D3 p1{phi[0], {1.0, 0.0, 0.0}}, p2{phi[1], {0.0, 1.0, 0.0}}, p3{phi[2], {0.0, 0.0, 1.0}};
@@ -344,7 +376,8 @@ __host__ __device__ int point_reverse(const DevMaterial<N>& M, int nss, double d
     double sinit[6];
     for (int i = 0; i < 6; ++i) sinit[i] = t[i] + P.dsg[i];
     const bool plastic = P.iel > 0 && fj2vom_h(true, sinit) > 0.0 && fj2vom_h(false, P.de) > 0.0;
-    for (int i = 0; i < 6; ++i) stb[i] = 0.0;
+    for (int i = 0; i < 6; ++i) stb[i] = deb[i] = 0.0;
+    for (int i = 0; i < 3; ++i) rb[i] = 0.0;
     double phg[3] = {0.0, 0.0, 0.0};   // cotangent of the angles of the call
     if (!plastic) {
         // s = S^-1 de: d s / d phi_k = -S^-1 (dS/dphi_k) s
@@ -354,6 +387,7 @@ __host__ __device__ int point_reverse(const DevMaterial<N>& M, int nss, double d
             for (int j = 0; j < 6; ++j) Sm[i][j] = S[i][j].v;
         }
         if (solve_transposed<6>(6, Sm, u)) return -1;
+        for (int i = 0; i < 6; ++i) deb[i] = u[i];   // s = S^-1 de
         for (int k = 0; k < 3; ++k) {
             double acc = 0.0;
             for (int i = 0; i < 6; ++i)
@@ -412,6 +446,7 @@ __host__ __device__ int point_reverse(const DevMaterial<N>& M, int nss, double d
             const int i = act[k];
             gb[i] += rot[i][0].v * rtb[0] + rot[i][1].v * rtb[1] + rot[i][2].v * rtb[2];
         }
+        for (int c = 0; c < 3; ++c) rb[c] -= rtb[c];   // rtn = rot gamma - r
     }
     // (2) slip resistances s0' = s1(s0, gamma)
     pull_s1<N, NM>(M, nss, P.s0, gam, H, s0b, s0b_out, pb, gb);
@@ -470,6 +505,19 @@ __host__ __device__ int point_reverse(const DevMaterial<N>& M, int nss, double d
         pb[4] += lam[6 + ii] * log(fabs(gam[i] / dt));
     }
     pull_s1<N, NM>(M, nss, P.s0, gam, H, cot, s0b_out, pb, nullptr);
+    // strain increment: E1 = de - ...
+    for (int i = 0; i < 6; ++i) deb[i] = -lam[i];
+    // rotation increment through t = TENS_ROT(r, st)
+    {
+        const D3 rd[3] = {D3{P.r[0], {1.0, 0.0, 0.0}}, D3{P.r[1], {0.0, 1.0, 0.0}}, D3{P.r[2], {0.0, 0.0, 1.0}}};
+        D3 td[6];
+        tens_rot_d(rd, P.st, td);
+        for (int k = 0; k < 3; ++k) {
+            double acc = 0.0;
+            for (int c = 0; c < 6; ++c) acc += tb[c] * td[c].d[k];
+            rb[k] += acc;
+        }
+    }
     // (5) start stress through t = TENS_ROT(r, st)
     for (int j = 0; j < 6; ++j) {
         double e[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, o[6];

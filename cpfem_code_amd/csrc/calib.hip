@@ -34,9 +34,10 @@ namespace cpfem {
 namespace {
 
 constexpr int kCalBlock = 256;
-// fields of one increment's record: de 6, r 3, dg 3, dsg 6, w 1
-constexpr int kRec = 19;
-constexpr int kRecDe = 0, kRecR = 6, kRecDg = 9, kRecDs = 12, kRecW = 18;
+// fields of one increment's record: de 6, r 3, dg 3, dsg 6, curve weight 1
+// (zero for excluded points), integration weight 1
+constexpr int kRec = 20;
+constexpr int kRecDe = 0, kRecR = 6, kRecDg = 9, kRecDs = 12, kRecW = 18, kRecWi = 19;
 constexpr int kAdjMaxSS = 24;   // slip systems the reverse kernel is compiled for
 
 struct Calib {
@@ -46,6 +47,7 @@ struct Calib {
     double* hist = nullptr;      // [ninc][kRec][n]
     std::vector<double> wsum;    // [ninc] total weight per increment
     std::vector<int> have;       // [ninc] 1 once recorded
+    std::vector<int> have_eq;    // [ninc] 1 once recorded with coordinates and matrix
     double *st0 = nullptr, *ph0 = nullptr;   // initial stress [6][n], angles [3][n]
     // replay point state, in the layout of the constitutive kernels
     double *st = nullptr, *ds = nullptr, *sv = nullptr, *dsv = nullptr, *t = nullptr, *chi = nullptr;
@@ -61,6 +63,16 @@ struct Calib {
     int* counters = nullptr;     // [4] device: unconverged, singular, elastic
     int* bad = nullptr;          // [n] 1: the replay stopped at maxit in some increment
     std::vector<int> excluded;   // [n] host: points given zero weight so far
+    // equilibrium correction: coordinates and Newton matrix of every increment
+    int equil = 0;
+    double* xhist = nullptr;     // [ninc][3*nnod]
+    double* vhist = nullptr;     // [ninc][nvals]
+    double* kcot = nullptr;      // [9][n] cotangents of de, r
+    double* elam = nullptr;      // [6][n] strain increments of the multiplier field
+    double* rlam = nullptr;      // [3][n]
+    double* avec = nullptr;      // [3*nnod]
+    long long solve_sweeps = 0;
+    int solves = 0;
     double* partial = nullptr;   // [nblocks][4]
     std::vector<double> hpartial;
     int nblocks = 0;
@@ -203,7 +215,8 @@ __global__ __launch_bounds__(64) void k_adjoint(int n, int nss, double dt, int c
                                                 const double* __restrict__ state, const double* __restrict__ rec,
                                                 const double* __restrict__ out, double* __restrict__ stb,
                                                 double* __restrict__ phb, double* __restrict__ s0b,
-                                                double* __restrict__ pb, int* __restrict__ counters) {
+                                                double* __restrict__ pb, int* __restrict__ counters, int dry,
+                                                const double* __restrict__ corr, double* __restrict__ kcot) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const DevMaterial<N>& M = Mp[0];
@@ -219,13 +232,22 @@ __global__ __launch_bounds__(64) void k_adjoint(int n, int nss, double dt, int c
     for (int i = 0; i < 6; ++i) P.so[i] = out[i * Nn + g];
     for (int i = 0; i < nss; ++i) P.srate[i] = out[(6 + i) * Nn + g];
     P.iel = elem_le[gp_elem[g]];
-    double sb[6], ph[3], s0[NM], stn[6], p[adj::kParams];
+    double sb[6], ph[3], s0[NM], stn[6], p[adj::kParams], deb[6], rb[3];
     for (int i = 0; i < 6; ++i) sb[i] = stb[i * Nn + g];
     sb[comp] += coef * rec[kRecW * Nn + g];
+    if (corr) {
+        const double wi = rec[kRecWi * Nn + g];
+        for (int i = 0; i < 6; ++i) sb[i] -= wi * corr[i * Nn + g];
+    }
     for (int i = 0; i < 3; ++i) ph[i] = phb[i * Nn + g];
     for (int i = 0; i < nss; ++i) s0[i] = s0b[i * Nn + g];
     for (int i = 0; i < adj::kParams; ++i) p[i] = 0.0;
-    const int rc = adj::point_reverse<N, NM>(M, nss, dt, P, sb, ph, s0, stn, p);
+    const int rc = adj::point_reverse<N, NM>(M, nss, dt, P, sb, ph, s0, stn, p, deb, rb);
+    if (kcot) {
+        for (int i = 0; i < 6; ++i) kcot[i * Nn + g] = deb[i];
+        for (int i = 0; i < 3; ++i) kcot[(6 + i) * Nn + g] = rb[i];
+    }
+    if (dry) return;
     if (rc < 0) atomicAdd(&counters[1], 1);
     if (rc == 0) atomicAdd(&counters[2], 1);
     for (int i = 0; i < 6; ++i) stb[i * Nn + g] = stn[i];
@@ -357,20 +379,50 @@ int replay_step(Ctx* c, Calib* K, int n, int comp, double* out, double sums[4], 
 }
 
 template <int N, int NM>
-void launch_adjoint(Ctx* c, Calib* K, int n, int comp, double coef, const double* state, const double* out) {
+void launch_adjoint(Ctx* c, Calib* K, int n, int comp, double coef, const double* state, const double* out, int dry,
+                    const double* corr, double* kcot) {
     const double* rec = K->hist + (size_t)(n - 1) * kRec * K->n;
     hipLaunchKernelGGL((k_adjoint<N, NM>), dim3((K->n + 63) / 64), dim3(64), 0, c->stream, K->n, K->nss,
                        c->tiem[n - 1], comp, coef, (const DevMaterial<N>*)K->dmat, (const int*)c->gp_elem,
-                       (const int*)c->elem_le, state, rec, out, K->stb, K->phb, K->s0b, K->pb, K->counters);
+                       (const int*)c->elem_le, state, rec, out, K->stb, K->phb, K->s0b, K->pb, K->counters, dry,
+                       corr, kcot);
 }
 
-void adjoint_step(Ctx* c, Calib* K, int n, int comp, double coef, const double* state, const double* out) {
+void launch_adjoint_any(Ctx* c, Calib* K, int n, int comp, double coef, const double* state, const double* out,
+                        int dry, const double* corr, double* kcot) {
     if (c->knss == 12)
-        launch_adjoint<12, 12>(c, K, n, comp, coef, state, out);
+        launch_adjoint<12, 12>(c, K, n, comp, coef, state, out, dry, corr, kcot);
     else if (c->knss == 18)
-        launch_adjoint<18, 18>(c, K, n, comp, coef, state, out);
+        launch_adjoint<18, 18>(c, K, n, comp, coef, state, out, dry, corr, kcot);
     else
-        launch_adjoint<kNGeneric, kAdjMaxSS>(c, K, n, comp, coef, state, out);
+        launch_adjoint<kNGeneric, kAdjMaxSS>(c, K, n, comp, coef, state, out, dry, corr, kcot);
+}
+
+// Reverse step of increment n.  With the equilibrium correction: a dry pass
+// gives the cotangents of every point's de and r, their transpose through the
+// kinematics is the right-hand side a of K lambda = a with the increment's
+// recorded Newton matrix K (symmetric; the stiffness NRITER solves with,
+// Stif_sol.f90:42-53), and the stress cotangents get -w B lambda: the
+// response of the converged displacement increment to the parameters and to
+// the state it started from.
+int adjoint_step(Ctx* c, Calib* K, int n, int comp, double coef, const double* state, const double* out) {
+    if (!K->equil) {
+        launch_adjoint_any(c, K, n, comp, coef, state, out, 0, nullptr, nullptr);
+        return 0;
+    }
+    const size_t D = 3 * (size_t)c->nnod;
+    const double* xn = K->xhist + (size_t)(n - 1) * D;
+    launch_adjoint_any(c, K, n, comp, coef, state, out, 1, nullptr, K->kcot);
+    CPFEM_HIP(kin_transpose(c, xn, K->kcot, K->avec));
+    int sweeps = 0;
+    double err = 0.0;
+    const int rc = solve_with(c, K->vhist + (size_t)(n - 1) * c->A.nvals(), K->avec, &sweeps, &err);
+    if (rc < 0 || (rc > 0 && rc != 904)) return rc;
+    K->solve_sweeps += sweeps;
+    K->solves += 1;
+    CPFEM_HIP(kin_apply(c, xn, c->ddx, K->elam, K->rlam));
+    launch_adjoint_any(c, K, n, comp, coef, state, out, 0, K->elam, nullptr);
+    return 0;
 }
 
 }  // namespace
@@ -412,6 +464,7 @@ extern "C" int cpfem_calib_begin(cpfem_ctx* c, int32_t ninc, int32_t ckpt_every,
     K->maxit = maxit;
     K->wsum.assign(ninc, 0.0);
     K->have.assign(ninc, 0);
+    K->have_eq.assign(ninc, 0);
     K->mat_bytes = material_bytes(c->knss);
     K->nblocks = nb_of(K->n);
     K->hpartial.assign((size_t)4 * K->nblocks, 0.0);
@@ -480,6 +533,15 @@ extern "C" int cpfem_calib_record(cpfem_ctx* c, int32_t inc) {
                        (const double*)c->dr, (const double*)c->dsv, (const double*)c->ds, rec);
     CPFEM_HIP(hipGetLastError());
     CPFEM_HIP(point_weights(c, rec + kRecW * n));
+    CPFEM_HIP(hipMemcpyAsync(rec + kRecWi * n, rec + kRecW * n, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    K->have_eq[inc - 1] = K->equil;
+    if (K->equil) {
+        const size_t D = 3 * (size_t)c->nnod;
+        CPFEM_HIP(hipMemcpyAsync(K->xhist + (size_t)(inc - 1) * D, c->x, sizeof(double) * D, hipMemcpyDeviceToDevice,
+                                 c->stream));
+        CPFEM_HIP(hipMemcpyAsync(K->vhist + (size_t)(inc - 1) * c->A.nvals(), c->A.vals,
+                                 sizeof(double) * c->A.nvals(), hipMemcpyDeviceToDevice, c->stream));
+    }
     hipLaunchKernelGGL(k_wsum, dim3(nb), dim3(kCalBlock), 0, c->stream, K->n, (const double*)(rec + kRecW * n),
                        K->partial);
     CPFEM_HIP(hipGetLastError());
@@ -500,7 +562,7 @@ extern "C" int cpfem_calib_history(cpfem_ctx* c, int32_t inc, double* rec_out) {
     CPFEM_HIP(hipMemcpy(f.data(), K->hist + (size_t)(inc - 1) * kRec * n, sizeof(double) * f.size(),
                         hipMemcpyDeviceToHost));
     for (size_t g = 0; g < n; ++g)
-        for (int q = 0; q < kRec; ++q) rec_out[g * kRec + q] = f[q * n + g];
+        for (int q = 0; q < 19; ++q) rec_out[g * 19 + q] = f[q * n + g];
     return 0;
 }
 
@@ -509,7 +571,7 @@ extern "C" int cpfem_calib_run(cpfem_ctx* c, const double* params, const double*
     if (!c || !c->calib || !params || !target || comp < 0 || comp > 5) return CPFEM_E_ARG;
     Calib* K = (Calib*)c->calib;
     for (int i = 0; i < K->ninc; ++i)
-        if (!K->have[i]) return CPFEM_E_ARG;
+        if (!K->have[i] || (K->equil && grad && !K->have_eq[i])) return CPFEM_E_ARG;
     if (!(params[0] > 0.0) || !(params[4] > 0.0)) return CPFEM_E_ARG;
     int rc = set_params(c, K, params);
     if (rc) return rc;
@@ -518,6 +580,8 @@ extern "C" int cpfem_calib_run(cpfem_ctx* c, const double* params, const double*
     const size_t n = (size_t)K->n;
     const size_t fs = (size_t)(6 + K->nfs) * n, fo = (size_t)(6 + K->nss) * n;
     CPFEM_HIP(hipMemsetAsync(K->counters, 0, 4 * sizeof(int), s));
+    K->solves = 0;
+    K->solve_sweeps = 0;
     hipEvent_t e0, e1, e2;
     CPFEM_HIP(hipEventCreate(&e0));
     CPFEM_HIP(hipEventCreate(&e1));
@@ -575,8 +639,9 @@ extern "C" int cpfem_calib_run(cpfem_ctx* c, const double* params, const double*
                 if ((rc = replay_step(c, K, inc, comp, K->seg_out + (size_t)(inc - n0) * fo, sums))) return rc;
             }
             for (int inc = n1; inc >= n0; --inc)
-                adjoint_step(c, K, inc, comp, coef[inc - 1], K->seg_state + (size_t)(inc - n0) * fs,
-                             K->seg_out + (size_t)(inc - n0) * fo);
+                if ((rc = adjoint_step(c, K, inc, comp, coef[inc - 1], K->seg_state + (size_t)(inc - n0) * fs,
+                                       K->seg_out + (size_t)(inc - n0) * fo)))
+                    return rc;
             CPFEM_HIP(hipGetLastError());
         }
         double g0[4], g1[4];
@@ -603,6 +668,8 @@ extern "C" int cpfem_calib_run(cpfem_ctx* c, const double* params, const double*
         stats->unconverged = (int32_t)unconv;
         stats->singular = cnt[1];
         stats->elastic = cnt[2];
+        stats->equilibrium_solves = K->solves;
+        stats->solve_sweeps = (double)K->solve_sweeps;
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
         stats->t_forward_ms = ms;
@@ -653,6 +720,34 @@ extern "C" int cpfem_calib_exclude(cpfem_ctx* c, const double* params, int32_t* 
         cnt += K->excluded[g];
     }
     if (excluded) *excluded = cnt;
+    return 0;
+}
+
+extern "C" int cpfem_calib_options(cpfem_ctx* c, int32_t flags) {
+    if (!c || !c->calib) return CPFEM_E_ARG;
+    Calib* K = (Calib*)c->calib;
+    if (!(flags & CPFEM_CALIB_EQUILIBRIUM)) {
+        K->equil = 0;
+        return 0;
+    }
+    if (c->precision != 0 || c->npts8 != c->ngps) return CPFEM_E_ARG;
+    // switching the correction on: before the first record, or on a record
+    // whose increments all kept their coordinates and matrices
+    for (int i = 0; i < K->ninc; ++i)
+        if (K->have[i] && !K->have_eq[i]) return CPFEM_E_ARG;
+    const size_t D = 3 * (size_t)c->nnod, n = (size_t)K->n;
+    hipError_t e = hipSuccess;
+    if (!K->xhist) {
+        e = calloc_dev(K, &K->xhist, (size_t)K->ninc * D, c->stream);
+        if (e == hipSuccess) e = calloc_dev(K, &K->vhist, (size_t)K->ninc * c->A.nvals(), c->stream);
+        if (e == hipSuccess) e = calloc_dev(K, &K->kcot, 9 * n, c->stream);
+        if (e == hipSuccess) e = calloc_dev(K, &K->elam, 6 * n, c->stream);
+        if (e == hipSuccess) e = calloc_dev(K, &K->rlam, 3 * n, c->stream);
+        if (e == hipSuccess) e = calloc_dev(K, &K->avec, D, c->stream);
+    }
+    CPFEM_HIP(e);
+    CPFEM_HIP(hipStreamSynchronize(c->stream));
+    K->equil = 1;
     return 0;
 }
 

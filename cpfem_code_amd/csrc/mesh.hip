@@ -499,6 +499,48 @@ __global__ __launch_bounds__(kBlock) void k_vol_sums(const KinArgs K, const doub
     block_partials<4>(v, partial, slot0 + blockIdx.x);
 }
 
+// Transpose of the point kinematics of 8-node bricks (DSTRNS, Sig_eps.f90:40-50):
+// for cotangents deb (engineering shears) and rb of every point's strain and
+// rotation increments, out[3*node+k] += sum over the element's points of
+// sum_i dN_a/dx_i Gbar(i,k), where Gbar is the cotangent of the displacement
+// gradient d(i,k) = sum_a dN_a/dx_i du_k.  One thread per element of one
+// colour (no two share a node).
+__global__ __launch_bounds__(kBlock) void k_kin_transpose8(const KinArgs K, int nel_color, const int* __restrict__ elems,
+                                                           const int* __restrict__ elem_gps,
+                                                           const double* __restrict__ cot, double* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nel_color) return;
+    const int e = elems[t];
+    double xl[8][3], sfd[8][3], ub[8][3];
+    for (int a = 0; a < 8; ++a) {
+        const int nd = K.elem_nodes[kMaxNodes * e + a];
+        for (int k = 0; k < 3; ++k) {
+            xl[a][k] = K.x[3 * nd + k];
+            ub[a][k] = 0.0;
+        }
+    }
+    const size_t n = (size_t)K.n;
+    for (int q = 0; q < 8; ++q) {
+        const int g = elem_gps[8 * e + q];
+        if (g < 0) continue;
+        (void)brick_shape(K.elem_lt[e], K.gp_local[g], xl, sfd);
+        double gb[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+        for (int i = 0; i < 3; ++i) {
+            const int j = (i + 1) % 3, k = (j + 1) % 3;
+            const double deb = cot[i * n + g], dsh = cot[(i + 3) * n + g], rb = cot[(6 + i) * n + g];
+            gb[i][i] += deb;
+            gb[k][j] += dsh + 0.5 * rb;   // del(i+3) = d(k,j) + d(j,k), rl(i) = (d(k,j) - d(j,k))/2
+            gb[j][k] += dsh - 0.5 * rb;
+        }
+        for (int a = 0; a < 8; ++a)
+            for (int k = 0; k < 3; ++k) ub[a][k] += sfd[a][0] * gb[0][k] + sfd[a][1] * gb[1][k] + sfd[a][2] * gb[2][k];
+    }
+    for (int a = 0; a < 8; ++a) {
+        const int nd = K.elem_nodes[kMaxNodes * e + a];
+        for (int k = 0; k < 3; ++k) out[3 * nd + k] += ub[a][k];
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_point_weights(const KinArgs K, double* __restrict__ w) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= K.n) return;
@@ -2247,6 +2289,56 @@ extern "C" int cpfem_volume_average(cpfem_ctx* c, double* out) {
     for (int i = 0; i < 6; ++i) out[7 + i] = vol > 0.0 ? sums[8 + i] / vol : 0.0;
     out[13] = vol > 0.0 ? sums[1] / vol : 0.0;
     return 0;
+}
+
+hipError_t cpfem::kin_transpose(Ctx* c, const double* x, const double* cot, double* out) {
+    KinArgs K;
+    memset(&K, 0, sizeof(K));
+    K.n = c->ngps;
+    K.gp_elem = c->gp_elem;
+    K.gp_local = c->gp_local;
+    K.elem_nodes = c->elem_nodes;
+    K.elem_lt = c->elem_lt;
+    K.elem_nd = c->elem_nd;
+    K.x = x;
+    const size_t D = 3 * (size_t)c->nnod;
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, c->stream, out, D, 0.0);
+    const int ncol_el = ((int)c->ekstart.size() - 1) / kKinds;
+    for (int k = 0; k < ncol_el; ++k) {
+        const int b0 = c->ekstart[kKinds * k + kElem8], cnt = c->ekstart[kKinds * k + kElem8 + 1] - b0;
+        if (cnt == 0) continue;
+        hipLaunchKernelGGL(k_kin_transpose8, dim3(nblk(cnt)), dim3(kBlock), 0, c->stream, K, cnt,
+                           (const int*)(c->elem_by_color + b0), (const int*)c->elem_gps, cot, out);
+    }
+    return hipGetLastError();
+}
+
+hipError_t cpfem::kin_apply(Ctx* c, const double* x, const double* dx, double* de, double* r) {
+    KinArgs K;
+    memset(&K, 0, sizeof(K));
+    K.n = c->ngps;
+    K.gp_elem = c->gp_elem;
+    K.gp_local = c->gp_local;
+    K.elem_nodes = c->elem_nodes;
+    K.elem_lt = c->elem_lt;
+    K.elem_nd = c->elem_nd;
+    K.x = x;
+    K.dx = dx;
+    K.smale = c->ctl.smale;
+    K.smalr = c->ctl.smalr;
+    K.de = de;
+    K.r = r;
+    K.kerr = c->sched.ck_kerr;
+    hipLaunchKernelGGL(k_kin<8>, dim3(nblk(c->ngps)), dim3(kBlock), 0, c->stream, K);
+    return hipGetLastError();
+}
+
+int cpfem::solve_with(Ctx* c, const double* vals, const double* rhs, int* sweeps, double* err) {
+    if (c->precision != 0) return CPFEM_E_ARG;
+    CPFEM_HIP(hipMemcpyAsync(c->A.vals, vals, sizeof(double) * c->A.nvals(), hipMemcpyDeviceToDevice, c->stream));
+    CPFEM_HIP(hipMemcpyAsync(c->rhs, rhs, sizeof(double) * 3 * (size_t)c->nnod, hipMemcpyDeviceToDevice, c->stream));
+    c->assembled = true;
+    return facsls_impl(c, sweeps, err);
 }
 
 hipError_t cpfem::point_weights(Ctx* c, double* w) {

@@ -252,14 +252,15 @@ typedef struct {
     int32_t unconverged;     /* replayed point updates stopped at maxit (forward pass) */
     int32_t singular;        /* reverse steps with a singular Jacobian */
     int32_t elastic;         /* reverse steps on the elastic branch */
-    int32_t pad_;
+    int32_t equilibrium_solves; /* solves of the equilibrium correction */
     double t_forward_ms;     /* forward replay */
     double t_reverse_ms;     /* segment recomputation and reverse steps */
+    double solve_sweeps;     /* SOR sweeps of the equilibrium correction, summed */
 } cpfem_calib_stats;
 int cpfem_calib_begin(cpfem_ctx *ctx, int32_t ninc, int32_t ckpt_every, double toler, int32_t maxit);
 int cpfem_calib_record(cpfem_ctx *ctx, int32_t inc);
 /* Recorded inputs of increment inc per point, rec[ngps][19]: de(6), dr(3),
- * dsv(1:3) (6..), ds(6) (12..), weight (18). */
+ * dsv(1:3) (6..), ds(6) (12..), curve weight (18; zero for excluded points). */
 int cpfem_calib_history(cpfem_ctx *ctx, int32_t inc, double *rec);
 int cpfem_calib_run(cpfem_ctx *ctx, const double *params, const double *target, int32_t comp, double *curve,
                     double *loss, double *grad, cpfem_calib_stats *stats);
@@ -271,6 +272,18 @@ int cpfem_calib_run(cpfem_ctx *ctx, const double *params, const double *target, 
  * Exclusions accumulate over calls (e.g. at each point of a finite-difference
  * stencil); *excluded returns the number of points excluded so far. */
 int cpfem_calib_exclude(cpfem_ctx *ctx, const double *params, int32_t *excluded);
+/* Options of the record, set after cpfem_calib_begin and before the first
+ * cpfem_calib_record.  CPFEM_CALIB_EQUILIBRIUM: also keep every increment's
+ * coordinates and Newton matrix (FP64 contexts with 8-node bricks only), and
+ * add to the gradient the response of the converged displacement increments:
+ * per increment of the reverse sweep, one solve with the recorded matrix
+ * (the symmetric stiffness NRITER uses, Stif_sol.f90:42-53, standing in for
+ * the exact Jacobian of the discrete equilibrium) and a second reverse step
+ * with the corrected stress cotangents.  Memory: 3 nnod + the matrix values
+ * per increment.  Flags 0 switches the correction off (the record is kept);
+ * it can be switched on again on a record made with it. */
+#define CPFEM_CALIB_EQUILIBRIUM 1
+int cpfem_calib_options(cpfem_ctx *ctx, int32_t flags);
 /* Change the checkpoint spacing of the reverse sweep (as ckpt_every of
  * cpfem_calib_begin); the record is kept. */
 int cpfem_calib_checkpoints(cpfem_ctx *ctx, int32_t ckpt_every);

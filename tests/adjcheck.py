@@ -33,7 +33,7 @@ def host_lib():
         raise RuntimeError("tests/native/libadjoint_host.so is not built (run __graft_entry__.build())")
     L = C.CDLL(LIB)
     L.adj_point_reverse_host.restype = C.c_int
-    L.adj_point_reverse_host.argtypes = [C.c_void_p, C.c_double, C.c_int] + [C.c_void_p] * 14
+    L.adj_point_reverse_host.argtypes = [C.c_void_p, C.c_double, C.c_int] + [C.c_void_p] * 16
     return L
 
 
@@ -83,7 +83,7 @@ def adjoint(L, mat, b, dt, cot, ctl=TIGHT, oracle=None):
     n, nss = len(b["iel"]), mat.nss
     ms = capi.material_struct(mat)
     out = dict(st=np.zeros((n, 6)), ph=np.zeros((n, 3)), s0=np.zeros((n, nss)), p=np.zeros((n, 5)),
-               rc=np.zeros(n, int))
+               de=np.zeros((n, 6)), r=np.zeros((n, 3)), rc=np.zeros(n, int))
     for g in range(n):
         f = lambda a: np.ascontiguousarray(a, np.float64)
         st, php, s0 = f(b["st"][g]), f(b["sv"][g, :3]), f(b["sv"][g, 3:3 + nss])
@@ -94,9 +94,12 @@ def adjoint(L, mat, b, dt, cot, ctl=TIGHT, oracle=None):
         s0b = f(cot["s0b"][g]).copy()
         stb = np.zeros(6)
         pb = np.zeros(5)
+        deb, rb = np.zeros(6), np.zeros(3)
         rc = L.adj_point_reverse_host(C.byref(ms), dt, int(b["iel"][g]), _p(st), _p(php), _p(s0), _p(de), _p(r),
-                                      _p(dg), _p(dsg), _p(so), _p(sr), _p(sb), _p(phb), _p(s0b), _p(stb), _p(pb))
+                                      _p(dg), _p(dsg), _p(so), _p(sr), _p(sb), _p(phb), _p(s0b), _p(stb), _p(pb),
+                                      _p(deb), _p(rb))
         out["st"][g], out["ph"][g], out["s0"][g], out["p"][g], out["rc"][g] = stb, phb, s0b, pb, rc
+        out["de"][g], out["r"][g] = deb, rb
     return out
 
 
@@ -104,7 +107,8 @@ def fd_gradients(oracle, mat, b, dt, cot, rel=1e-6, ctl=TIGHT):
     """Central differences of the per-point objective (step rel times the
     input's scale)."""
     n, nss = len(b["iel"]), mat.nss
-    res = dict(st=np.zeros((n, 6)), ph=np.zeros((n, 3)), s0=np.zeros((n, nss)), p=np.zeros((n, 5)))
+    res = dict(st=np.zeros((n, 6)), ph=np.zeros((n, 3)), s0=np.zeros((n, nss)), p=np.zeros((n, 5)),
+               de=np.zeros((n, 6)), r=np.zeros((n, 3)))
 
     def pert(field, cols, k, h):
         bb = {key: np.array(v, copy=True) for key, v in b.items()}
@@ -115,6 +119,14 @@ def fd_gradients(oracle, mat, b, dt, cot, rel=1e-6, ctl=TIGHT):
         h = rel * np.abs(b["st"]).max()
         res["st"][:, k] = (objective(oracle, mat, pert("st", range(6), k, h), dt, cot, ctl)
                            - objective(oracle, mat, pert("st", range(6), k, -h), dt, cot, ctl)) / (2 * h)
+    for k in range(6):
+        h = rel * np.abs(b["de"]).max()
+        res["de"][:, k] = (objective(oracle, mat, pert("de", range(6), k, h), dt, cot, ctl)
+                           - objective(oracle, mat, pert("de", range(6), k, -h), dt, cot, ctl)) / (2 * h)
+    for k in range(3):
+        h = rel * np.abs(b["r"]).max()
+        res["r"][:, k] = (objective(oracle, mat, pert("r", range(3), k, h), dt, cot, ctl)
+                          - objective(oracle, mat, pert("r", range(3), k, -h), dt, cot, ctl)) / (2 * h)
     for k in range(3):
         h = rel
         res["ph"][:, k] = (objective(oracle, mat, pert("sv", range(3), k, h), dt, cot, ctl)
@@ -225,10 +237,11 @@ def adjoint_replay_host(L, oracle, prob, params, hist, target, comp=2, ctl=TIGHT
             sb = f(stb[g]).copy()
             sb[comp] += coef * h["w"][g]
             ph, s0v, stn, p = f(phb[g]).copy(), f(s0b[g]).copy(), np.zeros(6), np.zeros(5)
+            deb, rb = np.zeros(6), np.zeros(3)
             L.adj_point_reverse_host(C.byref(ms), dt, int(iel[g]), _p(f(st0[g])), _p(f(sv0[g, :3])),
                                      _p(f(sv0[g, 3:3 + nss])), _p(f(h["de"][g])), _p(f(h["r"][g])),
                                      _p(f(h["dg"][g])), _p(f(h["ds"][g])), _p(f(so_all[g])), _p(f(sr_all[g])),
-                                     _p(sb), _p(ph), _p(s0v), _p(stn), _p(p))
+                                     _p(sb), _p(ph), _p(s0v), _p(stn), _p(p), _p(deb), _p(rb))
             stb[g], phb[g], s0b[g] = stn, ph, s0v
             pb += p
     grad = np.array([s0b.sum(), pb[1], pb[2], pb[3], pb[4]])

@@ -41,7 +41,8 @@ void fill(DevMaterial<kNGeneric>& d, const cpfem_material& m) {
 extern "C" int adj_point_reverse_host(const cpfem_material* m, double dt, int iel, const double* st,
                                       const double* php, const double* s0, const double* de, const double* r,
                                       const double* dg, const double* dsg, const double* so, const double* srate,
-                                      const double* sb, double* phb, double* s0b, double* stb, double* pb) {
+                                      const double* sb, double* phb, double* s0b, double* stb, double* pb,
+                                      double* deb, double* rb) {
     if (m->nss > kNM) return -2;
     std::unique_ptr<DevMaterial<kNGeneric>> M(new DevMaterial<kNGeneric>());
     fill(*M, *m);
@@ -65,7 +66,7 @@ extern "C" int adj_point_reverse_host(const cpfem_material* m, double dt, int ie
     P.iel = iel;
     double s0bl[kNM];
     for (int i = 0; i < m->nss; ++i) s0bl[i] = s0b[i];
-    const int rc = adj::point_reverse<kNGeneric, kNM>(*M, m->nss, dt, P, sb, phb, s0bl, stb, pb);
+    const int rc = adj::point_reverse<kNGeneric, kNM>(*M, m->nss, dt, P, sb, phb, s0bl, stb, pb, deb, rb);
     for (int i = 0; i < m->nss; ++i) s0b[i] = s0bl[i];
     return rc;
 }
