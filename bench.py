@@ -241,7 +241,11 @@ def calibration_leg(capi_mod, n, ctl, cpu=True):
     fe_rel = float(np.abs(c_all - path.fe_curve).max() / np.abs(path.fe_curve).max())
     nx = calibrate.exclude_nonconverging(path, p0, rel=1e-6)
     target, p_t = calibrate.seeded_target(path, p0, seed=4)
-    # gradient of the finite-element path: replay adjoint plus the equilibrium correction
+    # gradient of the finite-element path: replay adjoint plus the equilibrium correction, its
+    # solves with the relaxation factor tuned once on the last recorded system
+    t0 = time.time()
+    tuned = path.solver.autotune()
+    t_tune = time.time() - t0
     calibrate.loss_and_grad(path, p0, target)          # warm-up
     t0 = time.time()
     _, g_eq, _, st_eq = calibrate.loss_and_grad(path, p0, target)
@@ -269,6 +273,7 @@ def calibration_leg(capi_mod, n, ctl, cpu=True):
         "equilibrium_corrected": {
             "gradient": g_eq.tolist(), "loss_and_gradient_s": t_eq, "reverse_ms": st_eq["t_reverse_ms"],
             "solves": st_eq["equilibrium_solves"], "sor_sweeps": st_eq["solve_sweeps"],
+            "sor_tuning": {"result": tuned, "s": t_tune},
             "note": "adds the response of the converged displacement increments (one solve with each increment's "
                     "recorded Newton matrix); the gradient of the finite-element path itself, checked against "
                     "finite-element central differences in tests/test_gpu_calib.py"},
