@@ -416,7 +416,8 @@ int adjoint_step(Ctx* c, Calib* K, int n, int comp, double coef, const double* s
     CPFEM_HIP(kin_transpose(c, xn, K->kcot, K->avec));
     int sweeps = 0;
     double err = 0.0;
-    const int rc = solve_with(c, K->vhist + (size_t)(n - 1) * c->A.nvals(), K->avec, &sweeps, &err);
+    // consecutive increments have nearby multipliers: start from the last one
+    const int rc = solve_with(c, K->vhist + (size_t)(n - 1) * c->A.nvals(), K->avec, &sweeps, &err, K->solves > 0);
     if (rc < 0 || (rc > 0 && rc != 904)) return rc;
     K->solve_sweeps += sweeps;
     K->solves += 1;

@@ -1845,7 +1845,10 @@ static double residual(Ctx* c, const double* b, const double* y, double* res) {
 static int autotune_impl(Ctx* c, int flags);
 
 // FACSLS: solve A ddx = rhs from a zero start.
-static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out) {
+// warm: start from the ddx in place instead of zero (the reference starts from
+// zero, Stif_sol.f90:179; the calibration's solves of consecutive increments
+// start from the previous solution)
+static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out, bool warm = false) {
     if (c->omega_auto) {
         const int rc = autotune_impl(c, CPFEM_TUNE_LAUNCH | CPFEM_TUNE_OMEGA);
         if (rc) return rc;
@@ -1855,7 +1858,7 @@ static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out) {
     const int D = 3 * c->nnod;
     const int maxit = c->ctl.sor_maxit;
     CPFEM_HIP(hipEventRecord(c->ev[5], s));
-    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, s, c->ddx, (size_t)D, 0.0);
+    if (!warm) hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, s, c->ddx, (size_t)D, 0.0);
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, c->flags + 2, (size_t)2, 0);
     int sweeps = 0, code = 0;
     double err = 0.0;
@@ -2333,12 +2336,12 @@ hipError_t cpfem::kin_apply(Ctx* c, const double* x, const double* dx, double* d
     return hipGetLastError();
 }
 
-int cpfem::solve_with(Ctx* c, const double* vals, const double* rhs, int* sweeps, double* err) {
+int cpfem::solve_with(Ctx* c, const double* vals, const double* rhs, int* sweeps, double* err, bool warm) {
     if (c->precision != 0) return CPFEM_E_ARG;
     CPFEM_HIP(hipMemcpyAsync(c->A.vals, vals, sizeof(double) * c->A.nvals(), hipMemcpyDeviceToDevice, c->stream));
     CPFEM_HIP(hipMemcpyAsync(c->rhs, rhs, sizeof(double) * 3 * (size_t)c->nnod, hipMemcpyDeviceToDevice, c->stream));
     c->assembled = true;
-    return facsls_impl(c, sweeps, err);
+    return facsls_impl(c, sweeps, err, warm);
 }
 
 hipError_t cpfem::point_weights(Ctx* c, double* w) {

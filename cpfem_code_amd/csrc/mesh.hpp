@@ -141,9 +141,9 @@ hipError_t kin_transpose(Ctx* c, const double* x, const double* cot, double* out
 // Point kinematics (DSTRNS) of the nodal vector dx at coordinates x (8-node bricks).
 hipError_t kin_apply(Ctx* c, const double* x, const double* dx, double* de, double* r);
 // FACSLS on the FP64 matrix values vals (layout of A.vals) and right-hand side
-// rhs, with the context's solver settings; the solution is left in ddx and A.vals,
-// rhs are overwritten.
-int solve_with(Ctx* c, const double* vals, const double* rhs, int* sweeps, double* err);
+// rhs, with the context's solver settings, from zero or (warm) from the ddx in
+// place; the solution is left in ddx and A.vals, rhs are overwritten.
+int solve_with(Ctx* c, const double* vals, const double* rhs, int* sweeps, double* err, bool warm);
 // Releases the replay and adjoint state (calib.hip).
 void calib_free(Ctx* c);
 
