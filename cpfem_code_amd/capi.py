@@ -35,7 +35,7 @@ class CpfemControl(C.Structure):
                 ("efix", C.c_double), ("smalp", C.c_double), ("smalf", C.c_double),
                 ("smald", C.c_double), ("smale", C.c_double), ("smalr", C.c_double),
                 ("sor_omega", C.c_double), ("sor_tol", C.c_double),
-                ("sor_kind", C.c_int32), ("precision", C.c_int32)]
+                ("sor_kind", C.c_int32), ("precision", C.c_int32), ("stall_tol", C.c_double)]
 
 
 class FafErr(RuntimeError):
@@ -70,6 +70,7 @@ def control_struct(c, precision: int = 0) -> CpfemControl:
         setattr(s, name, float(getattr(c, name)))
     s.maxit, s.sor_maxit, s.sor_kind = int(c.maxit), int(c.sor_maxit), int(c.sor_kind)
     s.precision = int(precision)
+    s.stall_tol = float(getattr(c, "stall_tol", 0.0))
     return s
 
 

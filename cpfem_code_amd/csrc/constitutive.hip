@@ -310,6 +310,9 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
             double erepmd = 0.0;
             bool done = false, ovf = false;
             while (!done && iter < A.iter_limit) {
+                double s_top[6];   // stress at the top of this outer iteration (stall test)
+#pragma unroll
+                for (int i = 0; i < 6; ++i) s_top[i] = s[i];
                 // ---- SLPINI (crys-cons.f90:143-197) over all systems.  The
                 // loop stays rolled (one power per system) and stages each
                 // system's slip and resolved shear in the LDS slice of the
@@ -441,6 +444,18 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
 #pragma unroll
                 for (int i = 0; i < 6; ++i) s[i] = s[i] + erreps[i];
                 done = !((iter < C.maxit) && (erepmd > C.toler));
+                if (!done && C.stall_tol > 0.0) {
+                    // opt-in: the stress no longer moves (a fixed point of SCALE_IT's
+                    // cap and the Newton step); the remaining iterations would only
+                    // change it at the rounding level
+                    double dmax = 0.0, smax = 0.0;
+#pragma unroll
+                    for (int i = 0; i < 6; ++i) {
+                        if (fabs(s[i] - s_top[i]) > dmax) dmax = fabs(s[i] - s_top[i]);
+                        if (fabs(s[i]) > smax) smax = fabs(s[i]);
+                    }
+                    if (dmax <= C.stall_tol * smax) done = true;
+                }
                 flops += 24 * (int64_t)NS + 90 + 72 * (int64_t)nsa + 36 + 44 + factor_flops(nsa) +
                          6 * solver_flops(nsa) + factor_flops(6) + solver_flops(6) + 28;
             }
@@ -611,6 +626,7 @@ DevControl make_control(const cpfem_control& c) {
     d.afactor = c.afactor;
     d.maxit = c.maxit;
     d.pad_ = 0;
+    d.stall_tol = c.stall_tol;
     return d;
 }
 

@@ -67,6 +67,7 @@ struct DevControl {
     double toler, alpha, afactor;
     int maxit;
     int pad_;
+    double stall_tol;   // 0: to maxit; > 0: stop at a stationary stress (cpfem_control::stall_tol)
 };
 
 __device__ __forceinline__ int64_t factor_flops(int n) {

@@ -29,6 +29,7 @@ MODULE CPFEM_CAPI
         REAL(C_DOUBLE) :: smald, smale, smalr
         REAL(C_DOUBLE) :: sor_omega, sor_tol
         INTEGER(C_INT32_T) :: sor_kind, precision
+        REAL(C_DOUBLE) :: stall_tol
     END TYPE cpfem_control
 
     TYPE, BIND(C) :: cpfem_mesh

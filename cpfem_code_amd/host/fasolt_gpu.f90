@@ -125,6 +125,7 @@ SUBROUTINE FSINIT
     ctl%smald = smald; ctl%smale = smale; ctl%smalr = smalr
     ! the reference's compiled-in solver settings (Stif_sol.f90:173-210)
     ctl%sor_omega = 1.2d0; ctl%sor_kind = 0; ctl%sor_tol = toler; ctl%sor_maxit = 1000; ctl%precision = 0
+    ctl%stall_tol = 0.0d0
     OPEN(UNIT=ichan, FILE=TRIM(dir)//'cpfem.dat', STATUS='OLD', IOSTAT=ierr)
     IF (ierr .EQ. 0) THEN
         READ(ichan, *) om, kind, tol, mx, prec

@@ -84,6 +84,7 @@ class Control:
     sor_maxit: int = 1000        # Stif_sol.f90:173
     sor_tol: float = 1.0e-3      # reference: toler
     sor_kind: int = 0            # 0: reference |dx|^2/|x|^2 < tol^2, 1: |D^-1(b-Ax)|/|D^-1 b| < tol, D = diag(A)
+    stall_tol: float = 0.0       # 0: CPSTRS to maxit as the reference; > 0: stop at a stationary stress
 
 
 @dataclasses.dataclass

@@ -57,6 +57,12 @@ typedef struct {
     double sor_tol;       /* tolerance of the SOR stopping test */
     int32_t sor_kind;     /* 0: |dx|^2/|x|^2 < tol^2 (Stif_sol.f90:208-210); 1: |D^-1(b-Ax)|/|D^-1 b| < tol, D = diag(A) */
     int32_t precision;    /* 0: FP64 throughout; 1: FP32 element products and SOR sweeps, FP64 residuals */
+    double stall_tol;     /* 0: CPSTRS runs to maxit as the reference does (Cpstrs.f90:139).  > 0: a point
+                             also stops when an outer iteration moves its stress by at most stall_tol
+                             relative (max norm) while the Newton correction stays above toler: the stress
+                             has reached a fixed point of SCALE_IT's cap and the Newton step, which later
+                             iterations only move at the rounding level (results then differ from the
+                             reference's maxit-th iterate by about stall_tol / 10) */
 } cpfem_control;
 
 /*

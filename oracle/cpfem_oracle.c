@@ -57,6 +57,7 @@ typedef struct {
 typedef struct {
     real toler, alpha, afactor, efix, smalp, smalf, smald, smale, smalr;
     int maxit;
+    real stall_tol;
 } wctl;
 
 static void load_mat(wmat *w, const orc_material *m) {
@@ -86,6 +87,7 @@ static void load_ctl(wctl *w, const orc_control *c) {
     w->smale = R(c->smale);
     w->smalr = R(c->smalr);
     w->maxit = c->maxit;
+    w->stall_tol = R(c->stall_tol);
 }
 
 /* Scratch of module FAS_CPL (fas-com-mod.f90:62-66) for one point. */
@@ -561,6 +563,8 @@ static real cpstrs(cpwork *w, real dt, real el_comp[6][6], const real *del, cons
     while (l_iter) {
         ++iter;
         if (w->cnt) w->cnt->outer += 1;
+        real s_top[6];
+        for (int i = 0; i < 6; ++i) s_top[i] = s[i];
         for (int i = 0; i < ns; ++i) {
             real sum = R(0.0);
             for (int k = 0; k < 6; ++k) sum = sum + w->b[i][k] * s[k];
@@ -609,6 +613,14 @@ static real cpstrs(cpwork *w, real dt, real el_comp[6][6], const real *del, cons
         for (int i = 0; i < 6; ++i) erreps[i] = scale * erreps[i];
         for (int i = 0; i < 6; ++i) s[i] = s[i] + erreps[i];
         l_iter = (iter < maxcpsit) && (erepmd > tolercp);
+        if (l_iter && w->c->stall_tol > R(0.0)) {
+            real dmax = R(0.0), smax = R(0.0);
+            for (int i = 0; i < 6; ++i) {
+                if (fabs(s[i] - s_top[i]) > dmax) dmax = fabs(s[i] - s_top[i]);
+                if (fabs(s[i]) > smax) smax = fabs(s[i]);
+            }
+            if (dmax <= w->c->stall_tol * smax) l_iter = 0;
+        }
 #ifdef ORC_TRACE
         fprintf(stderr, "outer %d nsa %d erepmd %.3e scale %.3e\n", iter, nsa, (double)erepmd, (double)scale);
 #endif

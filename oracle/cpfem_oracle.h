@@ -49,6 +49,10 @@ typedef struct {
     int32_t keep_going;    /* 0: NRITER stops on a fatal code, as the reference's FAFERR does;
                               1: it continues on 987, 910 and 904 and returns the largest code
                               seen, the policy of the device library */
+    double stall_tol;      /* 0: CPSTRS runs to maxit (the reference).  > 0: the device library's
+                              opt-in stop at a stationary stress (max-norm change of an outer
+                              iteration <= stall_tol relative while erepmd > toler), restated so
+                              the two follow the same iteration path */
 } orc_control;
 
 /* Algorithmic operation counters of one constitutive call. */

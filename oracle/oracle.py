@@ -30,7 +30,7 @@ class OrcControl(C.Structure):
                 ("efix", C.c_double), ("smalp", C.c_double), ("smalf", C.c_double),
                 ("smald", C.c_double), ("smale", C.c_double), ("smalr", C.c_double),
                 ("sor_omega", C.c_double), ("sor_tol", C.c_double),
-                ("sor_kind", C.c_int32), ("keep_going", C.c_int32)]
+                ("sor_kind", C.c_int32), ("keep_going", C.c_int32), ("stall_tol", C.c_double)]
 
 
 class OrcCounters(C.Structure):
@@ -86,6 +86,7 @@ def control_struct(c) -> OrcControl:
     s.sor_maxit = int(c.sor_maxit)
     s.sor_kind = int(c.sor_kind)
     s.keep_going = int(getattr(c, "keep_going", 0))
+    s.stall_tol = float(getattr(c, "stall_tol", 0.0))
     return s
 
 

@@ -161,3 +161,36 @@ def test_doreorient_is_rotation_about_twin_axis():
         # the reference uses u unnormalised; the twin vectors are unit and orthogonal to 3 decimals
         assert np.abs(got - want).max() < 5e-3, j
     del sv0
+
+
+def test_stall_exit_stays_at_the_reference_fixed_point():
+    """The opt-in stop at a stationary stress (stall_tol) against the
+    reference's run to maxit on a recorded tension path: outputs within 1e-12
+    relative (the held points sit at a fixed point of SCALE_IT's cap,
+    Cpstrs.f90:59,75,187-230, reached to rounding), unchanged iteration counts
+    for the points that converge, and less work in total."""
+    import sys
+    sys.path.insert(0, __import__("os").path.dirname(__file__))
+    from adjcheck import point_materials, record_oracle
+    from cpfem_code_amd.problem import MAXSV, Control
+    prob = fcc_tension(2, ninc=8, seed=4)
+    hist = record_oracle(O64, prob, 8)
+    mat, iel, n = prob.materials[0], point_materials(prob), prob.ngps
+    st, sv = np.zeros((n, 6)), np.array(prob.sv, float)
+    flops = [0, 0]
+    for k, h in enumerate(hist):
+        dsv = np.zeros((n, MAXSV))
+        dsv[:, :3] = h["dg"]
+        args = (iel, h["de"], h["r"], st, h["ds"], sv, dsv)
+        a = O64.dstrss_batch([mat], Control(), float(prob.tiem[k]), *args)
+        b = O64.dstrss_batch([mat], Control(stall_tol=1e-14), float(prob.tiem[k]), *args)
+        for key in ("t", "chi", "srate", "dsv"):
+            x, y = a[key].reshape(n, -1), b[key].reshape(n, -1)
+            rel = np.abs(x - y).max(1) / np.maximum(np.abs(x).max(1), 1e-300)
+            assert rel.max() <= 1e-12, (k, key, rel.max())
+        held = a["outer"] >= Control().maxit
+        assert np.array_equal(a["flops"][~held], b["flops"][~held])
+        flops[0] += a["flops"].sum()
+        flops[1] += b["flops"].sum()
+        st, sv[:, :15] = st + a["ds"], sv[:, :15] + a["dsv"][:, :15]
+    assert flops[1] < 0.5 * flops[0], flops
