@@ -280,7 +280,7 @@ def calibration_leg(capi_mod, n, ctl, cpu=True):
         "max_rel_diff_vs_central_differences": float(np.max(np.abs(g - fd) / np.abs(fd))),
         "replay_vs_finite_element_curve_max_rel": fe_rel,
         "checkpoint_every": 10,
-        "note": "replay tolerance 1e-10 (maxit 60); points that stop at the iteration limit at the parameters or "
+        "note": "replay tolerance 1e-10 (maxit 60, stall_tol 1e-14); points that stop at the iteration limit at the parameters or "
                 "on their difference stencil get zero weight; the gradient holds the recorded kinematics fixed",
     }
     # the same gradient with the whole trajectory kept in HBM (no recomputation)
@@ -300,7 +300,7 @@ def calibration_leg(capi_mod, n, ctl, cpu=True):
         hist = [path.solver.calib_history(i) for i in range(1, 51)]
         t0 = time.time()
         k = 5
-        replay_oracle(Oracle(64), prob, p0, hist[:k], ctl=Ctl(toler=1e-10, maxit=60))
+        replay_oracle(Oracle(64), prob, p0, hist[:k], ctl=Ctl(toler=1e-10, maxit=60, stall_tol=1e-14))
         t1 = (time.time() - t0) / k
         out["cpu_baseline_replay"] = {"kind": "port", "cores": 1,
                                       "sample": f"oracle replay of increments 1-{k} of 50 (the yield transition)",

@@ -366,6 +366,9 @@ int replay_step(Ctx* c, Calib* K, int n, int comp, double* out, double sums[4], 
     cpfem_control ctl = c->ctl;
     ctl.toler = K->toler;
     ctl.maxit = K->maxit;
+    // points held by SCALE_IT's cap stop once their stress is stationary; they
+    // still end above toler and are found by cpfem_calib_exclude
+    ctl.stall_tol = 1e-14;
     const DevControl C = make_control(ctl);
     int counts[4] = {0, 0, 0, 0};
     const hipError_t he = run_constitutive(c->knss, K->dmat, C, c->tiem[n - 1], A, c->sched, s, counts);

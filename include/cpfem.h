@@ -236,7 +236,9 @@ int cpfem_autotune(cpfem_ctx *ctx, int32_t flags, double *omega, int32_t *waves_
  * most 24 systems.  toler and maxit are the CPSTRS/SLPFTA tolerance and
  * iteration limit of the replay: the gradient is that of the converged
  * equations, so the replay should converge far below the reference's toler
- * (1e-10 for a 1e-6 finite-difference check).  ckpt_every: increments per
+ * (1e-10 for a 1e-6 finite-difference check); the replay also stops points
+ * whose stress is stationary (stall_tol 1e-14 of cpfem_control), which
+ * still end above toler and are excluded.  ckpt_every: increments per
  * checkpoint of the reverse sweep, whose segments except the last are run
  * forward again from their checkpoints; ckpt_every <= 0 keeps the whole
  * trajectory from the forward pass (ninc * (15 + 2 nss) doubles per point)

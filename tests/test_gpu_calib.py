@@ -23,7 +23,7 @@ from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
 O64 = Oracle(64)
-RCTL = Control(toler=1e-10, maxit=60)   # the replay's settings (record_path's defaults)
+RCTL = Control(toler=1e-10, maxit=60, stall_tol=1e-14)   # the replay's settings (record_path's defaults)
 
 
 @pytest.fixture(scope="module")
