@@ -325,7 +325,7 @@ def main():
     ap.add_argument("--ref-increment", type=int, default=14,
                     help="mesh size n of the whole-increment comparison with the compiled reference program "
                          "(about 30 s of one core at 14); 0 skips it")
-    ap.add_argument("--omega-variants", default="mixed:auto,fp64:auto,mixed:1.9,fp64:1.6+stall,mixed:auto+stall",
+    ap.add_argument("--omega-variants", default="mixed:auto,fp64:auto,mixed:1.9,mixed:auto+stall",
                     help="comma list of path:omega[+stall] (path fp64 or mixed; omega a number, or 'auto' for "
                          "the omega and launch shape tuned at the first solve; +stall: CPSTRS also stops at a "
                          "stationary stress, stall_tol 1e-14) to run as well, reported beside the headline, "
