@@ -285,6 +285,13 @@ def calibration_leg(capi_mod, n, ctl, cpu=True):
     }
     # the same gradient with the whole trajectory kept in HBM (no recomputation)
     path.solver.calib_checkpoints(0)
+    # a fit of the five parameters to the seeded curve (L-BFGS through the torch wrapper)
+    t0 = time.time()
+    x_fit, hist = calibrate.fit(path, p0, target, iterations=20)
+    out["fit"] = {"s": time.time() - t0, "evaluations": len(hist), "loss_start": hist[0], "loss_end": min(hist),
+                  "params": x_fit.tolist(), "target_params": p_t.tolist(),
+                  "note": "torch.optim.LBFGS (strong Wolfe) on the replay loss, gradient from the adjoint, "
+                          "trajectory kept"}
     calibrate.loss_and_grad(path, p0, target)
     t0 = time.time()
     _, g_keep, _, st_keep = calibrate.loss_and_grad(path, p0, target)

@@ -180,3 +180,26 @@ def make_torch_loss():
             return grad_out * g.to(grad_out.device), None, None
 
     return CalibLoss
+
+
+def fit(path: RecordedPath, p_start, target, iterations: int = 20, lr: float = 1.0):
+    """Fit the five parameters to target with torch's L-BFGS on the replay loss
+    and its adjoint gradient (parameters scaled by p_start, so all start at 1).
+    Returns (parameters, loss history per evaluation)."""
+    import torch
+    CalibLoss = make_torch_loss()
+    scale = torch.tensor(np.asarray(p_start, np.float64))
+    x = torch.ones(5, dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.LBFGS([x], lr=lr, max_iter=iterations, tolerance_grad=1e-12, tolerance_change=1e-14,
+                            history_size=10, line_search_fn="strong_wolfe")
+    history = []
+
+    def closure():
+        opt.zero_grad()
+        loss = CalibLoss.apply(x * scale, path, target)
+        loss.backward()
+        history.append(float(loss))
+        return loss
+
+    opt.step(closure)
+    return (x.detach() * scale).numpy(), history

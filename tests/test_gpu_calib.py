@@ -179,3 +179,16 @@ def test_equilibrium_gradient_matches_finite_element_differences():
           f"\ncalib: replay only {g_rp[:4]} (rel {e_rp})", flush=True)
     assert (e_eq < 0.05).all(), e_eq
     assert (e_rp > 2 * e_eq).all(), (e_rp, e_eq)
+
+
+def test_fit_recovers_the_seeded_curve(path_excluded):
+    """L-BFGS over the five parameters, driven by the adjoint gradient through
+    the torch.autograd.Function, brings the replay loss against a curve made
+    at parameters 10 % away down by at least 1e3."""
+    prob, p = path_excluded
+    p0 = calibrate.material_params(prob.materials[0])
+    target, p_t = calibrate.seeded_target(p, p0, seed=4)
+    x, hist = calibrate.fit(p, p0, target, iterations=25)
+    print(f"calib: fit loss {hist[0]:.3e} -> {min(hist):.3e} in {len(hist)} evaluations;"
+          f" parameters {x} (target {p_t})", flush=True)
+    assert min(hist) <= 1e-3 * hist[0]
