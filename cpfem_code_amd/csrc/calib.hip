@@ -93,6 +93,22 @@ hipError_t calloc_dev(Calib* K, T** p, size_t n, hipStream_t s) {
 
 inline int nb_of(size_t n) { return (int)((n + kCalBlock - 1) / kCalBlock); }
 
+// timing events of one call, released on every return path
+struct Events {
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    hipError_t create() {
+        for (auto& x : e) {
+            const hipError_t r = hipEventCreate(&x);
+            if (r != hipSuccess) return r;
+        }
+        return hipSuccess;
+    }
+    ~Events() {
+        for (auto& x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
 // record slot: copies of the inputs of the last constitutive call
 __global__ void k_record(int n, const double* __restrict__ de, const double* __restrict__ dr,
                          const double* __restrict__ dsv, const double* __restrict__ ds, double* __restrict__ rec) {
@@ -586,10 +602,9 @@ extern "C" int cpfem_calib_run(cpfem_ctx* c, const double* params, const double*
     CPFEM_HIP(hipMemsetAsync(K->counters, 0, 4 * sizeof(int), s));
     K->solves = 0;
     K->solve_sweeps = 0;
-    hipEvent_t e0, e1, e2;
-    CPFEM_HIP(hipEventCreate(&e0));
-    CPFEM_HIP(hipEventCreate(&e1));
-    CPFEM_HIP(hipEventCreate(&e2));
+    Events ev;
+    CPFEM_HIP(ev.create());
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1], e2 = ev.e[2];
     CPFEM_HIP(hipEventRecord(e0, s));
     // forward replay with a checkpoint at the start of every segment
     hipLaunchKernelGGL(k_replay_init, dim3(nb), dim3(kCalBlock), 0, s, K->n, K->nss, (const double*)K->st0,
@@ -681,9 +696,6 @@ extern "C" int cpfem_calib_run(cpfem_ctx* c, const double* params, const double*
         if (grad) (void)hipEventElapsedTime(&ms, e1, e2);
         stats->t_reverse_ms = ms;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
     return cnt[1] > 0 ? CPFEM_E_SINGULAR : 0;
 }
 
