@@ -199,6 +199,25 @@ MODULE CPFEM_CAPI
             TYPE(C_PTR), VALUE :: stats
         END FUNCTION
 
+        INTEGER(C_INT) FUNCTION cpfem_calib_options(ctx, flags) BIND(C, name='cpfem_calib_options')
+            IMPORT :: C_INT, C_INT32_T, C_PTR
+            TYPE(C_PTR), VALUE :: ctx
+            INTEGER(C_INT32_T), VALUE :: flags        ! 1: CPFEM_CALIB_EQUILIBRIUM
+        END FUNCTION
+
+        INTEGER(C_INT) FUNCTION cpfem_calib_checkpoints(ctx, ckpt_every) BIND(C, name='cpfem_calib_checkpoints')
+            IMPORT :: C_INT, C_INT32_T, C_PTR
+            TYPE(C_PTR), VALUE :: ctx
+            INTEGER(C_INT32_T), VALUE :: ckpt_every
+        END FUNCTION
+
+        INTEGER(C_INT) FUNCTION cpfem_calib_history(ctx, inc, rec) BIND(C, name='cpfem_calib_history')
+            IMPORT :: C_INT, C_INT32_T, C_PTR, C_DOUBLE
+            TYPE(C_PTR), VALUE :: ctx
+            INTEGER(C_INT32_T), VALUE :: inc
+            REAL(C_DOUBLE), INTENT(OUT) :: rec(19, *)
+        END FUNCTION
+
         INTEGER(C_INT) FUNCTION cpfem_calib_end(ctx) BIND(C, name='cpfem_calib_end')
             IMPORT :: C_INT, C_PTR
             TYPE(C_PTR), VALUE :: ctx
