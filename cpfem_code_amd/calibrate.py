@@ -198,7 +198,7 @@ def fit(path: RecordedPath, p_start, target, iterations: int = 20, lr: float = 1
         opt.zero_grad()
         loss = CalibLoss.apply(x * scale, path, target)
         loss.backward()
-        history.append(float(loss))
+        history.append(float(loss.detach()))
         return loss
 
     opt.step(closure)
