@@ -346,8 +346,8 @@ def main():
                     help="drive each increment through the per-step calls and print every Newton iteration")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md, "
-                                   "not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this library runs on one GPU: the z-slab sharding over several GPUs is a "
+                                   "design, not an implementation, so nothing is launched", "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "no GPU"
