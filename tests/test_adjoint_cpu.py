@@ -115,3 +115,22 @@ def test_replay_adjoint_over_increments_matches_finite_differences():
         cd, _, _ = replay_oracle(O64, prob, dn, hist, ctl=rc)
         fd[k] = (((cu - target) ** 2).sum() - ((cd - target) ** 2).sum()) / (2 * h)
     assert np.all(np.abs(g - fd) <= 1e-5 * np.abs(fd) + 1e-7 * np.abs(fd).max()), (g, fd)
+
+
+def test_problem_with_params_applies_the_five_parameters():
+    """Host logic of the calibration harness: the five parameters land in
+    ampar(1), ampar(2), ampar(5), the off-diagonal latent ratio and the initial
+    slip resistances of every point, and material_params reads them back."""
+    prob = fcc_tension(2, ninc=2, seed=4)
+    p = np.array([17.5, 63.0, 190.0, 1.25, 0.021])
+    q = calibrate.problem_with_params(prob, p)
+    m = q.materials[0]
+    assert np.allclose(calibrate.material_params(m), p)
+    assert np.all(m.ampar[:, 0] == p[4]) and np.all(m.ampar[:, 1] == p[2]) and np.all(m.ampar[:, 4] == p[1])
+    off = ~np.eye(m.nss, dtype=bool)
+    assert np.all(m.smatrx[off] == p[3])
+    assert np.array_equal(np.diag(m.smatrx), np.diag(np.asarray(prob.materials[0].smatrx, float)))
+    assert np.all(q.sv[:, 3:3 + m.nss] == p[0])
+    assert np.array_equal(q.sv[:, :3], prob.sv[:, :3])
+    # the original problem is untouched
+    assert calibrate.material_params(prob.materials[0])[0] == float(prob.materials[0].crss[0])
