@@ -16,6 +16,13 @@
 !   omega  sor_kind(0 update, 1 residual)  sor_tol  sor_maxit  precision(0 FP64, 1 mixed)
 ! omega <= 0 lets the library choose omega and the sweep launch shape on the
 ! first system it solves (cpfem_autotune).
+!
+! Calibration (no counterpart in the reference): with an optional file
+! calib.dat next to fafcdf.dat holding the five parameters (tau0 taus h0 q m),
+! the stress component (1..6) and one target value per increment, every
+! increment's constitutive inputs are recorded (cpfem_calib_record) and after
+! the run the replay loss sum (curve - target)^2 and its adjoint gradient are
+! written to calib.txt (points that do not converge excluded).
 !-----------------------------------------------------------------------
 MODULE HOST_STATE
     USE ISO_C_BINDING
@@ -45,7 +52,10 @@ PROGRAM FASOLT_GPU
     IMPLICIT NONE
     CHARACTER(len=4) :: text
     CHARACTER(len=100) :: f_name
-    INTEGER(4) :: nsave, icurr, isave(1:100), rc, ntw
+    INTEGER(4) :: nsave, icurr, isave(1:100), rc, ntw, ierr, ccomp, nexcl
+    LOGICAL :: calib
+    REAL(C_DOUBLE) :: cpar(5), closs, cgrad(5)
+    REAL(C_DOUBLE), ALLOCATABLE :: ctarget(:), ccurve(:)
     TYPE(cpfem_mesh) :: mesh
 
     CALL FSINIT
@@ -69,11 +79,29 @@ PROGRAM FASOLT_GPU
         READ(*, *) (isave(icurr), icurr = 1, nsave)
     END IF
 
+    ! optional calibration record
+    calib = .FALSE.
+    OPEN(UNIT=ichan, FILE=TRIM(dir)//'calib.dat', STATUS='OLD', IOSTAT=ierr)
+    IF (ierr .EQ. 0) THEN
+        ALLOCATE(ctarget(ninc), ccurve(ninc))
+        READ(ichan, *) cpar
+        READ(ichan, *) ccomp
+        READ(ichan, *) ctarget
+        CLOSE(ichan)
+        rc = cpfem_calib_begin(ctx, ninc, 10, 1.0d-10, 60)
+        calib = rc .EQ. 0
+        IF (.NOT. calib) WRITE(*, '(10X,''calib.dat ignored: cpfem_calib_begin returned '',I4)') rc
+    END IF
+
     OPEN(unit=55, file="twins", action="write", status="replace")
     DO inc = 1, ninc
         rc = cpfem_update_node(ctx)                 ! UPDATE_NODE
         IF (rc .NE. 0) CALL FAFERR(rc)
         CALL NRITER
+        IF (calib) THEN
+            rc = cpfem_calib_record(ctx, inc)
+            IF (rc .NE. 0) CALL FAFERR(rc)
+        END IF
         rc = cpfem_update_gptv(ctx)                 ! UPDATE_GPTV
         IF (rc .NE. 0) CALL FAFERR(rc)
         CALL REORIENT_LOG(ntw)                      ! DoReorient + twins log
@@ -94,6 +122,21 @@ PROGRAM FASOLT_GPU
     CALL PULL_STATE
     CALL WTDATA(TRIM(f_name)//'-'//TRIM(ADJUSTL(text))//'.frs')
     CALL WRITESLIPS(TRIM(f_name)//'-'//TRIM(ADJUSTL(text))//'.slp')
+    IF (calib) THEN
+        rc = cpfem_calib_exclude(ctx, cpar, nexcl)
+        IF (rc .NE. 0) CALL FAFERR(rc)
+        rc = cpfem_calib_run(ctx, cpar, ctarget, ccomp - 1, ccurve, closs, cgrad, C_NULL_PTR)
+        IF (rc .NE. 0) CALL FAFERR(rc)
+        OPEN(UNIT=ochan, FILE=TRIM(dir)//'calib.txt', STATUS='REPLACE')
+        WRITE(ochan, '(A,I8)') 'excluded points ', nexcl
+        WRITE(ochan, '(A,ES24.16)') 'loss ', closs
+        WRITE(ochan, '(A,5ES24.16)') 'gradient ', cgrad
+        DO icurr = 1, ninc
+            WRITE(ochan, '(I6,2ES24.16)') icurr, ccurve(icurr), ctarget(icurr)
+        END DO
+        CLOSE(ochan)
+        rc = cpfem_calib_end(ctx)
+    END IF
     rc = cpfem_destroy(ctx)
 END PROGRAM FASOLT_GPU
 
