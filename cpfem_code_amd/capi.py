@@ -88,6 +88,7 @@ def lib() -> C.CDLL:
                                          _P, _P, _P, _P, _P, _P, _P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P,
                                          _P, _P]
         L.cpfem_device_info.argtypes = [C.c_char_p, C.c_int32, _P]
+        L.cpfem_twin_stream.argtypes = [C.c_void_p, C.c_uint64]
         _lib = L
     return _lib
 
@@ -105,9 +106,17 @@ def device_info():
     return buf.value.decode(), int(cus[0])
 
 
+def twin_stream(epoch: int) -> None:
+    """Epoch of the twin draws of the next dstrss_batch call (each call advances it)."""
+    _check(lib().cpfem_twin_stream(None, C.c_uint64(int(epoch))), "cpfem_twin_stream")
+
+
 def dstrss_batch(material, control, dt, iel, de, r, st, ds, sv, dsv, twinsys=None, timestwinned=None,
-                 raise_on_error=False) -> Dict[str, np.ndarray]:
-    """Batched DSTRSS on the GPU (host arrays in the reference's per-point layout)."""
+                 raise_on_error=False, epoch=None) -> Dict[str, np.ndarray]:
+    """Batched DSTRSS on the GPU (host arrays in the reference's per-point layout).
+    epoch: stream epoch of this call's twin draws (None: the next of the library's counter)."""
+    if epoch is not None:
+        twin_stream(epoch)
     n = len(iel)
     ms = material_struct(material)
     cs = control_struct(control)
@@ -158,7 +167,8 @@ class CpfemStats(C.Structure):
                 ("gp_updates", C.c_int64), ("pts_continued", C.c_int64), ("pts_wide", C.c_int64),
                 ("t_sweep_ms", C.c_double), ("sweep_launches", C.c_int64),
                 ("sor_omega", C.c_double), ("t_tune_ms", C.c_double), ("sor_wpb", C.c_int32),
-                ("tune_sweeps", C.c_int32)]
+                ("tune_sweeps", C.c_int32), ("pts_987", C.c_int64), ("pts_987_last", C.c_int32),
+                ("pad_", C.c_int32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -372,6 +382,10 @@ class Solver:
         out = np.zeros(14)
         _check(self.L.cpfem_volume_average(self.h, _ptr(out)), "cpfem_volume_average")
         return dict(volume=float(out[0]), st=out[1:7].copy(), ep=out[7:13].copy(), eps=float(out[13]))
+
+    def twin_stream(self, epoch):
+        """Epoch of the twin draws of the next ELVALS (each ELVALS advances it)."""
+        _check(self.L.cpfem_twin_stream(self.h, C.c_uint64(int(epoch))), "cpfem_twin_stream")
 
     def set_tflag(self, tflag):
         _check(self.L.cpfem_set_tflag(self.h, int(tflag)), "cpfem_set_tflag")

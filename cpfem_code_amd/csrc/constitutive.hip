@@ -2,6 +2,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
 #include <vector>
 
 #include "common.hpp"
@@ -332,7 +333,7 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
                     if (M.any_twin && fabs(M.ssang[i]) > 1.5) {
                         gi = 0.0;
                         if ((sum > s0i) && (timestwinned != M.twinlim) && (twinsys == 0)) {
-                            if (twin_draw(rng_key) < M.twinprob) {
+                            if (twin_draw(rng_key, iter + 1, i) < M.twinprob) {
                                 tflag = 1;
                                 twinsys = i + 1;
                             }
@@ -926,6 +927,13 @@ void free_schedule(ConstSchedule& S) {
 
 using namespace cpfem;
 
+// Stream epoch of the twin draws of the next cpfem_dstrss_batch call: every
+// call takes its own, so repeated calls draw afresh as the reference's
+// RANDOM_NUMBER stream does (crys-cons.f90:169).
+static std::atomic<uint64_t> g_batch_epoch{1};
+
+void cpfem::set_batch_epoch(uint64_t next) { g_batch_epoch.store(next); }
+
 extern "C" int cpfem_dstrss_batch(const cpfem_material* mat, const cpfem_control* ctl, double dt, int32_t n,
                                   const int32_t* iel, const double* de, const double* r, const double* st,
                                   double* ds, const double* sv, double* dsv, int32_t ldsv, double* t, double* chi,
@@ -1008,7 +1016,7 @@ extern "C" int cpfem_dstrss_batch(const cpfem_material* mat, const cpfem_control
     A.srate = dd.p + (hsr - h.data());
     A.flops = reinterpret_cast<long long*>(dl.p);
     A.transc = reinterpret_cast<long long*>(dl.p + N);
-    A.rng_epoch = 0;
+    A.rng_epoch = g_batch_epoch.fetch_add(1);
     ConstSchedule S;
     memset(&S, 0, sizeof(S));
     hipError_t he = alloc_schedule(S, n);

@@ -272,12 +272,20 @@ __device__ __forceinline__ void schmid(const double sdr[3], const double spn[3],
     b[5] = sdr[0] * spn[1] + sdr[1] * spn[0];
 }
 
-// Splitmix64 draw for the twin test (only consulted when twinprob < 1).
-__device__ __forceinline__ double twin_draw(uint64_t key) {
-    uint64_t z = key + 0x9E3779B97F4A7C15ull;
+// Draw for the twin test of system sys in outer iteration outer (1-based) of
+// a point whose stream key is key = (call epoch << 32) ^ point.  The
+// reference draws RANDOM_NUMBER afresh at every check (crys-cons.f90:168-173);
+// here every check consumes its own counter: the splitmix64 finaliser of the
+// key, offset by the check's counter times the golden-ratio increment, mixed
+// again.  Only consulted when twinprob < 1.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    z = z ^ (z >> 31);
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double twin_draw(uint64_t key, int outer, int sys) {
+    const uint64_t h = mix64(key + 0x9E3779B97F4A7C15ull);
+    const uint64_t z = mix64(h + ((uint64_t)outer * 64ull + (uint64_t)sys + 1ull) * 0x9E3779B97F4A7C15ull);
     return (double)(z >> 11) * (1.0 / 9007199254740992.0);
 }
 

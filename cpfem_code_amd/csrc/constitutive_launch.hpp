@@ -39,5 +39,7 @@ void free_schedule(ConstSchedule& S);
 // after the first pass, counts[1] = points needing the full-width kernel.
 hipError_t run_constitutive(int nss, const void* dmat, const DevControl& C, double dt, const ConstArgs& A,
                             ConstSchedule& S, hipStream_t stream, int counts[4]);
+// Stream epoch of the twin draws of the next cpfem_dstrss_batch call.
+void set_batch_epoch(uint64_t next);
 
 }  // namespace cpfem

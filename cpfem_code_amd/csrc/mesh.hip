@@ -88,14 +88,17 @@ __global__ void k_axpy_nodes(double* __restrict__ y, const double* __restrict__ 
     if (i < n) y[i] = y[i] + a * x[i];
 }
 
-// Sums of erepmd, flops and transcendentals over the points (report values).
+// Sums of erepmd, flops and transcendentals over the points and the count of
+// points whose slip Newton ended with FAFERR 987 (report values).
 __global__ void k_point_sums(const double* __restrict__ erepmd, const long long* __restrict__ flops,
-                             const long long* __restrict__ transc, int n, double* partial) {
+                             const long long* __restrict__ transc, const int* __restrict__ err, int n,
+                             double* partial) {
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         v[0] += erepmd[i];
         v[1] += (double)flops[i];
         v[2] += (double)transc[i];
+        v[3] += err[i] == 987 ? 1.0 : 0.0;
     }
     block_partials<4>(v, partial, blockIdx.x);
 }
@@ -939,6 +942,11 @@ __global__ __launch_bounds__(kBlock) void k_residual(const SorArgs A, const T* _
     block_partials<4>(v, partial, blockIdx.x);
 }
 
+__global__ void k_round_vals(const double* __restrict__ a, float* __restrict__ b, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = (float)__builtin_nontemporal_load(a + i);
+}
+
 __global__ void k_to_float(const double* __restrict__ a, float* __restrict__ b, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (float)a[i];
@@ -1250,6 +1258,7 @@ static int build_structure(Ctx* c, const cpfem_mesh* m, const std::vector<int>& 
     char* vp = nullptr;
     if (dalloc(c, &vp, c->A.nvals() * vbytes)) return CPFEM_E_HIP;
     c->A.vals = vp;
+    if (c->precision == 2 && dalloc(c, &c->A.vals32, c->A.nvals())) return CPFEM_E_HIP;
     CPFEM_HIP(hipMemcpyAsync(c->A.diag_off, diag_off.data(), nnod * sizeof(int), hipMemcpyHostToDevice, c->stream));
     CPFEM_HIP(hipMemcpyAsync(c->A.cols, cols.data(), cols.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     CPFEM_HIP(hipMemcpyAsync(c->A.scatter, scat.data(), scat.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
@@ -1281,7 +1290,8 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
     // sv(1:3) Euler angles, sv(4:3+nss) slip resistances; FS3PRE may declare
     // more state variables (fs3p-gps.f90:30-31), which are carried unchanged
     if (m->nsv < 3 + nss || m->nsv > 63) return CPFEM_E_ARG;
-    if (ctl->precision == 1 && ctl->sor_kind != 1) return CPFEM_E_ARG;
+    if (ctl->precision < 0 || ctl->precision > 2) return CPFEM_E_ARG;
+    if (ctl->precision != 0 && ctl->sor_kind != 1) return CPFEM_E_ARG;
     if (!(ctl->sor_omega < 2.0)) return CPFEM_E_ARG;   // SOR on an SPD system converges for 0 < omega < 2
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CPFEM_E_NOGPU;
@@ -1396,7 +1406,7 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
         TRY(dalloc(c, &c->rhs, D));
         TRY(dalloc(c, &c->ddx, D));
         TRY(dalloc(c, &c->xold, D));
-        if (c->precision == 1) {
+        if (c->precision != 0) {
             TRY(dalloc(c, &c->rhs32, D));
             TRY(dalloc(c, &c->del32, D));
         }
@@ -1675,7 +1685,8 @@ static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
     // reductions for the report
     const int nb = std::min(nblk(c->ngps), c->red.nblocks);
     hipLaunchKernelGGL(k_point_sums, dim3(nb), dim3(kBlock), 0, s, (const double*)c->erepmd,
-                       (const long long*)c->flops, (const long long*)c->transc, c->ngps, c->red.partial);
+                       (const long long*)c->flops, (const long long*)c->transc, (const int*)c->gperr, c->ngps,
+                       c->red.partial);
     double r[4];
     reduce_now(c, nb, r);
     int fl[2];
@@ -1684,6 +1695,8 @@ static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
     c->stats.flops_const += (int64_t)llround(r[1]);
     c->stats.transc_const += (int64_t)llround(r[2]);
     c->stats.gp_updates += c->ngps;
+    c->stats.pts_987 += (int64_t)llround(r[3]);
+    c->stats.pts_987_last = (int32_t)llround(r[3]);
     c->stats.t_const_ms += elapsed(c->ev[0], c->ev[1]);
     c->stats.t_asm_ms += elapsed(c->ev[1], c->ev[2]);
     if (sumcpiter) *sumcpiter = r[0];
@@ -1726,6 +1739,7 @@ static int suvals_impl(Ctx* c, int inc, double* sfc) {
     c->stats.sfc = sqrt(r[0]);
     if (sfc) *sfc = c->stats.sfc;
     c->assembled = true;
+    c->vals32_fresh = false;
     return 0;
 }
 
@@ -1771,6 +1785,22 @@ static int sor_partial_blocks(Ctx* c) {
     return slot;
 }
 
+// Matrix values the FP32 sweeps read: the FP32 matrix itself (precision 1) or
+// the rounded copy of the FP64 matrix (precision 2).
+static const float* sweep_vals32(const Ctx* c) {
+    return c->precision == 2 ? c->A.vals32 : (const float*)c->A.vals;
+}
+
+// Precision 2: round the assembled FP64 matrix (penalties included) into the
+// copy the FP32 sweeps read, once per assembled system.
+static void round_sweep_matrix(Ctx* c) {
+    if (c->precision != 2 || c->vals32_fresh) return;
+    const size_t nv = c->A.nvals();
+    hipLaunchKernelGGL(k_round_vals, dim3((unsigned)std::min<size_t>((nv + kBlock - 1) / kBlock, 65536)),
+                       dim3(kBlock), 0, c->stream, (const double*)c->A.vals, c->A.vals32, nv);
+    c->vals32_fresh = true;
+}
+
 // A batch of sweeps captured once as a graph and replayed (kernel nodes only).
 static hipError_t sweep_graph(Ctx* c, int which, hipGraphExec_t* exec_out) {
     if (c->graphs[which]) {
@@ -1800,7 +1830,7 @@ static hipError_t sweep_graph(Ctx* c, int which, hipGraphExec_t* exec_out) {
         // d to ddx after the last
         (void)D;
         for (int k = 0; k < kSweepBatch; ++k)
-            sor_sweep<float, float, false>(c, (const float*)c->A.vals, c->rhs32, c->del32, c->red.partial);
+            sor_sweep<float, float, false>(c, sweep_vals32(c), c->rhs32, c->del32, c->red.partial);
     }
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(s, &g);
@@ -1849,6 +1879,7 @@ static int autotune_impl(Ctx* c, int flags);
 // zero, Stif_sol.f90:179; the calibration's solves of consecutive increments
 // start from the previous solution)
 static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out, bool warm = false) {
+    round_sweep_matrix(c);
     if (c->omega_auto) {
         const int rc = autotune_impl(c, CPFEM_TUNE_LAUNCH | CPFEM_TUNE_OMEGA);
         if (rc) return rc;
@@ -1890,7 +1921,7 @@ static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out, bool warm = fal
         // residuals is one refinement cycle: the sweeps run in FP32 on the
         // correction equation A d = r from d = 0, r formed in FP64 by the
         // previous residual, then ddx += d.
-        const bool mixed = c->precision == 1;
+        const bool mixed = c->precision != 0;
         double rel = 1.0, prev_rel = -1.0;
         int prev_sweeps = 0, batches = 1;
         if (mixed) CPFEM_HIP(hipMemcpyAsync(c->xold, c->rhs, D * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -1959,7 +1990,7 @@ static void drop_graphs(Ctx* c) {
 }
 
 // Graph the solve replays for this context's settings.
-static int solve_graph_kind(const Ctx* c) { return c->precision == 1 ? 2 : (c->ctl.sor_kind == 0 ? 0 : 1); }
+static int solve_graph_kind(const Ctx* c) { return c->precision != 0 ? 2 : (c->ctl.sor_kind == 0 ? 0 : 1); }
 
 // Device time (ms) of `reps` replays of the solve's sweep graph from a zero start.
 static hipError_t time_sweep_graph(Ctx* c, int reps, float* ms) {
@@ -1994,7 +2025,7 @@ static hipError_t time_sweep_graph(Ctx* c, int reps, float* ms) {
 static hipError_t tune_window(Ctx* c, int W, double* rel) {
     hipStream_t s = c->stream;
     const int D = 3 * c->nnod;
-    const bool mixed = c->precision == 1;
+    const bool mixed = c->precision != 0;
     hipGraphExec_t gx;
     hipError_t e = sweep_graph(c, mixed ? 2 : 1, &gx);
     if (e != hipSuccess) return e;
@@ -2020,6 +2051,7 @@ static double predicted_sweeps(double r1, double r2, int W, double tol) {
 
 static int autotune_impl(Ctx* c, int flags) {
     if (!c->assembled) return CPFEM_E_ARG;
+    round_sweep_matrix(c);
     hipStream_t s = c->stream;
     const int D = 3 * c->nnod;
     const auto t0 = std::chrono::steady_clock::now();
@@ -2053,14 +2085,14 @@ static int autotune_impl(Ctx* c, int flags) {
         static const double ladder[] = {1.0, 1.3, 1.5, 1.65, 1.75, 1.82, 1.87, 1.91, 1.94, 1.96, 1.98};
         // window: an eighth of the sweep limit, 64 to 256 sweeps, whole batches
         const int W = std::max(64, std::min(256, c->ctl.sor_maxit / 8)) / kSweepBatch * kSweepBatch;
-        const double tol = c->precision == 1 ? std::max(c->ctl.sor_tol, 1e-12) : std::max(c->ctl.sor_tol, 1e-14);
+        const double tol = c->precision != 0 ? std::max(c->ctl.sor_tol, 1e-12) : std::max(c->ctl.sor_tol, 1e-14);
         double best_om = c->ctl.sor_omega, best_n = 1e300;
         int since_best = 0;
         for (double om : ladder) {
             c->ctl.sor_omega = om;
             drop_graphs(c);
             hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, s, c->ddx, (size_t)D, 0.0);
-            if (c->precision == 1)
+            if (c->precision != 0)
                 (void)hipMemcpyAsync(c->xold, c->rhs, D * sizeof(double), hipMemcpyDeviceToDevice, s);
             double r1 = 0.0, r2 = 0.0;
             if (fail(tune_window(c, W, &r1))) break;
@@ -2356,6 +2388,14 @@ hipError_t cpfem::point_weights(Ctx* c, double* w) {
     K.x = c->x;
     hipLaunchKernelGGL(k_point_weights, dim3(nblk(c->ngps)), dim3(kBlock), 0, c->stream, K, w);
     return hipGetLastError();
+}
+
+extern "C" int cpfem_twin_stream(cpfem_ctx* c, uint64_t next_epoch) {
+    if (c)
+        c->rng_epoch = next_epoch - 1;   // ELVALS advances it before use
+    else
+        cpfem::set_batch_epoch(next_epoch);
+    return 0;
 }
 
 extern "C" int cpfem_set_tflag(cpfem_ctx* c, int32_t tflag) {

@@ -68,6 +68,7 @@ struct Matrix {
     std::vector<int> gstart;   // [ncolor+1] group range of each colour
     int* cols = nullptr;       // [ngroups][P][64] neighbour node of each slot
     void* vals = nullptr;      // [ngroups][P][9][64] double or float
+    float* vals32 = nullptr;   // precision 2: FP32 copy of the FP64 vals that the sweeps read
     int* diag_off = nullptr;   // [nnod] index of entry 0 of each node's diagonal block
     int* scatter = nullptr;    // per element nd*nd entries from scat_start[e]: index of entry 0 of the
                                // block of node pair (a, b) at a*nd + b
@@ -78,7 +79,7 @@ struct Matrix {
 
 struct Ctx {
     int nlmnt = 0, nnod = 0, ngps = 0, nsv = 0, nss = 0, ninc1 = 0, nmat = 0;   // nss: largest of the materials
-    int precision = 0;
+    int precision = 0;           // cpfem_control::precision (0 FP64, 1 FP32 matrix, 2 FP32 sweeps on an FP64 matrix)
     int knss = 0;                // slip-system count the constitutive kernels are compiled for
     cpfem_control ctl;
     cpfem_material mat0;         // host copy of material 1
@@ -124,6 +125,7 @@ struct Ctx {
     int sor_wpb = 4;             // waves per workgroup of the colour sweeps (cpfem_autotune)
     bool omega_auto = false;     // sor_omega <= 0 at creation: tuned at the first solve
     bool assembled = false;      // A and rhs hold a system (set by SUVALS)
+    bool vals32_fresh = false;   // precision 2: A.vals32 holds the rounded copy of the current A.vals
     // timing and counters of the last calls
     static constexpr int kEvents = 10;   // [7], [8]: around each sweep batch
     hipEvent_t ev[kEvents] = {};

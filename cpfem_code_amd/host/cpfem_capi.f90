@@ -48,6 +48,8 @@ MODULE CPFEM_CAPI
         INTEGER(C_INT64_T) :: sweep_launches
         REAL(C_DOUBLE) :: sor_omega, t_tune_ms
         INTEGER(C_INT32_T) :: sor_wpb, tune_sweeps
+        INTEGER(C_INT64_T) :: pts_987
+        INTEGER(C_INT32_T) :: pts_987_last, pad_
     END TYPE cpfem_stats
 
     INTERFACE

@@ -56,7 +56,12 @@ typedef struct {
     double sor_omega;     /* Stif_sol.f90:173 alphal; in (0, 2), or <= 0: chosen by cpfem_autotune at the first solve */
     double sor_tol;       /* tolerance of the SOR stopping test */
     int32_t sor_kind;     /* 0: |dx|^2/|x|^2 < tol^2 (Stif_sol.f90:208-210); 1: |D^-1(b-Ax)|/|D^-1 b| < tol, D = diag(A) */
-    int32_t precision;    /* 0: FP64 throughout; 1: FP32 element products and SOR sweeps, FP64 residuals */
+    int32_t precision;    /* 0: FP64 throughout.  1: FP32 element products and matrix, FP32 SOR sweeps on the
+                             correction equation, residuals accumulated in FP64 from the FP32 matrix (the
+                             solve converges to the FP32-rounded system's solution).  2: FP64 element
+                             products, matrix and residuals; the SOR sweeps run in FP32 on a rounded copy of
+                             the matrix (iterative refinement of the FP64 system: the solve meets the same
+                             FP64 residual test as precision 0).  1 and 2 need sor_kind 1. */
     double stall_tol;     /* 0: CPSTRS runs to maxit as the reference does (Cpstrs.f90:139).  > 0: a point
                              also stops when an outer iteration moves its stress by at most stall_tol
                              relative (max norm) while the Newton correction stays above toler: the stress
@@ -145,6 +150,10 @@ typedef struct {
     double t_tune_ms;         /* time spent in cpfem_autotune during the call (automatic omega) */
     int32_t sor_wpb;          /* waves per workgroup of the colour-sweep launches */
     int32_t tune_sweeps;      /* sweeps run by cpfem_autotune during the call */
+    int64_t pts_987;          /* point updates whose slip Newton ended above the 0.5 residual (FAFERR 987,
+                                 crys-cons.f90:108), summed over the ELVALS calls */
+    int32_t pts_987_last;     /* such points in the last ELVALS (for NRITER: its final iteration) */
+    int32_t pad_;
 } cpfem_stats;
 
 /* Upload mesh, materials and control; build the block sparsity, colourings
@@ -198,6 +207,16 @@ int cpfem_volume_average(cpfem_ctx *ctx, double *out);
  * raises (crys-cons.f90:171), RDDATA clears (Faslt_io_nowin.f90:133) and the
  * driver tests before DoReorient (Fasolt3.f90:43). */
 int cpfem_set_tflag(cpfem_ctx *ctx, int32_t tflag);
+/* Stream of the twin draws.  SLPINI draws RANDOM_NUMBER afresh at every twin
+ * check with twinprob < 1 (crys-cons.f90:165-173), from the compiler
+ * runtime's stream, which cannot be reproduced.  Here the draw of twin system
+ * i in outer CPSTRS iteration k of point g is a counter-based hash of
+ * (epoch, g, k, i): every check is an independent uniform number.  Each
+ * ELVALS of a context takes the next epoch of the context's counter, each
+ * cpfem_dstrss_batch call the next of a process-wide counter.  This sets the
+ * epoch the next such call uses (ctx NULL: the batch counter), for repeatable
+ * runs; with twinprob = 1 no draw is consulted. */
+int cpfem_twin_stream(cpfem_ctx *ctx, uint64_t next_epoch);
 /* One full increment: UPDATE_NODE, NRITER, UPDATE_GPTV, DoReorient (Fasolt3.f90:38-46). */
 int cpfem_increment(cpfem_ctx *ctx, int32_t inc, cpfem_stats *stats);
 /* Right-hand side and solution of the last solve (3*nnod each, may be NULL). */

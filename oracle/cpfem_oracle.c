@@ -106,6 +106,8 @@ typedef struct {
     int32_t timestwinned;
     int32_t *tflag;
     uint64_t rng_key;
+    int outer;                         /* CPSTRS outer iteration (1-based), the twin draw's counter */
+    int capped;                        /* SCALE_IT scaled in the current outer iteration */
 } cpwork;
 
 static void note_err(int *err, int code) {
@@ -388,14 +390,22 @@ static void plastc(cpwork *w, real dt, real *am) {
 /* crys-cons.f90 slip routines                                              */
 /* ------------------------------------------------------------------------ */
 
-/* Draw for the twin test.  The reference calls RANDOM_NUMBER
- * (crys-cons.f90:169); this counter-based splitmix64 draw is the build's own
- * and only matters when twinprob < 1. */
-static double twin_draw(uint64_t key) {
-    uint64_t z = key + 0x9E3779B97F4A7C15ull;
+/* Draw for the twin test of system sys (0-based) in outer iteration outer
+ * (1-based) of a point with stream key key = (call epoch << 32) ^ point.  The
+ * reference calls RANDOM_NUMBER afresh at every check (crys-cons.f90:168-173),
+ * a compiler-runtime stream that cannot be reproduced; this counter-based draw
+ * is the build's own, shared with the HIP kernel: the splitmix64 finaliser of
+ * the key, offset by the check's counter times the golden-ratio increment,
+ * mixed again, so every check is an independent uniform draw.  Only matters
+ * when twinprob < 1. */
+static uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    z = z ^ (z >> 31);
+    return z ^ (z >> 31);
+}
+static double twin_draw(uint64_t key, int outer, int sys) {
+    const uint64_t h = mix64(key + 0x9E3779B97F4A7C15ull);
+    const uint64_t z = mix64(h + ((uint64_t)outer * 64ull + (uint64_t)sys + 1ull) * 0x9E3779B97F4A7C15ull);
     return (double)(z >> 11) * (1.0 / 9007199254740992.0);
 }
 
@@ -411,7 +421,8 @@ static void slpini(cpwork *w, real dt, real *am) {
         if (fabs(m->ssang[i]) > R(1.5)) {
             w->gam[i] = R(0.0);
             if ((w->tau[i] > w->s0[i]) && (w->timestwinned != m->twinlim) && (*w->twinsys == 0)) {
-                real twinrn = R(twin_draw(w->rng_key));
+                real twinrn = R(twin_draw(w->rng_key, w->outer, i));
+                if (w->cnt) w->cnt->draws += 1;
                 if (twinrn < m->twinprob) {
                     *w->tflag = 1;
                     *w->twinsys = i + 1;
@@ -452,6 +463,7 @@ static void scale_it(cpwork *w, const real *am, real *s, real gamcap, real *scal
         }
     }
     if (gammax > gamcap) {
+        w->capped = 1;
         *scale = gamcap / gammax;
         real smult = spower(R(0.9) * *scale, w->c->afactor * amax);
         if (smult == R(0.0)) smult = R(0.004) / amax;
@@ -562,6 +574,8 @@ static real cpstrs(cpwork *w, real dt, real el_comp[6][6], const real *del, cons
     real erepmd = R(0.0), scale;
     while (l_iter) {
         ++iter;
+        w->outer = iter;
+        w->capped = 0;
         if (w->cnt) w->cnt->outer += 1;
         real s_top[6];
         for (int i = 0; i < 6; ++i) s_top[i] = s[i];
@@ -682,6 +696,7 @@ static int dstrss(const wmat *mats, const wctl *c, real dt, int iel, real *deff,
                 chi[j][i] = d;
             }
         err = w.err;
+        if (cnt) cnt->capped = w.capped;
         count(cnt, 82, 0);
     } else {
         *deff = R(0.0);
@@ -706,6 +721,12 @@ static int dstrss(const wmat *mats, const wctl *c, real dt, int iel, real *deff,
     return err;
 }
 
+/* Stream epoch of the next orc_dstrss_batch call's twin draws (advanced per
+ * call, as the library's batch entry point does). */
+static uint64_t g_batch_epoch = 1;
+void orc_set_batch_epoch(uint64_t epoch) { g_batch_epoch = epoch; }
+double orc_twin_draw(uint64_t key, int outer, int sys) { return twin_draw(key, outer, sys); }
+
 int orc_dstrss_batch(const orc_material *mat, int nmat, const orc_control *ctl, double dt, int n,
                      const int32_t *iel, const double *de, const double *r, const double *st, double *ds,
                      double *sv, double *dsv, int ldsv, double *t, double *chi, double *deff,
@@ -717,6 +738,7 @@ int orc_dstrss_batch(const orc_material *mat, int nmat, const orc_control *ctl, 
     wctl wc;
     load_ctl(&wc, ctl);
     int worst = 0;
+    const uint64_t epoch = g_batch_epoch++;
     for (int g = 0; g < n; ++g) {
         real lde[6], lr[3], lst[6], lds[6], lt[6], lchi[6][6], lsv[ORC_MAXSV], lvd[ORC_MAXSV], lsr[ORC_MAXSS];
         real ldeff = R(0.0);
@@ -735,7 +757,7 @@ int orc_dstrss_batch(const orc_material *mat, int nmat, const orc_control *ctl, 
         orc_counters *c = cnt ? &cnt[g] : NULL;
         if (c) memset(c, 0, sizeof(*c));
         int e = dstrss(wm, &wc, R(dt), iel[g], &ldeff, lde, lst, lds, lt, lr, lchi, lsv, lvd, lsr,
-                       &twinsys[g], timestwinned[g], tflag, sumcpiter, c, (uint64_t)g);
+                       &twinsys[g], timestwinned[g], tflag, sumcpiter, c, (epoch << 32) ^ (uint64_t)g);
         if (err) err[g] = e;
         if (e > worst) worst = e;
         for (int i = 0; i < 6; ++i) {
@@ -1118,6 +1140,7 @@ int orc_elvals(orc_model *m) {
     const real efix = wc.efix, small = R(1.e-7);
     const real dt = R(m->tiem[m->inc - 1]);
     int worst = 0;
+    m->n987 = m->n987_capped = m->nmaxit = 0;
     for (int el = 0; el < m->nlmnt; ++el) {
         int iel = m->le[el], itp = m->lt[el], nn = m->nd[el], ng = m->np[el];
         int in2[60];
@@ -1182,6 +1205,11 @@ int orc_elvals(orc_model *m) {
                 m->mat_code = mc;
                 m->flops += c.flops;
                 m->transc += c.transc;
+                if (e == 987) {
+                    m->n987 += 1;
+                    if (c.capped) m->n987_capped += 1;
+                }
+                if (c.outer >= wc.maxit) m->nmaxit += 1;
                 note_err(&worst, e);
                 for (int k = 0; k < 3 + ns; ++k) m->dsv[(size_t)ORC_MAXSV * j + k] = ldsv[k];
                 for (int k = 0; k < ns; ++k) m->srate[(size_t)ORC_MAXSS * j + k] = lsr[k];

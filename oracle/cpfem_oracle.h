@@ -62,6 +62,8 @@ typedef struct {
     int32_t outer;      /* CPSTRS outer stress iterations */
     int32_t inner;      /* SLPFTA inner iterations, summed over outer */
     int32_t max_nsa;    /* largest active set seen */
+    int32_t draws;      /* twin draws made (checks with twinprob consulted) */
+    int32_t capped;     /* SCALE_IT scaled in the last outer iteration */
     int32_t pad_;
 } orc_counters;
 
@@ -97,6 +99,10 @@ typedef struct {
     int32_t nit;                    /* NR iterations of the last NRITER */
     double sdx, sfc, alp2;          /* last NR report values */
     uint64_t rng_calls;             /* counter of the twin random draws */
+    int32_t n987;                   /* points of the last ELVALS whose slip Newton ended with 987 */
+    int32_t n987_capped;            /* ... of which SCALE_IT held in their last outer iteration */
+    int32_t nmaxit;                 /* points of the last ELVALS that ran CPSTRS to maxit */
+    int32_t pad_;
 } orc_model;
 
 #ifdef __cplusplus
@@ -111,6 +117,11 @@ int orc_dstrss_batch(const orc_material *mat, int nmat, const orc_control *ctl,
                      int32_t *twinsys, const int32_t *timestwinned,
                      int32_t *tflag, double *sumcpiter, orc_counters *cnt,
                      int32_t *err);
+/* Stream epoch of the next batch call's twin draws (each call advances it). */
+void orc_set_batch_epoch(uint64_t epoch);
+/* The twin draw of system sys (0-based) in outer iteration outer (1-based) of
+ * the point stream key = (epoch << 32) ^ point. */
+double orc_twin_draw(uint64_t key, int outer, int sys);
 /* Building blocks exposed for closed-form tests. */
 double orc_spower(double a, double b);
 double orc_fj2vom(int lstress, const double *x);

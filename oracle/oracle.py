@@ -35,7 +35,8 @@ class OrcControl(C.Structure):
 
 class OrcCounters(C.Structure):
     _fields_ = [("flops", C.c_int64), ("transc", C.c_int64), ("outer", C.c_int32), ("inner", C.c_int32),
-                ("max_nsa", C.c_int32), ("pad_", C.c_int32)]
+                ("max_nsa", C.c_int32), ("draws", C.c_int32),
+                ("capped", C.c_int32), ("pad_", C.c_int32)]
 
 
 _P = C.c_void_p
@@ -55,7 +56,8 @@ class OrcModel(C.Structure):
                 ("is_", _P), ("ss", _P),
                 ("sumcpiter", C.c_double), ("flops", C.c_int64), ("transc", C.c_int64),
                 ("sor_sweeps", C.c_int32), ("sor_error", C.c_double), ("nit", C.c_int32),
-                ("sdx", C.c_double), ("sfc", C.c_double), ("alp2", C.c_double), ("rng_calls", C.c_uint64)]
+                ("sdx", C.c_double), ("sfc", C.c_double), ("alp2", C.c_double), ("rng_calls", C.c_uint64),
+                ("n987", C.c_int32), ("n987_capped", C.c_int32), ("nmaxit", C.c_int32), ("pad_", C.c_int32)]
 
 
 def material_struct(m) -> OrcMaterial:
@@ -109,6 +111,9 @@ class Oracle:
         L = self.lib
         L.orc_spower.restype = C.c_double
         L.orc_spower.argtypes = [C.c_double, C.c_double]
+        L.orc_set_batch_epoch.argtypes = [C.c_uint64]
+        L.orc_twin_draw.restype = C.c_double
+        L.orc_twin_draw.argtypes = [C.c_uint64, C.c_int, C.c_int]
         L.orc_fj2vom.restype = C.c_double
         L.orc_fj2vom.argtypes = [C.c_int, _P]
         L.orc_factor.argtypes = [C.c_int, _P]
@@ -194,8 +199,11 @@ class Oracle:
 
     # -- per-point constitutive batch ---------------------------------------
     def dstrss_batch(self, materials, control, dt, iel, de, r, st, ds, sv, dsv, twinsys=None,
-                     timestwinned=None) -> Dict[str, np.ndarray]:
+                     timestwinned=None, epoch=None) -> Dict[str, np.ndarray]:
+        """epoch: stream epoch of this call's twin draws (None: the next of the oracle's own counter)."""
         n = len(iel)
+        if epoch is not None:
+            self.lib.orc_set_batch_epoch(int(epoch))
         mats = (OrcMaterial * len(materials))(*[material_struct(m) for m in materials])
         ctl = control_struct(control)
         iel = np.ascontiguousarray(iel, np.int32)
@@ -225,7 +233,8 @@ class Oracle:
         outer = np.array([c.outer for c in cnt], np.int32)
         inner = np.array([c.inner for c in cnt], np.int32)
         max_nsa = np.array([c.max_nsa for c in cnt], np.int32)
-        return dict(max_nsa=max_nsa, t=t, ds=ds, chi=chi.reshape(n, 6, 6).transpose(0, 2, 1), deff=deff, dsv=dsv, srate=srate,
+        draws = np.array([c.draws for c in cnt], np.int32)
+        return dict(max_nsa=max_nsa, draws=draws, t=t, ds=ds, chi=chi.reshape(n, 6, 6).transpose(0, 2, 1), deff=deff, dsv=dsv, srate=srate,
                     twinsys=tw, tflag=int(tflag[0]), sumcpiter=float(sumcp[0]), err=err, flops=flops,
                     transc=transc, outer=outer, inner=inner)
 

@@ -55,16 +55,16 @@ def test_fcc_fixture_batch():
     compare(fcc_aluminium(), b)
 
 
-def sensitivity(mat, b, dt, delta=1e-13, seed=0):
+def sensitivity(mat, b, dt, delta=1e-13, seed=0, epoch=None):
     """Per-point relative change of the oracle's outputs under a relative input
     perturbation delta of de and st (same iteration path required).  Points whose
     change exceeds 1e4 * delta amplify rounding by more than 1e4."""
     rng = np.random.default_rng(seed)
-    o = O64.dstrss_batch([mat], Control(), dt, **b)
+    o = O64.dstrss_batch([mat], Control(), dt, epoch=epoch, **b)
     p = dict(b)
     p["de"] = b["de"] * (1 + delta * rng.standard_normal(b["de"].shape))
     p["st"] = b["st"] * (1 + delta * rng.standard_normal(b["st"].shape))
-    q = O64.dstrss_batch([mat], Control(), dt, **p)
+    q = O64.dstrss_batch([mat], Control(), dt, epoch=epoch, **p)
     s = np.zeros(len(b["iel"]))
     for k in ("t", "chi", "srate", "dsv"):
         s = np.maximum(s, per_point_rel(q[k], o[k]))
@@ -72,13 +72,14 @@ def sensitivity(mat, b, dt, delta=1e-13, seed=0):
     return s
 
 
-def compare_conditioned(mat, b, dt=1e-3, min_good=0.98):
+def compare_conditioned(mat, b, dt=1e-3, min_good=0.98, epoch=None):
     """1e-10 on every point with rounding amplification below 1e4; the rest within
-    100x their own measured sensitivity."""
+    100x their own measured sensitivity.  epoch: stream epoch of the twin draws
+    (the same on both sides; only consulted with twinprob < 1)."""
     ctl = Control()
-    g = capi.dstrss_batch(mat, ctl, dt, b["iel"], b["de"], b["r"], b["st"], b["ds"], b["sv"], b["dsv"])
-    o = O64.dstrss_batch([mat], ctl, dt, **b)
-    sens = sensitivity(mat, b, dt)
+    g = capi.dstrss_batch(mat, ctl, dt, b["iel"], b["de"], b["r"], b["st"], b["ds"], b["sv"], b["dsv"], epoch=epoch)
+    o = O64.dstrss_batch([mat], ctl, dt, epoch=epoch, **b)
+    sens = sensitivity(mat, b, dt, epoch=epoch)
     nss = mat.nss
     same_path = g["flops"] == o["flops"]
     assert same_path.mean() >= 0.99, same_path.mean()
@@ -114,6 +115,24 @@ def test_hcp_fresh_batches_with_twins(seed):
     b = make_gp_batch(mat, 500, seed=300 + seed, elastic_frac=0.05, stress=30.0)
     g, o = compare_conditioned(mat, b)
     assert g["tflag"] == 1 and (g["twinsys"] > 0).any()
+
+
+@pytest.mark.parametrize("epoch", [5, 6])
+def test_hcp_twinprob_below_one(epoch):
+    """twinprob < 1 (crys-cons.f90:165-173): every twin check draws afresh from
+    the counter stream the kernel shares with the oracle, so which points twin,
+    and on which system, is the same on both sides; the twinned count follows
+    the per-check Bernoulli model (tests/test_twin_draw.py) on the kernel's
+    own draws too."""
+    import dataclasses
+    p = 0.3
+    mat = dataclasses.replace(hcp_magnesium(), twinprob=p, twinlim=1)
+    b = make_gp_batch(mat, 1500, seed=11, stress=60.0)
+    g, o = compare_conditioned(mat, b, epoch=epoch)
+    checks = int(o["draws"].sum())
+    twinned = int((g["twinsys"] != 0).sum())
+    assert 0 < twinned < int((o["draws"] > 0).sum())
+    assert abs(twinned - p * checks) < 4.0 * np.sqrt(checks * p * (1 - p)), (twinned, checks)
 
 
 @pytest.mark.parametrize("name", ["dstrss_bcc.npz", "dstrss_hcp24.npz"])
