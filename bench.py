@@ -2,27 +2,40 @@
 
 Workload (default): FCC aluminium, 64x64x64 8-node bricks with 2x2x2 points
 (262,144 elements, 2,097,152 points, 823,875 DOFs), seeded random texture,
-plane-strain compression -1e-3 per increment, SOR omega 1.6 to a relative
+plane-strain compression -1e-3 per increment, SOR to a relative (Jacobi-scaled)
 residual of 1e-8.  One step = one load increment: UPDATE_NODE, the damped
 Newton loop (point updates, assembly, BCs, SOR solve, update) and the commit,
-with all state resident in HBM.  The FP64 path and the mixed path (FP32
-element products and SOR sweeps, FP64 residual refinement) are both timed;
-``value`` is the FP64 path's point-update rate.
+with all state resident in HBM.
 
-``roofline`` describes the dominant kernel, the FP64 colour-sweep kernel:
-algorithmic bytes per launch (matrix values, columns, row pointers and the
-vectors of one sweep, divided over the colour launches) over its average
-launch time, taken from HIP events recorded on the library's stream around
-every replay of the sweep graphs inside the timed increments.  ``traffic`` is
-the HBM bytes per launch that the FETCH_SIZE and WRITE_SIZE counters measured
-for the same kernel (profiles/pmc_traffic.json, written from rocprofv3 runs of
-cpfem_code_amd/sweep_probe.py), or null when that file is absent.
+Legs, in this order (each bounded by wall time, see --budget):
+  1. FP64 path, SOR relaxation factor and sweep launch shape chosen by the
+     library's autotuner at the first solve (in the untimed warm-up): the
+     headline ``value`` (point updates per second of whole-increment wall time).
+  2. Mixed path (FP32 SOR sweeps on a rounded copy of the FP64 matrix, FP64
+     residuals and element products; precision 2), same settings.
+  3. The same two paths at the configured omega 1.6, one timed increment each.
+  4. One Newton system of the workload solved by each path against a tight
+     FP64 solve: the displacement error of each at the 1e-8 residual.
+  5. cpu_baseline: the FP64 oracle (the CPU restatement of the same algorithm,
+     one core) on whole increments of the CPU-runnable 8^3 uniaxial-tension
+     case, next to the GPU on the same increments.
+Timed steps stop early when a leg's time is spent; ``steps`` is the number of
+increments actually timed on the headline path.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 64]
+``roofline`` describes the kernel with the largest share of the headline
+increments: the constitutive kernels (FP64 vector ALU: algorithmic FLOPs of
+the point updates from the kernels' own counters over their event-timed
+duration per Newton iteration) or the FP64 colour-sweep kernel (algorithmic
+bytes per launch over the event-timed launch duration); ``traffic`` comes from
+the committed counter summary (profiles/pmc_traffic.json).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 64] [--budget S]
+    python bench.py --sweep-probe 64     # fixed-length SOR run for profiler passes
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -34,6 +47,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12          # B/s
 FP64_PEAK = 78.6e12        # FLOP/s, vector
 FP32_PEAK = 157.3e12
+METRIC = "Gauss-pt updates/s + s/increment at 64³ FCC, FP64 vs mixed-prec SOR"
 
 
 def log(msg):
@@ -47,13 +61,14 @@ def sor_bytes(prob, nnzb, s_a, s_x):
 
 def nnzb_structured(dims):
     nx, ny, nz = dims
+
     def f(n):
         return 3 * (n + 1) - 2
     return f(nx) * f(ny) * f(nz)
 
 
 def load_traffic():
-    """HBM bytes per launch of the sweep kernels from the committed counter summary."""
+    """HBM bytes per launch of the profiled kernels from the committed counter summary."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if not os.path.exists(path):
         return None
@@ -62,12 +77,19 @@ def load_traffic():
     return {k: v["hbm_bytes_per_launch"] for k, v in d.get("kernels", {}).items()}
 
 
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
 def traced_increment(sv, inc, ctl):
-    """One increment through the per-step entry points (UPDATE_NODE, then ELVALS,
-    SUVALS, FACSLS and the update per Newton iteration, then the cumulative slip,
-    UPDATE_GPTV and DoReorient), with one progress line per Newton iteration.
-    Same state and stopping test as cpfem_increment; the record is the change
-    of the library's counters over the increment."""
+    """One increment through the per-step entry points with one progress line
+    per Newton iteration (same state and stopping test as cpfem_increment)."""
     before = sv.stats()
     t0 = time.perf_counter()
     sv.update_node()
@@ -90,273 +112,287 @@ def traced_increment(sv, inc, ctl):
     sv.update_gptv()
     sv.reorient()
     after = sv.stats()
-    keep = ("sor_omega", "sor_wpb")
+    keep = ("sor_omega", "sor_wpb", "pts_987_last")
     rec = {k: after[k] - before[k] for k in after if k not in ("nit", "sor_sweeps", "err", "sdx", "sfc", "sumcpiter",
-                                                                "alp2", "sor_error") + keep}
+                                                                "alp2", "sor_error", "pad_") + keep}
     rec.update({k: after[k] for k in keep})
     rec.update(nit=nit, code=code, t_total_ms=1e3 * (time.perf_counter() - t0))
     return rec
 
 
-def run_path(prob, precision, warmup, steps, capi, trace=False):
+def run_path(prob, precision, warmup, steps, warm_s, timed_s, capi, trace=False, label=""):
+    """Warm-up increments (at least one, at most `warmup`, stopping once warm_s
+    is spent), then timed increments (at least one, at most `steps`, stopping
+    once timed_s is spent).  Returns the timed records, their wall time, the
+    warm-up records and the solver's final state (on request)."""
     t0 = time.time()
     sv = capi.Solver(prob, precision=precision)
-    log(f"[bench] precision={precision} setup {time.time() - t0:.1f} s")
-    import torch
+    log(f"[bench] {label}: precision={precision} setup {time.time() - t0:.1f} s")
     step = (lambda i: traced_increment(sv, i, prob.control)) if trace else sv.increment
     inc = 0
     warm = []
-    for _ in range(warmup):
+    t_w = time.perf_counter()
+    while inc < max(warmup, 1):
         inc += 1
         st = step(inc)
         warm.append(st)
-        log(f"[bench] warmup inc {inc}: nit {st['nit']} sweeps {st['sor_sweeps_total']} "
-            f"{st['t_total_ms']:.1f} ms code {st['code']} omega {st['sor_omega']:.3g} waves/block {st['sor_wpb']} "
-            f"tune {st['t_tune_ms']:.1f} ms ({st['tune_sweeps']} sweeps)")
-    torch.cuda.synchronize()
+        log(f"[bench] {label} warmup inc {inc}: nit {st['nit']} sweeps {st['sor_sweeps_total']} "
+            f"{st['t_total_ms']:.1f} ms code {st['code']} 987-points {st['pts_987_last']} omega "
+            f"{st['sor_omega']:.3g} waves/block {st['sor_wpb']} tune {st['t_tune_ms']:.1f} ms")
+        if time.perf_counter() - t_w > warm_s:
+            break
     recs = []
     t0 = time.perf_counter()
-    for _ in range(steps):
+    while len(recs) < max(steps, 1) and inc < prob.ninc1:
         inc += 1
         st = step(inc)
         recs.append(st)
-        log(f"[bench] inc {inc}: nit {st['nit']} sweeps {st['sor_sweeps_total']} total {st['t_total_ms']:.1f} ms "
-            f"const {st['t_const_ms']:.1f} asm {st['t_asm_ms']:.1f} sor {st['t_sor_ms']:.1f} code {st['code']} "
-            f"pts {st['gp_updates']} continued {st['pts_continued']} wide {st['pts_wide']} "
-            f"sweep launches {st['sweep_launches']} {st['t_sweep_ms']:.1f} ms")
-    torch.cuda.synchronize()
+        log(f"[bench] {label} inc {inc}: nit {st['nit']} sweeps {st['sor_sweeps_total']} total "
+            f"{st['t_total_ms']:.1f} ms const {st['t_const_ms']:.1f} asm {st['t_asm_ms']:.1f} sor "
+            f"{st['t_sor_ms']:.1f} code {st['code']} 987-points {st['pts_987_last']} (updates {st['pts_987']}) "
+            f"pts {st['gp_updates']} continued {st['pts_continued']} wide {st['pts_wide']}")
+        if time.perf_counter() - t0 > timed_s:
+            break
     wall = time.perf_counter() - t0
-    state = sv.get_state()
     sv.close()
-    return recs, wall, state, warm
+    return recs, wall, warm, inc
 
 
-def cpu_increment_compare(capi_mod, n=14):
-    """One whole load increment of the compiled reference program (fasolt3, one
-    core, its own solver settings: SOR omega 1.2, at most 1000 sweeps, stop on
-    the relative update below toler) next to the same increment on the GPU
-    with those settings.  The reference has no 8-node brick with 8 points and
-    caps its mesh at 8,100 elements, so the mesh is n^3 8-node bricks with one
-    point (lt 12) on the plane-strain load path."""
-    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
-    import tempfile
-    import refio
-    from cpfem_code_amd import fsd
-    from cpfem_code_amd.problem import LT_BRICK_1GP, Control, fcc_plane_strain
-    if not refio.available():
-        return None
-    p = fcc_plane_strain(n, ninc=1, seed=2, lt=LT_BRICK_1GP, control=Control())
-    d = tempfile.mkdtemp(prefix="cpfem_ref_")
-    fsd.write_fsd(p, os.path.join(d, "cmp.fsd"), ninc=1)
-    fsd.write_control_files(d, p.control, solver_file=False)
-    log(f"[bench] reference fasolt3: one increment on {n}^3 one-point bricks (one core)")
-    t0 = time.perf_counter()
-    refio.fasolt3(d, "cmp", 1)
-    t_ref = time.perf_counter() - t0
-    frs = os.path.join(d, "cmp-1.frs")
-    if not os.path.exists(frs):
-        return {"error": "the reference stopped with a FAFERR code", "reference_s": t_ref}
-    nit_ref = 0
-    for ln in open(os.path.join(d, "fafrep.txt")).read().splitlines():
-        tok = ln.split()
-        if len(tok) == 5 and tok[0].isdigit():
-            nit_ref = int(tok[0])
-    ref = fsd.read_frs(frs)
-    warm = capi_mod.Solver(p, precision=0)
-    warm.increment(1)
-    warm.close()
-    sv = capi_mod.Solver(p, precision=0)
-    t0 = time.perf_counter()
-    st = sv.increment(1)
-    t_gpu = time.perf_counter() - t0
-    x = sv.get_state()["x"]
-    sv.close()
-    # the same increment solved tightly (residual 1e-10, tuned omega): how far
-    # each loose solve is from the converged displacements
-    import dataclasses
-    tight = dataclasses.replace(p, control=dataclasses.replace(p.control, sor_kind=1, sor_tol=1e-10, sor_omega=0.0,
-                                                               sor_maxit=500000))
-    sv = capi_mod.Solver(tight, precision=0)
-    sv.increment(1)
-    u_tight = sv.get_state()["x"] - p.x
-    sv.close()
-    u_ref = ref["x"] - p.x
-    scale = np.abs(u_tight).max()
+def leg_summary(recs, wall, warm, first_inc):
+    n = len(recs)
+    s = {k: sum(r[k] for r in recs) for k in ("nit", "gp_updates", "sor_sweeps_total", "flops_const",
+                                               "transc_const", "pts_987", "sweep_launches")}
+    t = {k: sum(r[k] for r in recs) * 1e-3 for k in ("t_const_ms", "t_asm_ms", "t_sor_ms", "t_suvals_ms",
+                                                      "t_update_ms", "t_sweep_ms")}
     return {
-        "mesh": f"{n}^3 8-node bricks, one point each (lt 12), FCC plane strain, {p.ngps} points, {p.ndof} DOFs",
-        "settings": "the reference's: SOR omega 1.2, <= 1000 sweeps, relative-update stop at toler 1e-3; FP64 "
-                    "on the GPU, REAL(4) in the reference",
-        "reference_s_per_increment": t_ref, "reference_cores": 1, "reference_nr_iterations": nit_ref,
-        "gpu_s_per_increment": t_gpu, "gpu_nr_iterations": st["nit"], "ratio": t_ref / t_gpu,
-        "displacement_diff_gpu_vs_reference": float(np.abs((x - p.x) - u_ref).max() / scale),
-        "displacement_diff_reference_vs_tight": float(np.abs(u_ref - u_tight).max() / scale),
-        "displacement_diff_gpu_vs_tight": float(np.abs((x - p.x) - u_tight).max() / scale),
-        "note": "reference time includes its .fsd read and .frs write; both stop their SOR solves on the "
-                "reference's loose relative-update rule, which leaves part of the slowly converging modes of "
-                "this mesh (free x face) in the displacements; the differences are max-norm, relative to the "
-                "largest displacement of a tight solve of the same increment",
+        "increments_timed": list(range(first_inc, first_inc + n)),
+        "s_per_increment": wall / n,
+        "s_per_increment_each": [r["t_total_ms"] * 1e-3 for r in recs],
+        "gp_updates_per_s": s["gp_updates"] / wall,
+        "gp_updates_per_s_constitutive": s["gp_updates"] / t["t_const_ms"],
+        "nr_iterations_per_increment": s["nit"] / n,
+        "nr_iterations_each": [r["nit"] for r in recs],
+        "sor_sweeps_per_increment": s["sor_sweeps_total"] / n,
+        "constitutive_s_per_increment": t["t_const_ms"] / n,
+        "assembly_s_per_increment": t["t_asm_ms"] / n,
+        "sor_s_per_increment": t["t_sor_ms"] / n,
+        "bcs_s_per_increment": t["t_suvals_ms"] / n,
+        "update_s_per_increment": t["t_update_ms"] / n,
+        "constitutive_flops_per_point_update": s["flops_const"] / max(s["gp_updates"], 1),
+        "constitutive_transcendentals_per_point_update": s["transc_const"] / max(s["gp_updates"], 1),
+        "sor_omega": recs[-1]["sor_omega"],
+        "sor_waves_per_workgroup": recs[-1]["sor_wpb"],
+        "omega_tuning_s_once": sum(r["t_tune_ms"] for r in warm + recs) * 1e-3,
+        "faferr_codes_each": [r["code"] for r in recs],
+        "points_987_final_iteration_each": [r["pts_987_last"] for r in recs],
+        "point_updates_987_each": [r["pts_987"] for r in recs],
+        "warmup_increments": len(warm),
     }
 
 
-def cpu_baseline(prob, state, seconds, capi_mod):
-    """Reference DSTRSS (REAL(4), compiled from the reference sources) on a sample of points."""
-    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
-    import refio
-    if not refio.available():
-        return None
-    rng = np.random.default_rng(0)
-    n = 2000
-    idx = np.sort(rng.choice(prob.ngps, size=min(n, prob.ngps), replace=False))
-    iel = np.ones(len(idx), np.int32)
-    b = dict(de=state["de"][idx], r=state["dr"][idx], st=state["st"][idx] - state["ds"][idx],
-             ds=state["ds"][idx], sv=state["sv"][idx] - state["dsv"][idx], dsv=state["dsv"][idx])
-    mat, ctl = prob.materials[0], prob.control
-    done, t_total = 0, 0.0
-    while t_total < seconds:
+def roofline_of(prob, recs, traffic, key_sor, s_a):
+    """Roofline objects of the constitutive phase and the sweep kernel of a leg."""
+    nnzb = nnzb_structured(prob.dims)
+    launches = sum(r["sweep_launches"] for r in recs)
+    t_sw = sum(r["t_sweep_ms"] for r in recs) * 1e-3
+    nsw = sum(r["sor_sweeps_total"] for r in recs)
+    per_launch = sor_bytes(prob, nnzb, s_a, s_a) * nsw / launches
+    dur = t_sw / launches
+    sor = {"bound": "hbm", "achieved": per_launch / dur / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+           "frac": per_launch / dur / HBM_PEAK, "traffic": traffic.get(key_sor) if traffic else None,
+           "bytes_per_launch": per_launch, "launch_us": dur * 1e6, "launches": launches,
+           "kernel": f"{key_sor} (one colour of one SOR sweep)"}
+    nit = sum(r["nit"] for r in recs)
+    flops = sum(r["flops_const"] for r in recs)
+    t_c = sum(r["t_const_ms"] for r in recs) * 1e-3
+    const = {"bound": "fp64-valu", "achieved": flops / t_c / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+             "frac": flops / t_c / FP64_PEAK, "traffic": traffic.get("k_const") if traffic else None,
+             "flops_per_launch": flops / nit, "launch_us": t_c / nit * 1e6, "launches": nit,
+             "kernel": "k_const<12,*> (the constitutive kernels of one Newton iteration: every point's DSTRSS; "
+                       "one 'launch' = the constitutive phase of one ELVALS, event-timed on the library stream)"}
+    asm_flops = 75968 * prob.nlmnt * nit
+    t_a = sum(r["t_asm_ms"] for r in recs) * 1e-3
+    asm = {"bound": "fp64-valu", "achieved": asm_flops / t_a / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+           "frac": asm_flops / t_a / FP64_PEAK, "traffic": traffic.get("k_assemble<double>") if traffic else None,
+           "kernel": "k_assemble<double> (element assembly of one Newton iteration)"}
+    return {"sor": sor, "constitutive": const, "assembly": asm,
+            "share_s": {"sor": sum(r["t_sor_ms"] for r in recs) * 1e-3, "constitutive": t_c, "assembly": t_a}}
+
+
+def mixed_accuracy(prob, capi):
+    """One Newton system (the first of increment 1) solved by each path to the
+    1e-8 scaled residual, against a 1e-12 FP64 solve of the same system."""
+    import dataclasses
+    base = dataclasses.replace(prob.control, sor_omega=1.94, sor_maxit=400000)
+    out = {}
+
+    def solve(precision, tol):
+        sv = capi.Solver(prob, precision=precision, control=dataclasses.replace(base, sor_tol=tol))
+        sv.update_node()
+        sv.elvals(1)
+        sv.suvals(1)
+        code, sweeps, err = sv.facsls()
+        _, ddx = sv.system()
+        sv.close()
+        return ddx, sweeps, err
+
+    exact, _, _ = solve(0, 1e-12)
+    scale = np.abs(exact).max()
+    for name, prec in (("fp64", 0), ("mixed", 2), ("mixed_fp32_matrix", 1)):
+        d, sw, err = solve(prec, prob.control.sor_tol)
+        out[name] = {"precision": prec, "sweeps": sw, "scaled_residual": err,
+                     "displacement_rel_error_vs_tight_fp64": float(np.abs(d - exact).max() / scale)}
+    out["note"] = ("max-norm error of the solution of one Newton system relative to its largest component, each "
+                   "path stopped at the 1e-8 Jacobi-scaled residual (omega 1.94), against a 1e-12 FP64 solve of "
+                   "the same system")
+    return out
+
+
+def cpu_baseline(capi, seconds):
+    """The FP64 oracle (the CPU restatement, one core) on whole increments of the
+    CPU-runnable case: FCC uniaxial tension along z, 8^3 bricks x 8 points,
+    SOR omega 1.6 to the 1e-8 scaled residual; then the GPU on the same
+    increments.  Both continue past FAFERR 987 (the reference would stop)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle, OracleModel
+    from cpfem_code_amd.problem import Control, fcc_tension
+    ctl = Control(sor_kind=1, sor_tol=1e-8, sor_omega=1.6, sor_maxit=200000)
+    prob = fcc_tension(8, ninc=50, seed=0, control=ctl)
+    om = OracleModel(Oracle(64), prob)
+    om.ctl.keep_going = 1
+    upd, t_cpu, inc = 0, 0.0, 0
+    while t_cpu < seconds and inc < 50:
+        inc += 1
         t0 = time.perf_counter()
-        refio.dstrss([mat], ctl, float(prob.tiem[0]), iel, b["de"], b["r"], b["st"], b["ds"], b["sv"], b["dsv"])
-        t_total += time.perf_counter() - t0
-        done += len(idx)
-    return dict(value=done / t_total, unit="GP-updates/s", cores=1, kind="reference",
-                sample=f"{done} point updates (DSTRSS, REAL(4) reference compiled with flang -O2) on 1 core: "
-                       f"{len(idx)} points drawn from the benchmark state after the timed increments, "
-                       f"{t_total:.1f} s including the harness file I/O")
+        om.increment(inc)
+        t_cpu += time.perf_counter() - t0
+        upd += prob.ngps * om.m.nit
+    # the GPU on the same increments (FP64 path, same settings)
+    sv = capi.Solver(prob, precision=0)
+    sv.increment(1)
+    sv.close()
+    sv = capi.Solver(prob, precision=0)
+    g_upd, t_gpu = 0, 0.0
+    for i in range(1, inc + 1):
+        t0 = time.perf_counter()
+        st = sv.increment(i)
+        t_gpu += time.perf_counter() - t0
+        g_upd += st["gp_updates"]
+    sv.close()
+    return dict(value=upd / t_cpu, unit="GP-updates/s", cores=1, kind="port",
+                sample=f"FP64 oracle (CPU restatement of the reference path, gcc -O2, one thread) on increments "
+                       f"1-{inc} of FCC uniaxial tension, 8^3 bricks x 8 points ({prob.ngps} points), SOR omega 1.6 "
+                       f"to the 1e-8 scaled residual: {upd} point updates in {t_cpu:.1f} s",
+                host_cpu=cpu_model(), host_nproc=os.cpu_count(),
+                gpu_same_increments={"gp_updates_per_s": g_upd / t_gpu, "s": t_gpu,
+                                     "ratio_gpu_over_cpu": (g_upd / t_gpu) / (upd / t_cpu)})
 
 
-def calibration_leg(capi_mod, n, ctl, cpu=True):
-    """Gradient of the macroscopic stress-strain curve (16^3 FCC tension, 50
-    increments, FP64): the load path on the GPU with the inputs of every
-    point's last constitutive call recorded, then the replay loss
-    sum (curve - target)^2 and its adjoint gradient with respect to tau0,
-    taus, h0, latent ratio and m, next to central differences (relative step
-    1e-6) of the same replay."""
+def sweep_probe(n, sweeps, copy_mib):
+    """Fixed-length SOR run on the first Newton system of the n^3 workload:
+    exactly `sweeps` colour sweeps in the FP64 path and in both mixed paths,
+    each timed with HIP events, then a device copy of copy_mib MiB whose
+    FETCH_SIZE / WRITE_SIZE readings calibrate the counters under the same
+    profiler."""
+    from cpfem_code_amd import capi
+    from cpfem_code_amd.problem import Control, fcc_plane_strain
+    if sweeps % 16:
+        raise SystemExit("--sweep-probe needs a multiple of the 16-sweep batch")
+    ctl = Control(sor_kind=1, sor_tol=1e-30, sor_omega=1.6, sor_maxit=sweeps)
+    prob = fcc_plane_strain(n, ninc=3, seed=2, control=ctl)
+    print(f"sweep probe n={n} sweeps={sweeps} nodes={prob.nnod}", flush=True)
+    for prec in (0, 2, 1):
+        sv = capi.Solver(prob, precision=prec)
+        sv.update_node()
+        sv.elvals(1)
+        sv.suvals(1)
+        t0 = time.perf_counter()
+        code, nsw, err = sv.facsls()
+        wall = time.perf_counter() - t0
+        st = sv.stats()
+        per = 1e3 * st["t_sweep_ms"] / max(st["sweep_launches"], 1)
+        print(f"precision {prec}: {nsw} sweeps, {st['sweep_launches']} colour launches, {per:.2f} us per launch "
+              f"(events), facsls wall {1e3 * wall:.1f} ms, residual {err:.3e}", flush=True)
+        sv.close()
+    import torch
+    nbytes = copy_mib << 20
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"calibration copy: {nbytes} bytes read and {nbytes} bytes written", flush=True)
+
+
+def calibration_leg(capi_mod, n, ctl):
+    """Gradient of the macroscopic stress-strain curve (FCC tension, 50
+    increments, FP64) by the adjoint, next to central differences."""
     import dataclasses
     from cpfem_code_amd import calibrate
     from cpfem_code_amd.problem import fcc_tension
-    # the path's solves to the same 1e-8 residual with omega tuned at the first solve
     prob = fcc_tension(n, ninc=50, seed=4, control=dataclasses.replace(ctl, sor_omega=0.0))
     t0 = time.time()
-    path = calibrate.record_path(prob, ninc=50, comp=2, ckpt_every=10, equilibrium=True)
+    path = calibrate.record_path(prob, ninc=50, comp=2, ckpt_every=10, equilibrium=False)
     t_record = time.time() - t0
     p0 = calibrate.material_params(prob.materials[0])
-    c_all = path.solver.calib_run(p0, np.zeros(50), grad=False)[0]
-    fe_rel = float(np.abs(c_all - path.fe_curve).max() / np.abs(path.fe_curve).max())
     nx = calibrate.exclude_nonconverging(path, p0, rel=1e-6)
-    target, p_t = calibrate.seeded_target(path, p0, seed=4)
-    # gradient of the finite-element path: replay adjoint plus the equilibrium correction, its
-    # solves with the relaxation factor tuned once on the last recorded system
-    t0 = time.time()
-    tuned = path.solver.autotune()
-    t_tune = time.time() - t0
-    calibrate.loss_and_grad(path, p0, target)          # warm-up
-    t0 = time.time()
-    _, g_eq, _, st_eq = calibrate.loss_and_grad(path, p0, target)
-    t_eq = time.time() - t0
-    # gradient of the replay (kinematics held fixed), checked against its central differences
-    path.solver.calib_options(0)
-    calibrate.loss_and_grad(path, p0, target)          # warm-up
-    t0 = time.time()
-    loss, g, curve, st = calibrate.loss_and_grad(path, p0, target)
-    t_grad = time.time() - t0
-    t0 = time.time()
-    _, _, _, stf = calibrate.loss_and_grad(path, p0, target, grad=False)
-    t_fwd = time.time() - t0
-    t0 = time.time()
-    fd = calibrate.central_differences(path, p0, target, rel=1e-6)
-    t_fd = time.time() - t0
-    out = {
-        "workload": f"FCC Al uniaxial tension along z, {n}^3 8-node bricks x 8 points, 50 increments of 1e-3, FP64",
-        "points": prob.ngps, "increments": 50, "parameters": list(calibrate.PARAM_NAMES),
-        "params": p0.tolist(), "target_params_seed4": p_t.tolist(),
-        "points_excluded_nonconverging": nx,
-        "record_s": t_record, "replay_forward_s": t_fwd, "loss_and_gradient_s": t_grad,
-        "forward_ms_in_gradient": st["t_forward_ms"], "reverse_ms_in_gradient": st["t_reverse_ms"],
-        "central_differences_s": t_fd, "loss": loss, "gradient": g.tolist(), "central_differences": fd.tolist(),
-        "equilibrium_corrected": {
-            "gradient": g_eq.tolist(), "loss_and_gradient_s": t_eq, "reverse_ms": st_eq["t_reverse_ms"],
-            "solves": st_eq["equilibrium_solves"], "sor_sweeps": st_eq["solve_sweeps"],
-            "sor_tuning": {"result": tuned, "s": t_tune},
-            "note": "adds the response of the converged displacement increments (one solve with each increment's "
-                    "recorded Newton matrix); the gradient of the finite-element path itself, checked against "
-                    "finite-element central differences in tests/test_gpu_calib.py"},
-        "max_rel_diff_vs_central_differences": float(np.max(np.abs(g - fd) / np.abs(fd))),
-        "replay_vs_finite_element_curve_max_rel": fe_rel,
-        "checkpoint_every": 10,
-        "note": "replay tolerance 1e-10 (maxit 60, stall_tol 1e-14); points that stop at the iteration limit at the parameters or "
-                "on their difference stencil get zero weight; the gradient holds the recorded kinematics fixed",
-    }
-    # the same gradient with the whole trajectory kept in HBM (no recomputation)
-    path.solver.calib_checkpoints(0)
-    # a fit of the five parameters to the seeded curve (L-BFGS through the torch wrapper)
-    t0 = time.time()
-    x_fit, hist = calibrate.fit(path, p0, target, iterations=20)
-    out["fit"] = {"s": time.time() - t0, "evaluations": len(hist), "loss_start": hist[0], "loss_end": min(hist),
-                  "params": x_fit.tolist(), "target_params": p_t.tolist(),
-                  "note": "torch.optim.LBFGS (strong Wolfe) on the replay loss, gradient from the adjoint, "
-                          "trajectory kept"}
+    target, _ = calibrate.seeded_target(path, p0, seed=4)
     calibrate.loss_and_grad(path, p0, target)
     t0 = time.time()
-    _, g_keep, _, st_keep = calibrate.loss_and_grad(path, p0, target)
-    out["trajectory_kept"] = {"loss_and_gradient_s": time.time() - t0, "reverse_ms": st_keep["t_reverse_ms"],
-                              "gradient_equal_to_checkpointed": bool(np.array_equal(g_keep, g))}
-    if cpu:
-        # the oracle's replay of the first increments on one core
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        from oracle import Oracle
-        from adjcheck import replay_oracle
-        from cpfem_code_amd.problem import Control as Ctl
-        hist = [path.solver.calib_history(i) for i in range(1, 51)]
-        t0 = time.time()
-        k = 5
-        replay_oracle(Oracle(64), prob, p0, hist[:k], ctl=Ctl(toler=1e-10, maxit=60, stall_tol=1e-14))
-        t1 = (time.time() - t0) / k
-        out["cpu_baseline_replay"] = {"kind": "port", "cores": 1,
-                                      "sample": f"oracle replay of increments 1-{k} of 50 (the yield transition)",
-                                      "s_per_increment": t1, "value": prob.ngps / t1, "unit": "GP-updates/s",
-                                      "gpu_replay_s_per_increment": t_fwd / 50}
+    loss, g, _, st = calibrate.loss_and_grad(path, p0, target)
+    t_grad = time.time() - t0
+    fd = calibrate.central_differences(path, p0, target, rel=1e-6)
     path.solver.close()
-    return out
+    return {"workload": f"FCC uniaxial tension, {n}^3 bricks x 8 points, 50 increments, FP64",
+            "record_s": t_record, "points_excluded": nx, "loss_and_gradient_s": t_grad, "loss": loss,
+            "gradient": g.tolist(), "central_differences": fd.tolist(),
+            "max_rel_diff": float(np.max(np.abs(g - fd) / np.abs(fd)))}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=None, help="mesh size (default: the workload's)")
-    ap.add_argument("--workload", default="plane_strain",
-                    choices=["plane_strain", "tension", "hcp_compression"],
-                    help="plane_strain: FCC 64^3 plane-strain compression (the headline line); tension: FCC "
-                         "uniaxial tension (32^3 FP64 drop-in case); hcp_compression: Mg-like HCP with twins (48^3)")
-    ap.add_argument("--paths", default="fp64,mixed")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--ref-increment", type=int, default=14,
-                    help="mesh size n of the whole-increment comparison with the compiled reference program "
-                         "(about 30 s of one core at 14); 0 skips it")
-    ap.add_argument("--omega-variants", default="mixed:auto,fp64:auto,mixed:1.9,mixed:auto+stall",
-                    help="comma list of path:omega[+stall] (path fp64 or mixed; omega a number, or 'auto' for "
-                         "the omega and launch shape tuned at the first solve; +stall: CPSTRS also stops at a "
-                         "stationary stress, stall_tol 1e-14) to run as well, reported beside the headline, "
-                         "which uses the configured 1.6 and the reference's iteration limit; empty skips them")
-    ap.add_argument("--calibration", type=int, default=16,
-                    help="mesh size of the calibration leg (replay loss and adjoint gradient over 50 increments "
-                         "of tension); 0 skips it")
+    ap.add_argument("--workload", default="plane_strain", choices=["plane_strain", "tension", "hcp_compression"],
+                    help="plane_strain: FCC 64^3 plane-strain compression (the headline); tension: FCC uniaxial "
+                         "tension (32^3 FP64 drop-in case); hcp_compression: Mg-like HCP with twins (48^3)")
+    ap.add_argument("--omega", default="auto",
+                    help="SOR relaxation factor of legs 1-2: 'auto' (tuned with the launch shape at the first "
+                         "solve, in the warm-up) or a number")
+    ap.add_argument("--budget", type=float, default=360.0,
+                    help="wall seconds for the timed work of all legs; each leg gets a share and stops early")
+    ap.add_argument("--configured-omega", type=float, default=1.6,
+                    help="omega of leg 3 (the configured factor); 0 skips the leg")
+    ap.add_argument("--mixed-precision", type=int, default=2, choices=[1, 2],
+                    help="mixed path: 2 = FP32 sweeps on a rounded copy of the FP64 matrix (default), 1 = FP32 "
+                         "element products and matrix")
+    ap.add_argument("--stall-tol", type=float, default=0.0,
+                    help="> 0: CPSTRS also stops at a stationary stress (opt-in; the reference runs to maxit)")
+    ap.add_argument("--accuracy", type=int, default=1, help="0 skips leg 4 (one-system accuracy check)")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample; 0 skips it")
+    ap.add_argument("--calibration", type=int, default=0, help="mesh size of an opt-in calibration leg (0: none)")
     ap.add_argument("--trace-iterations", action="store_true",
                     help="drive each increment through the per-step calls and print every Newton iteration")
+    ap.add_argument("--sweep-probe", type=int, default=0,
+                    help="profiling mode: run exactly this many SOR sweeps per path on the first system and exit")
+    ap.add_argument("--copy-mib", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this library runs on one GPU: the z-slab sharding over several GPUs is a "
                                    "design, not an implementation, so nothing is launched", "n_gpus": args.gpus}))
         sys.exit(2)
-    import torch
-    assert torch.cuda.is_available(), "no GPU"
+    t_start = time.time()
     from cpfem_code_amd import capi
     from cpfem_code_amd.problem import Control, fcc_plane_strain, fcc_tension, hcp_compression
+    if args.sweep_probe:
+        sweep_probe(args.n or 64, args.sweep_probe, args.copy_mib)
+        return
     name, cus = capi.device_info()
-    ctl = Control(sor_kind=1, sor_tol=1e-8, sor_omega=1.6, sor_maxit=200000)
-    t0 = time.time()
-    ninc = args.warmup + args.steps + 1
+    import dataclasses
+    ctl = Control(sor_kind=1, sor_tol=1e-8, sor_omega=1.6, sor_maxit=400000, stall_tol=args.stall_tol)
+    ninc = 100
     if args.workload == "plane_strain":
         args.n = args.n or 64
         prob = fcc_plane_strain(args.n, ninc=ninc, seed=2, control=ctl)
@@ -370,134 +406,74 @@ def main():
         prob = hcp_compression(args.n, ninc=ninc, seed=3, control=ctl)
         wl = (f"Mg-like HCP (12 slip + 6 twin systems, twinprob 1) compression along x, basal fibre, "
               f"{args.n}^3 8-node bricks x 8 points, eps_xx=-1e-3/increment")
-    wl += ", SOR omega 1.6 to 1e-8 relative residual"
+    om_head = 0.0 if args.omega == "auto" else float(args.omega)
+    wl += f", SOR to 1e-8 relative residual, omega {'tuned at the first solve' if om_head == 0 else om_head}"
     log(f"[bench] {name} {cus} CUs; problem {args.n}^3: {prob.nlmnt} elements, {prob.ngps} points, "
-        f"{prob.ndof} DOFs (generated in {time.time() - t0:.1f} s)")
-    results = {}
-    last_state = None
-    for path in args.paths.split(","):
-        prec = 0 if path == "fp64" else 1
-        recs, wall, state, _ = run_path(prob, prec, args.warmup, args.steps, capi, trace=args.trace_iterations)
-        results[path] = (recs, wall)
-        if prec == 0:
-            last_state = state
-    omega_runs = []
-    variants = [v.strip() for v in args.omega_variants.split(",") if v.strip()]
-    for v in variants:
-        import dataclasses
-        path, _, oms = v.partition(":") if ":" in v else ("mixed", ":", v)
-        oms, _, opt = oms.partition("+")
-        stall = 1e-14 if opt == "stall" else 0.0
-        om = 0.0 if oms == "auto" else float(oms)
-        pv = dataclasses.replace(prob, control=dataclasses.replace(ctl, sor_omega=om, stall_tol=stall))
-        recs_o, wall_o, _, warm_o = run_path(pv, 0 if path == "fp64" else 1, args.warmup, args.steps, capi)
-        run = {
-            "omega": recs_o[-1]["sor_omega"], "set_as": oms, "precision": path,
-            "stall_tol": stall,
-            "waves_per_workgroup": recs_o[-1]["sor_wpb"],
-            "s_per_increment": wall_o / args.steps,
-            "sweeps_per_increment": sum(r["sor_sweeps_total"] for r in recs_o) / args.steps,
-            "nr_iterations_per_increment": sum(r["nit"] for r in recs_o) / args.steps,
-            "gp_updates_per_s": sum(r["gp_updates"] for r in recs_o) / wall_o,
-            "constitutive_s_per_increment": sum(r["t_const_ms"] for r in recs_o) * 1e-3 / args.steps,
-            "sor_s_per_increment": sum(r["t_sor_ms"] for r in recs_o) * 1e-3 / args.steps,
-            "note": "same workload and 1e-8 relative residual with the SOR relaxation factor set in cpfem.dat; "
-                    "not the headline value, which keeps the configured omega 1.6"}
-        if stall:
-            run["note"] += ("; '+stall': CPSTRS also stops at a stationary stress (stall_tol 1e-14, "
-                            "results within ~1e-13 of the reference's run to maxit)")
-        if om == 0.0:
-            run["tune_s_once"] = sum(r["t_tune_ms"] for r in warm_o + recs_o) * 1e-3
-            run["tune_sweeps_once"] = sum(r["tune_sweeps"] for r in warm_o + recs_o)
-            run["note"] += ("; 'auto': omega <= 0, tuned with the sweep launch shape at the first solve (in the "
-                            "untimed warm-up increment, cost tune_s_once)")
-        omega_runs.append(run)
-    recs64, wall64 = results["fp64"]
-    nit = sum(r["nit"] for r in recs64)
-    gp_updates = sum(r["gp_updates"] for r in recs64)
-    t_const = sum(r["t_const_ms"] for r in recs64) * 1e-3
-    t_asm = sum(r["t_asm_ms"] for r in recs64) * 1e-3
-    t_sor = sum(r["t_sor_ms"] for r in recs64) * 1e-3
-    sweeps = sum(r["sor_sweeps_total"] for r in recs64)
-    flops = sum(r["flops_const"] for r in recs64)
-    transc = sum(r["transc_const"] for r in recs64)
-    ms_per_inc = 1e3 * wall64 / args.steps
-    value = gp_updates / wall64
-    nnzb = nnzb_structured(prob.dims)
+        f"{prob.ndof} DOFs")
+    B = args.budget
+    p_head = dataclasses.replace(prob, control=dataclasses.replace(ctl, sor_omega=om_head))
+    legs = {}
     traffic = load_traffic()
-    # roofline of each hot kernel: algorithmic work per launch / event-timed launch duration
-    asm_flops = 75968 * prob.nlmnt * nit
-
-    def sweep_roof(recs, s_a, s_x, key):
-        launches = sum(r["sweep_launches"] for r in recs)
-        t = sum(r["t_sweep_ms"] for r in recs) * 1e-3
-        nsw = sum(r["sor_sweeps_total"] for r in recs)
-        per_launch = sor_bytes(prob, nnzb, s_a, s_x) * nsw / launches
-        dur = t / launches
-        tr = traffic.get(key) if traffic else None
-        return {"bound": "hbm", "achieved": per_launch / dur / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "traffic": tr, "frac": per_launch / dur / HBM_PEAK, "bytes_per_launch": per_launch,
-                "launch_us": dur * 1e6, "launches": launches}
-
-    roof = {
-        "sor_fp64": sweep_roof(recs64, 8, 8, "k_sor_color<double>"),
-        "constitutive": {"bound": "mfma", "achieved": flops / t_const / 1e12, "peak": FP64_PEAK / 1e12,
-                         "unit": "TFLOP/s", "traffic": None, "frac": flops / t_const / FP64_PEAK},
-        "assembly": {"bound": "mfma", "achieved": asm_flops / t_asm / 1e12, "peak": FP64_PEAK / 1e12,
-                     "unit": "TFLOP/s", "traffic": None, "frac": asm_flops / t_asm / FP64_PEAK},
-    }
-    extra = {}
-    if "mixed" in results:
-        recs32, wall32 = results["mixed"]
-        roof["sor_fp32"] = sweep_roof(recs32, 4, 4, "k_sor_color<float>")
-        extra["s_per_increment_mixed"] = wall32 / args.steps
-        extra["mixed_sor_s_per_increment"] = sum(r["t_sor_ms"] for r in recs32) * 1e-3 / args.steps
-        extra["mixed_sweeps_per_increment"] = sum(r["sor_sweeps_total"] for r in recs32) / args.steps
-        extra["mixed_nr_iterations_per_increment"] = sum(r["nit"] for r in recs32) / args.steps
-        extra["mixed_speedup_vs_fp64"] = wall64 / wall32
-        extra["faferr_codes_per_increment_mixed"] = [r["code"] for r in recs32]
-    if omega_runs:
-        extra["mixed_with_omega"] = omega_runs
-    shares = {"sor_fp64": t_sor, "constitutive": t_const, "assembly": t_asm}
-    dom = max(shares, key=shares.get)
-    cpu = cpu_baseline(prob, last_state, args.cpu_seconds, capi) if args.cpu_seconds > 0 else None
-    if args.cpu_seconds > 0 and args.ref_increment > 0:
-        extra["cpu_reference_increment"] = cpu_increment_compare(capi, args.ref_increment)
+    # 1-2: the headline paths
+    recs64, wall64, warm64, _ = run_path(p_head, 0, args.warmup, args.steps, 0.08 * B, 0.22 * B, capi,
+                                         trace=args.trace_iterations, label="fp64")
+    first = len(warm64) + 1
+    legs["fp64"] = leg_summary(recs64, wall64, warm64, first)
+    recsm, wallm, warmm, _ = run_path(p_head, args.mixed_precision, args.warmup, args.steps, 0.08 * B, 0.18 * B,
+                                      capi, trace=args.trace_iterations, label="mixed")
+    legs["mixed"] = leg_summary(recsm, wallm, warmm, len(warmm) + 1)
+    # 3: the configured relaxation factor, one timed increment each (increment 2)
+    if args.configured_omega > 0 and time.time() - t_start < 0.75 * B:
+        pc = dataclasses.replace(prob, control=dataclasses.replace(ctl, sor_omega=args.configured_omega))
+        for lab, prec in (("fp64", 0), ("mixed", args.mixed_precision)):
+            if time.time() - t_start > 0.9 * B:
+                break
+            r, w, wm, _ = run_path(pc, prec, 1, 1, 0.0, 0.0, capi, label=f"{lab} omega {args.configured_omega}")
+            legs[f"{lab}_omega_{args.configured_omega}"] = leg_summary(r, w, wm, len(wm) + 1)
+    acc = None
+    if args.accuracy and args.workload == "plane_strain" and time.time() - t_start < B:
+        acc = mixed_accuracy(prob, capi)
+    cpu = cpu_baseline(capi, args.cpu_seconds) if args.cpu_seconds > 0 else None
+    h = legs["fp64"]
+    roof = roofline_of(prob, recs64, traffic, "k_sor_color<double>", 8)
+    roof_m = roofline_of(prob, recsm, traffic, "k_sor_color<float>", 4)
+    dom = max(("sor", "constitutive", "assembly"), key=lambda k: roof["share_s"][k])
     line = {
-        "metric": "Gauss-pt updates/s + s/increment at 64^3 FCC, FP64 vs mixed-prec SOR",
-        "value": value,
+        "metric": METRIC,
+        "value": h["gp_updates_per_s"],
         "unit": "GP-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_inc,
+        "steps": len(recs64),
+        "warmup": len(warm64),
+        "ms_per_step": 1e3 * h["s_per_increment"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded mesh, texture and load path; no external data)",
         "config": {"workload": wl, "elements": prob.nlmnt, "points": prob.ngps, "dofs": prob.ndof},
-        "s_per_increment_fp64": wall64 / args.steps,
-        "gp_updates_per_s_constitutive": gp_updates / t_const,
-        "nr_iterations_per_increment": nit / args.steps,
-        "sor_sweeps_per_increment": sweeps / args.steps,
-        "assembly_s_per_increment": t_asm / args.steps,
-        "sor_s_per_increment": t_sor / args.steps,
-        "constitutive_s_per_increment": t_const / args.steps,
-        "constitutive_flops_per_point_update": flops / gp_updates,
-        "constitutive_transcendentals_per_point_update": transc / gp_updates,
-        "roofline": dict(roof[dom], kernel={"sor_fp64": "k_sor_color<double> (one colour of one SOR sweep)",
-                                            "constitutive": "k_const", "assembly": "k_assemble<double>"}[dom]),
-        "roofline_all": roof,
+        "s_per_increment_fp64": h["s_per_increment"],
+        "s_per_increment_mixed": legs["mixed"]["s_per_increment"],
+        "mixed_speedup_vs_fp64": h["s_per_increment"] / legs["mixed"]["s_per_increment"],
+        "mixed_precision_mode": args.mixed_precision,
+        "roofline": roof[dom],
+        "roofline_all": {"fp64": {k: v for k, v in roof.items()}, "mixed": {"sor": roof_m["sor"],
+                                                                            "constitutive": roof_m["constitutive"]}},
         "cpu_baseline": cpu,
-        "faferr_codes_per_increment": [r["code"] for r in recs64],
-        "device": name,
+        "legs": legs,
+        "mixed_accuracy_one_system": acc,
+        "device": f"{name} ({cus} CUs)",
+        "note": ("value: point updates (points x Newton iterations) per second of whole-increment wall time on "
+                 "the FP64 path over the timed increments (their indices in legs.fp64.increments_timed); the "
+                 "relaxation factor is the library autotuner's (tuned once, in the warm-up; legs.*.sor_omega); "
+                 "legs.*_omega_1.6 time increment 2 at the configured omega 1.6.  Both paths continue past "
+                 "FAFERR 987 (points whose slip Newton ends above 0.5; counts per increment in legs.*), where "
+                 "the reference program stops."),
+        "bench_wall_s": time.time() - t_start,
     }
-    line.update(extra)
     if args.calibration > 0:
-        line["calibration"] = calibration_leg(capi, args.calibration, ctl, cpu=args.cpu_seconds > 0)
-    if cpu:
-        line["gp_rate_vs_cpu_reference"] = value / cpu["value"]
+        line["calibration"] = calibration_leg(capi, args.calibration, ctl)
+        line["bench_wall_s"] = time.time() - t_start
     print(json.dumps(line))
 
 
