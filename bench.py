@@ -113,8 +113,10 @@ def traced_increment(sv, inc, ctl):
     sv.reorient()
     after = sv.stats()
     keep = ("sor_omega", "sor_wpb", "pts_987_last")
-    rec = {k: after[k] - before[k] for k in after if k not in ("nit", "sor_sweeps", "err", "sdx", "sfc", "sumcpiter",
-                                                                "alp2", "sor_error", "pad_") + keep}
+    rec = {k: (after[k] - before[k]) if not isinstance(after[k], list) else
+           [a - b for a, b in zip(after[k], before[k])]
+           for k in after if k not in ("nit", "sor_sweeps", "err", "sdx", "sfc", "sumcpiter", "alp2", "sor_error",
+                                       "pad_") + keep}
     rec.update({k: after[k] for k in keep})
     rec.update(nit=nit, code=code, t_total_ms=1e3 * (time.perf_counter() - t0))
     return rec
@@ -178,6 +180,8 @@ def leg_summary(recs, wall, warm, first_inc):
         "sor_s_per_increment": t["t_sor_ms"] / n,
         "bcs_s_per_increment": t["t_suvals_ms"] / n,
         "update_s_per_increment": t["t_update_ms"] / n,
+        "constitutive_pass_s_per_increment": [sum(r["t_const_pass_ms"][q] for r in recs) * 1e-3 / n
+                                              for q in range(4)],
         "constitutive_flops_per_point_update": s["flops_const"] / max(s["gp_updates"], 1),
         "constitutive_transcendentals_per_point_update": s["transc_const"] / max(s["gp_updates"], 1),
         "sor_omega": recs[-1]["sor_omega"],

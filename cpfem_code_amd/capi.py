@@ -168,10 +168,12 @@ class CpfemStats(C.Structure):
                 ("t_sweep_ms", C.c_double), ("sweep_launches", C.c_int64),
                 ("sor_omega", C.c_double), ("t_tune_ms", C.c_double), ("sor_wpb", C.c_int32),
                 ("tune_sweeps", C.c_int32), ("pts_987", C.c_int64), ("pts_987_last", C.c_int32),
-                ("pad_", C.c_int32)]
+                ("pad_", C.c_int32), ("t_const_pass_ms", C.c_double * 4)]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["t_const_pass_ms"] = list(self.t_const_pass_ms)
+        return d
 
 
 _CTX = C.c_void_p

@@ -816,8 +816,20 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
     A.ovf_list = S.list_ovf;
     A.ovf_count = S.counts + 2;
     A.iter_limit = kFirstPassIters < C.maxit ? kFirstPassIters : C.maxit;
+    for (int q = 0; q < 4; ++q) S.pass_ms[q] = 0.0;
+    const bool timed = S.ev[0] != nullptr;
+    auto mark = [&](int k) {
+        if (timed) (void)hipEventRecord(S.ev[k], stream);
+    };
+    auto span = [&](int q, int a, int b) {
+        float ms = 0.f;
+        if (timed && hipEventElapsedTime(&ms, S.ev[a], S.ev[b]) == hipSuccess) S.pass_ms[q] += ms;
+    };
+    mark(0);
     launch_one<N, kAmax>(dmat, C, dt, A, A.n, stream);
+    mark(1);
     if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
+    span(0, 0, 1);
     counts[0] = S.host_counts[0];
     A.first = 0;
     // tier 1, pass 2: the unfinished points, compacted, up to maxit
@@ -828,8 +840,11 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
         A.cont_list = S.list_b;
         A.cont_count = S.counts + 1;
         A.iter_limit = C.maxit;
+        mark(1);
         launch_one<N, kAmax>(dmat, C, dt, A, S.host_counts[0], stream);
+        mark(2);
         if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
+        span(1, 1, 2);
     }
     counts[1] = S.host_counts[2];
     // tier 2: overflow of tier 1 (list_ovf) -> overflow into list_a
@@ -845,8 +860,11 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
         A.ovf_list = S.list_a;
         A.ovf_count = S.counts + 0;
         A.iter_limit = C.maxit;
+        mark(2);
         launch_one<N, kAmaxMid>(dmat, C, dt, A, S.host_counts[2], stream);
+        mark(3);
         if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
+        span(2, 2, 3);
         counts[2] = S.host_counts[0];
         // tier 3: overflow of tier 2 (list_a)
         if (S.host_counts[0] > 0) {
@@ -857,8 +875,11 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
             A.ovf_list = S.list_ovf;
             A.ovf_count = S.counts + 2;
             hipLaunchKernelGGL(k_set1, dim3(1), dim3(64), 0, stream, S.counts + 2, 0);
+            mark(3);
             launch_one<N, kAmaxWide>(dmat, C, dt, A, S.host_counts[0], stream);
+            mark(4);
             if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
+            span(3, 3, 4);
             counts[3] = S.host_counts[2];
             if constexpr (N == kNGeneric) {
                 // tier 4: overflow of tier 3 (list_ovf); materials with more than
@@ -872,8 +893,11 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
                     A.cont_count = S.counts + 1;
                     A.ovf_list = S.list_a;
                     A.ovf_count = S.counts + 0;
+                    mark(3);
                     launch_one<N, kAmaxGeneric>(dmat, C, dt, A, S.host_counts[2], stream);
+                    mark(4);
                     if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
+                    span(3, 3, 4);
                     if (S.host_counts[0] > 0) return hipErrorNotSupported;
                 }
             } else {
@@ -911,6 +935,8 @@ hipError_t alloc_schedule(ConstSchedule& S, int n) {
     if ((e = hipMalloc((void**)&S.gp, sizeof(double) * kConstMaxWidth * (size_t)n)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&S.counts, sizeof(int) * 4)) != hipSuccess) return e;
     if ((e = hipHostMalloc((void**)&S.host_counts, sizeof(int) * 4, 0)) != hipSuccess) return e;
+    for (int k = 0; k < 5; ++k)
+        if ((e = hipEventCreate(&S.ev[k])) != hipSuccess) return e;
     return hipSuccess;
 }
 
@@ -920,7 +946,9 @@ void free_schedule(ConstSchedule& S) {
     for (void* q : p)
         if (q) (void)hipFree(q);
     if (S.host_counts) (void)hipHostFree(S.host_counts);
-    memset(&S, 0, sizeof(S));
+    for (int k = 0; k < 5; ++k)
+        if (S.ev[k]) (void)hipEventDestroy(S.ev[k]);
+    S = ConstSchedule();
 }
 
 }  // namespace cpfem

@@ -28,6 +28,8 @@ struct ConstSchedule {
     double* gp = nullptr;      // [kConstMaxWidth][n] slips of the last hardening update
     int* counts = nullptr;     // [4] device
     int* host_counts = nullptr;// [4] pinned
+    hipEvent_t ev[5] = {};     // around the passes of the last run (timing)
+    double pass_ms[4] = {};    // device time of the last run: pass 1, pass 2, 8-wide, 12(+)-wide tiers
 };
 
 DevControl make_control(const cpfem_control& c);

@@ -1374,6 +1374,10 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
     } while (0)
     {
         TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        {
+            const char* g = getenv("CPFEM_SOR_GRAPHS");
+            c->use_graphs = !(g && g[0] == '0');
+        }
         for (int i = 0; i < Ctx::kEvents; ++i) TRY(hipEventCreate(&c->ev[i]));
         const size_t G = c->ngps, D = 3 * (size_t)c->nnod, nsv = c->nsv;
         TRY(dalloc(c, &c->elem_nodes, en.size()));
@@ -1464,13 +1468,14 @@ fail:
 #undef TRY
 }
 
+static void drop_graphs(Ctx* c);
+
 extern "C" int cpfem_destroy(cpfem_ctx* c) {
     if (!c) return 0;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     calib_free(c);
     free_schedule(c->sched);
-    for (int i = 0; i < 3; ++i)
-        if (c->graphs[i]) (void)hipGraphExecDestroy(c->graphs[i]);
+    drop_graphs(c);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->red.host) (void)hipHostFree(c->red.host);
     for (int i = 0; i < Ctx::kEvents; ++i)
@@ -1615,6 +1620,7 @@ static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
     }
     c->stats.pts_continued += counts[0];
     c->stats.pts_wide += counts[1];
+    for (int q = 0; q < 4; ++q) c->stats.t_const_pass_ms[q] += c->sched.pass_ms[q];
     CPFEM_HIP(hipEventRecord(c->ev[1], s));
     // zero fc and the matrix values
     const size_t D = 3 * (size_t)c->nnod, nv = c->A.nvals();
@@ -1801,16 +1807,9 @@ static void round_sweep_matrix(Ctx* c) {
     c->vals32_fresh = true;
 }
 
-// A batch of sweeps captured once as a graph and replayed (kernel nodes only).
-static hipError_t sweep_graph(Ctx* c, int which, hipGraphExec_t* exec_out) {
-    if (c->graphs[which]) {
-        *exec_out = c->graphs[which];
-        return hipSuccess;
-    }
+// The kernels of one batch of kSweepBatch sweeps, enqueued on the context's stream.
+static void enqueue_batch(Ctx* c, int which) {
     hipStream_t s = c->stream;
-    const int D = 3 * c->nnod;
-    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) return e;
     if (which == 0) {
         // FP64 sweeps with the relative-update test after each one
         const double tol2 = c->ctl.sor_tol * c->ctl.sor_tol;
@@ -1828,18 +1827,46 @@ static hipError_t sweep_graph(Ctx* c, int which, hipGraphExec_t* exec_out) {
         // FP32 sweeps on the correction equation A d = r (rhs32, del32); the
         // caller forms r before the first batch of a refinement cycle and adds
         // d to ddx after the last
-        (void)D;
         for (int k = 0; k < kSweepBatch; ++k)
             sor_sweep<float, float, false>(c, sweep_vals32(c), c->rhs32, c->del32, c->red.partial);
     }
+}
+
+// A batch of sweeps captured once as a graph and replayed (kernel nodes only).
+// The captured graph is kept until the executable graph is dropped.
+static hipError_t sweep_graph(Ctx* c, int which, hipGraphExec_t* exec_out) {
+    if (c->graphs[which]) {
+        *exec_out = c->graphs[which];
+        return hipSuccess;
+    }
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    enqueue_batch(c, which);
     hipGraph_t g = nullptr;
-    e = hipStreamEndCapture(s, &g);
+    e = hipStreamEndCapture(c->stream, &g);
     if (e != hipSuccess) return e;
     e = hipGraphInstantiate(&c->graphs[which], g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return e;
+    }
+    c->graph_src[which] = g;
     *exec_out = c->graphs[which];
     return hipSuccess;
+}
+
+// One batch of sweeps: a replay of its graph, or (graphs switched off with
+// CPFEM_SOR_GRAPHS=0, e.g. under a tracing profiler) the same kernels
+// launched one by one.
+static hipError_t run_batch(Ctx* c, int which) {
+    if (!c->use_graphs) {
+        enqueue_batch(c, which);
+        return hipGetLastError();
+    }
+    hipGraphExec_t gx;
+    hipError_t e = sweep_graph(c, which, &gx);
+    if (e != hipSuccess) return e;
+    return hipGraphLaunch(gx, c->stream);
 }
 
 static double residual(Ctx* c, const double* b, const double* y, double* res) {
@@ -1896,12 +1923,10 @@ static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out, bool warm = fal
     if (c->precision == 0 && c->ctl.sor_kind == 0) {
         // the reference's relative-update test after every sweep, decided on the device
         const double tol2 = c->ctl.sor_tol * c->ctl.sor_tol;
-        hipGraphExec_t gx;
-        CPFEM_HIP(sweep_graph(c, 0, &gx));
         int hf[2] = {0, 0};
         while (true) {
             CPFEM_HIP(hipEventRecord(c->ev[7], s));
-            CPFEM_HIP(hipGraphLaunch(gx, s));
+            CPFEM_HIP(run_batch(c, 0));
             CPFEM_HIP(hipEventRecord(c->ev[8], s));
             CPFEM_HIP(hipMemcpyAsync(c->red.host + 4, c->flags + 2, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
             CPFEM_HIP(hipStreamSynchronize(s));
@@ -1925,8 +1950,7 @@ static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out, bool warm = fal
         double rel = 1.0, prev_rel = -1.0;
         int prev_sweeps = 0, batches = 1;
         if (mixed) CPFEM_HIP(hipMemcpyAsync(c->xold, c->rhs, D * sizeof(double), hipMemcpyDeviceToDevice, s));
-        hipGraphExec_t gx;
-        CPFEM_HIP(sweep_graph(c, mixed ? 2 : 1, &gx));
+        const int which = mixed ? 2 : 1;
         while (sweeps < maxit) {
             batches = std::max(1, std::min(batches, (maxit - sweeps) / kSweepBatch));
             if (mixed) {
@@ -1935,7 +1959,7 @@ static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out, bool warm = fal
                 hipLaunchKernelGGL(k_fill<float>, dim3(nblk(D)), dim3(kBlock), 0, s, c->del32, (size_t)D, 0.0f);
             }
             CPFEM_HIP(hipEventRecord(c->ev[7], s));
-            for (int k = 0; k < batches; ++k) CPFEM_HIP(hipGraphLaunch(gx, s));
+            for (int k = 0; k < batches; ++k) CPFEM_HIP(run_batch(c, which));
             CPFEM_HIP(hipEventRecord(c->ev[8], s));
             if (mixed)
                 hipLaunchKernelGGL(k_add_float, dim3(nblk(D)), dim3(kBlock), 0, s, c->ddx, (const float*)c->del32, D);
@@ -1982,11 +2006,16 @@ static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out, bool warm = fal
 // past the best candidate (the sweep count is unimodal in omega for SOR on an
 // SPD system).
 static void drop_graphs(Ctx* c) {
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < 3; ++i) {
         if (c->graphs[i]) {
             (void)hipGraphExecDestroy(c->graphs[i]);
             c->graphs[i] = nullptr;
         }
+        if (c->graph_src[i]) {
+            (void)hipGraphDestroy(c->graph_src[i]);
+            c->graph_src[i] = nullptr;
+        }
+    }
 }
 
 // Graph the solve replays for this context's settings.
@@ -1997,9 +2026,7 @@ static hipError_t time_sweep_graph(Ctx* c, int reps, float* ms) {
     hipStream_t s = c->stream;
     const int D = 3 * c->nnod;
     const int which = solve_graph_kind(c);
-    hipGraphExec_t gx;
-    hipError_t e = sweep_graph(c, which, &gx);
-    if (e != hipSuccess) return e;
+    hipError_t e;
     float best = 1e30f;
     for (int rep = 0; rep < 3; ++rep) {
         if (which == 2) {
@@ -2011,7 +2038,7 @@ static hipError_t time_sweep_graph(Ctx* c, int reps, float* ms) {
         hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, c->flags + 2, (size_t)2, 0);
         if ((e = hipEventRecord(c->ev[7], s)) != hipSuccess) return e;
         for (int k = 0; k < reps; ++k)
-            if ((e = hipGraphLaunch(gx, s)) != hipSuccess) return e;
+            if ((e = run_batch(c, which)) != hipSuccess) return e;
         if ((e = hipEventRecord(c->ev[8], s)) != hipSuccess) return e;
         if ((e = hipEventSynchronize(c->ev[8])) != hipSuccess) return e;
         if (rep > 0) best = std::min(best, elapsed(c->ev[7], c->ev[8]));   // the first is a warm-up
@@ -2026,15 +2053,13 @@ static hipError_t tune_window(Ctx* c, int W, double* rel) {
     hipStream_t s = c->stream;
     const int D = 3 * c->nnod;
     const bool mixed = c->precision != 0;
-    hipGraphExec_t gx;
-    hipError_t e = sweep_graph(c, mixed ? 2 : 1, &gx);
-    if (e != hipSuccess) return e;
+    hipError_t e;
     if (mixed) {
         hipLaunchKernelGGL(k_to_float, dim3(nblk(D)), dim3(kBlock), 0, s, (const double*)c->xold, c->rhs32, D);
         hipLaunchKernelGGL(k_fill<float>, dim3(nblk(D)), dim3(kBlock), 0, s, c->del32, (size_t)D, 0.0f);
     }
     for (int k = 0; k < W / kSweepBatch; ++k)
-        if ((e = hipGraphLaunch(gx, s)) != hipSuccess) return e;
+        if ((e = run_batch(c, mixed ? 2 : 1)) != hipSuccess) return e;
     if (mixed) hipLaunchKernelGGL(k_add_float, dim3(nblk(D)), dim3(kBlock), 0, s, c->ddx, (const float*)c->del32, D);
     *rel = residual(c, c->rhs, c->ddx, mixed ? c->xold : nullptr);
     return hipGetLastError();

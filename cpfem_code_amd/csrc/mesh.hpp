@@ -122,6 +122,8 @@ struct Ctx {
     Reducer red;
     ConstSchedule sched;
     hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};  // sweep batches: FP64 update test, FP64, FP32
+    hipGraph_t graph_src[3] = {nullptr, nullptr, nullptr};   // their captured graphs
+    bool use_graphs = true;      // CPFEM_SOR_GRAPHS=0: launch the batches kernel by kernel
     int sor_wpb = 4;             // waves per workgroup of the colour sweeps (cpfem_autotune)
     bool omega_auto = false;     // sor_omega <= 0 at creation: tuned at the first solve
     bool assembled = false;      // A and rhs hold a system (set by SUVALS)

@@ -50,6 +50,7 @@ MODULE CPFEM_CAPI
         INTEGER(C_INT32_T) :: sor_wpb, tune_sweeps
         INTEGER(C_INT64_T) :: pts_987
         INTEGER(C_INT32_T) :: pts_987_last, pad_
+        REAL(C_DOUBLE) :: t_const_pass_ms(4)
     END TYPE cpfem_stats
 
     INTERFACE

@@ -154,6 +154,9 @@ typedef struct {
                                  crys-cons.f90:108), summed over the ELVALS calls */
     int32_t pts_987_last;     /* such points in the last ELVALS (for NRITER: its final iteration) */
     int32_t pad_;
+    double t_const_pass_ms[4];/* device time of the constitutive passes: every point's first outer iterations,
+                                 the continuation of the unfinished ones, the 8-wide and the 12-wide (or wider)
+                                 active-set tiers */
 } cpfem_stats;
 
 /* Upload mesh, materials and control; build the block sparsity, colourings

@@ -127,7 +127,7 @@ def test_hcp_twinprob_below_one(epoch):
     import dataclasses
     p = 0.3
     mat = dataclasses.replace(hcp_magnesium(), twinprob=p, twinlim=1)
-    b = make_gp_batch(mat, 1500, seed=11, stress=60.0)
+    b = make_gp_batch(mat, 1500, seed=11, stress=30.0)
     g, o = compare_conditioned(mat, b, epoch=epoch)
     checks = int(o["draws"].sum())
     twinned = int((g["twinsys"] != 0).sum())
