@@ -89,6 +89,7 @@ def lib() -> C.CDLL:
                                          _P, _P]
         L.cpfem_device_info.argtypes = [C.c_char_p, C.c_int32, _P]
         L.cpfem_twin_stream.argtypes = [C.c_void_p, C.c_uint64]
+        L.cpfem_dstrss_batch_noalloc.argtypes = L.cpfem_dstrss_batch.argtypes
         _lib = L
     return _lib
 
@@ -112,9 +113,10 @@ def twin_stream(epoch: int) -> None:
 
 
 def dstrss_batch(material, control, dt, iel, de, r, st, ds, sv, dsv, twinsys=None, timestwinned=None,
-                 raise_on_error=False, epoch=None) -> Dict[str, np.ndarray]:
+                 raise_on_error=False, epoch=None, noalloc=False) -> Dict[str, np.ndarray]:
     """Batched DSTRSS on the GPU (host arrays in the reference's per-point layout).
-    epoch: stream epoch of this call's twin draws (None: the next of the library's counter)."""
+    epoch: stream epoch of this call's twin draws (None: the next of the library's counter).
+    noalloc: use cpfem_dstrss_batch_noalloc (no device allocation; at most 8192 points)."""
     if epoch is not None:
         twin_stream(epoch)
     n = len(iel)
@@ -137,7 +139,8 @@ def dstrss_batch(material, control, dt, iel, de, r, st, ds, sv, dsv, twinsys=Non
     sumcp = np.zeros(1)
     flops = np.zeros(n, np.int64)
     err = np.zeros(n, np.int32)
-    code = lib().cpfem_dstrss_batch(C.byref(ms), C.byref(cs), float(dt), n, _ptr(iel), _ptr(de), _ptr(r), _ptr(st),
+    fn = lib().cpfem_dstrss_batch_noalloc if noalloc else lib().cpfem_dstrss_batch
+    code = fn(C.byref(ms), C.byref(cs), float(dt), n, _ptr(iel), _ptr(de), _ptr(r), _ptr(st),
                                     _ptr(ds), _ptr(sv), _ptr(dsv), sv.shape[1], _ptr(t), _ptr(chi), _ptr(deff),
                                     _ptr(srate), _ptr(tw), _ptr(tt), _ptr(tflag), _ptr(sumcp), _ptr(flops), _ptr(err))
     if code < 0:

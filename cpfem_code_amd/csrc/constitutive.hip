@@ -3,6 +3,7 @@
 #include <string.h>
 
 #include <atomic>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -1096,4 +1097,240 @@ extern "C" int cpfem_device_info(char* name, int32_t len, int32_t* cus) {
     }
     if (cus) *cus = p.multiProcessorCount;
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant of cpfem_dstrss_batch.  Every device buffer it
+// needs (the staging of the caller's host arrays, the field-major point
+// arrays, the material record and the pass schedule) is carved from one
+// file-scope device array, `scratch`; each GPU holds its own copy, whose
+// address is looked up once per device.  Calls on one GPU share it, so calls
+// that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int32_t kNoAllocMaxPoints = 8192;
+constexpr size_t kScratchBytes = size_t(40) << 20;
+constexpr int kMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static unsigned char* g_scratch_ptr[kMaxDevices];      // device address of `scratch` on each GPU
+static int g_noalloc_counts[kMaxDevices][4];            // host side of the pass counters
+static std::mutex g_scratch_mu;
+
+// soa[i*n + g] = aos[g*ld + i] for i < k (field-major copy of a per-point array)
+__global__ void k_gather_fields(const double* __restrict__ aos, int ld, int k, int n, double* __restrict__ soa) {
+    const size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (idx < (size_t)k * n) {
+        const size_t g = idx % (size_t)n, i = idx / (size_t)n;
+        soa[idx] = aos[g * (size_t)ld + i];
+    }
+}
+
+// aos[g*ld + i] = soa[i*n + g] for i < k (entries i >= k of aos are left as they are)
+__global__ void k_scatter_fields(const double* __restrict__ soa, int k, int n, double* __restrict__ aos, int ld) {
+    const size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (idx < (size_t)k * n) {
+        const size_t g = idx % (size_t)n, i = idx / (size_t)n;
+        aos[g * (size_t)ld + i] = soa[idx];
+    }
+}
+
+// sumcpiter + sum of erepmd in point order (the order of the host sum of
+// cpfem_dstrss_batch, so the two entry points give the same bits)
+__global__ void k_ordered_sum(const double* __restrict__ v, int n, double start, double* out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double s = start;
+        for (int g = 0; g < n; ++g) s += v[g];
+        *out = s;
+    }
+}
+
+// Carves 256-byte-aligned pieces out of the scratch array.
+struct Carver {
+    unsigned char* base;
+    size_t off = 0;
+    template <typename T>
+    T* take(size_t count) {
+        off = (off + 255) & ~size_t(255);
+        T* p = reinterpret_cast<T*>(base + off);
+        off += count * sizeof(T);
+        return p;
+    }
+};
+
+static int scratch_for_device(unsigned char** out, int* dev_out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return CPFEM_E_NOGPU;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fprintf(stderr, "cpfem: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+            return CPFEM_E_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    *dev_out = dev;
+    return 0;
+}
+
+extern "C" int cpfem_dstrss_batch_noalloc(const cpfem_material* mat, const cpfem_control* ctl, double dt, int32_t n,
+                                          const int32_t* iel, const double* de, const double* r, const double* st,
+                                          double* ds, const double* sv, double* dsv, int32_t ldsv, double* t,
+                                          double* chi, double* deff, double* srate, int32_t* twinsys,
+                                          const int32_t* timestwinned, int32_t* tflag, double* sumcpiter,
+                                          int64_t* flops, int32_t* err) {
+    if (!mat || !ctl || n < 0 || ldsv < 3 + mat->nss || ldsv > CPFEM_MAXSV) return CPFEM_E_ARG;
+    if (n > kNoAllocMaxPoints) {
+        fprintf(stderr, "cpfem_dstrss_batch_noalloc: %d points exceed the scratch limit of %d\n", n,
+                kNoAllocMaxPoints);
+        return CPFEM_E_ARG;
+    }
+    const int nss = mat->nss;
+    const size_t mbytes = material_bytes(nss);
+    if (mbytes == 0) return CPFEM_E_NSS;
+    if (n == 0) return 0;
+    for (int g = 0; g < n; ++g)
+        if (iel[g] != 1 && iel[g] != -1) return CPFEM_E_ARG;  // one material set per batch
+    unsigned char* base = nullptr;
+    int dev = 0;
+    if (const int rc = scratch_for_device(&base, &dev)) return rc;
+    const size_t N = (size_t)n;
+    const int nsv = 3 + nss;
+    Carver cv{base};
+    // staging of the caller's per-point arrays in their own layout
+    double* s_de = cv.take<double>(6 * N);
+    double* s_r = cv.take<double>(3 * N);
+    double* s_st = cv.take<double>(6 * N);
+    double* s_ds = cv.take<double>(6 * N);
+    double* s_sv = cv.take<double>((size_t)ldsv * N);
+    double* s_dsv = cv.take<double>((size_t)ldsv * N);
+    double* s_srate = cv.take<double>((size_t)CPFEM_MAXSS * N);
+    double* s_t = cv.take<double>(6 * N);
+    double* s_chi = cv.take<double>(36 * N);
+    // field-major point arrays the kernels read and write
+    double* f_de = cv.take<double>(6 * N);
+    double* f_r = cv.take<double>(3 * N);
+    double* f_st = cv.take<double>(6 * N);
+    double* f_ds = cv.take<double>(6 * N);
+    double* f_t = cv.take<double>(6 * N);
+    double* f_chi = cv.take<double>(36 * N);
+    double* f_deff = cv.take<double>(N);
+    double* f_rep = cv.take<double>(N);
+    double* f_sv = cv.take<double>((size_t)nsv * N);
+    double* f_dsv = cv.take<double>((size_t)nsv * N);
+    double* f_srate = cv.take<double>((size_t)nss * N);
+    int32_t* f_int = cv.take<int32_t>(4 * N + 2);          // iel, twinsys, timestwinned, err, tflag, errmax
+    long long* f_l = cv.take<long long>(2 * N);            // flops, transc
+    double* f_sum = cv.take<double>(1);
+    char* f_mat = cv.take<char>(mbytes);
+    ConstSchedule S;
+    S.ck_s = cv.take<double>(6 * N);
+    S.ck_iter = cv.take<int>(N);
+    S.ck_kerr = cv.take<int>(N);
+    S.ck_inner = cv.take<int>(N);
+    S.sorted = cv.take<int>(N);
+    S.bins = cv.take<int>(1024);
+    S.list_a = cv.take<int>(N);
+    S.list_b = cv.take<int>(N);
+    S.list_ovf = cv.take<int>(N);
+    S.el = cv.take<double>(21 * N);
+    S.gp = cv.take<double>((size_t)kConstMaxWidth * N);
+    S.counts = cv.take<int>(4);
+    S.host_counts = g_noalloc_counts[dev];
+    if (cv.off > kScratchBytes) return CPFEM_E_ARG;   // cannot happen within kNoAllocMaxPoints
+    hipStream_t s0 = 0;
+    // material record, built in host stack memory
+    alignas(16) unsigned char hm[sizeof(DevMaterial<kNGeneric>)];
+    build_material(nss, hm, *mat, ctl->toler);
+    CPFEM_HIP(hipMemcpy(f_mat, hm, mbytes, hipMemcpyHostToDevice));
+    // inputs
+    CPFEM_HIP(hipMemcpy(s_de, de, 6 * N * sizeof(double), hipMemcpyHostToDevice));
+    CPFEM_HIP(hipMemcpy(s_r, r, 3 * N * sizeof(double), hipMemcpyHostToDevice));
+    CPFEM_HIP(hipMemcpy(s_st, st, 6 * N * sizeof(double), hipMemcpyHostToDevice));
+    CPFEM_HIP(hipMemcpy(s_ds, ds, 6 * N * sizeof(double), hipMemcpyHostToDevice));
+    CPFEM_HIP(hipMemcpy(s_sv, sv, (size_t)ldsv * N * sizeof(double), hipMemcpyHostToDevice));
+    CPFEM_HIP(hipMemcpy(s_dsv, dsv, (size_t)ldsv * N * sizeof(double), hipMemcpyHostToDevice));
+    CPFEM_HIP(hipMemcpy(s_srate, srate, (size_t)CPFEM_MAXSS * N * sizeof(double), hipMemcpyHostToDevice));
+    CPFEM_HIP(hipMemcpy(f_int, iel, N * sizeof(int32_t), hipMemcpyHostToDevice));
+    CPFEM_HIP(hipMemcpy(f_int + N, twinsys, N * sizeof(int32_t), hipMemcpyHostToDevice));
+    CPFEM_HIP(hipMemcpy(f_int + 2 * N, timestwinned, N * sizeof(int32_t), hipMemcpyHostToDevice));
+    const int32_t flags0[2] = {*tflag, 0};
+    CPFEM_HIP(hipMemcpy(f_int + 4 * N, flags0, 2 * sizeof(int32_t), hipMemcpyHostToDevice));
+    auto gather = [&](const double* aos, int ld, int k, double* soa) {
+        const size_t tot = (size_t)k * N;
+        hipLaunchKernelGGL(k_gather_fields, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s0, aos, ld, k, n, soa);
+    };
+    auto scatter = [&](const double* soa, int k, double* aos, int ld) {
+        const size_t tot = (size_t)k * N;
+        hipLaunchKernelGGL(k_scatter_fields, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s0, soa, k, n, aos,
+                           ld);
+    };
+    gather(s_de, 6, 6, f_de);
+    gather(s_r, 3, 3, f_r);
+    gather(s_st, 6, 6, f_st);
+    gather(s_ds, 6, 6, f_ds);
+    gather(s_sv, ldsv, nsv, f_sv);
+    gather(s_dsv, ldsv, nsv, f_dsv);
+    gather(s_srate, CPFEM_MAXSS, nss, f_srate);
+    // the err array and the counters start at zero (the scratch keeps the last call's contents)
+    hipLaunchKernelGGL(k_zero_n, dim3((n + 255) / 256), dim3(256), 0, s0, f_int + 3 * N, n);
+    CPFEM_HIP(hipGetLastError());
+    ConstArgs A;
+    memset(&A, 0, sizeof(A));
+    A.n = n;
+    A.mesh = 0;
+    A.iel = f_int;
+    A.twinsys = f_int + N;
+    A.timestwinned = f_int + 2 * N;
+    A.err = f_int + 3 * N;
+    A.tflag = f_int + 4 * N;
+    A.errmax = f_int + 4 * N + 1;
+    A.de = f_de;
+    A.r = f_r;
+    A.st = f_st;
+    A.ds = f_ds;
+    A.t = f_t;
+    A.chi = f_chi;
+    A.dep = f_deff;
+    A.erepmd = f_rep;
+    A.sv = f_sv;
+    A.dsv = f_dsv;
+    A.srate = f_srate;
+    A.flops = f_l;
+    A.transc = f_l + N;
+    A.rng_epoch = g_batch_epoch.fetch_add(1);
+    DevControl C = make_control(*ctl);
+    int counts[4];
+    hipError_t he = run_constitutive(nss, f_mat, C, dt, A, S, s0, counts);
+    if (he == hipErrorNotSupported) return CPFEM_E_WIDTH;
+    if (he != hipSuccess) return CPFEM_E_HIP;
+    // outputs back into the staging (caller's layout), then to the host
+    scatter(f_ds, 6, s_ds, 6);
+    scatter(f_t, 6, s_t, 6);
+    scatter(f_chi, 36, s_chi, 36);
+    scatter(f_dsv, nsv, s_dsv, ldsv);
+    scatter(f_srate, nss, s_srate, CPFEM_MAXSS);
+    hipLaunchKernelGGL(k_ordered_sum, dim3(1), dim3(64), 0, s0, (const double*)f_rep, n,
+                       sumcpiter ? *sumcpiter : 0.0, f_sum);
+    CPFEM_HIP(hipGetLastError());
+    CPFEM_HIP(hipDeviceSynchronize());
+    CPFEM_HIP(hipMemcpy(ds, s_ds, 6 * N * sizeof(double), hipMemcpyDeviceToHost));
+    CPFEM_HIP(hipMemcpy(t, s_t, 6 * N * sizeof(double), hipMemcpyDeviceToHost));
+    CPFEM_HIP(hipMemcpy(chi, s_chi, 36 * N * sizeof(double), hipMemcpyDeviceToHost));
+    CPFEM_HIP(hipMemcpy(deff, f_deff, N * sizeof(double), hipMemcpyDeviceToHost));
+    CPFEM_HIP(hipMemcpy(dsv, s_dsv, (size_t)ldsv * N * sizeof(double), hipMemcpyDeviceToHost));
+    CPFEM_HIP(hipMemcpy(srate, s_srate, (size_t)CPFEM_MAXSS * N * sizeof(double), hipMemcpyDeviceToHost));
+    CPFEM_HIP(hipMemcpy(twinsys, f_int + N, N * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (err) CPFEM_HIP(hipMemcpy(err, f_int + 3 * N, N * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (flops) CPFEM_HIP(hipMemcpy(flops, f_l, N * sizeof(int64_t), hipMemcpyDeviceToHost));
+    int32_t flags[2];
+    CPFEM_HIP(hipMemcpy(flags, f_int + 4 * N, 2 * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (sumcpiter) CPFEM_HIP(hipMemcpy(sumcpiter, f_sum, sizeof(double), hipMemcpyDeviceToHost));
+    if (flags[0]) *tflag = 1;
+    return flags[1];
 }

@@ -100,6 +100,25 @@ int cpfem_dstrss_batch(const cpfem_material *mat, const cpfem_control *ctl, doub
                        int32_t *tflag, double *sumcpiter, int64_t *flops, int32_t *err);
 
 
+/*
+ * cpfem_dstrss_batch without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and error codes as cpfem_dstrss_batch (host pointers),
+ * but the staging, the field-major point arrays, the material record and the
+ * pass schedule live in one file-scope device array of 40 MiB per GPU, held
+ * until the process ends; batches of more than 8192 points are rejected with
+ * CPFEM_E_ARG.  It calls no hipMalloc/hipFree and no allocating routine.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * race.
+ */
+int cpfem_dstrss_batch_noalloc(const cpfem_material *mat, const cpfem_control *ctl, double dt, int32_t n,
+                               const int32_t *iel, const double *de, const double *r, const double *st,
+                               double *ds, const double *sv, double *dsv, int32_t ldsv,
+                               double *t, double *chi, double *deff, double *srate,
+                               int32_t *twinsys, const int32_t *timestwinned,
+                               int32_t *tflag, double *sumcpiter, int64_t *flops, int32_t *err);
+
 /* ---------------------------------------------------------------------- */
 /* Device-resident problem: the NRITER call sequence (Stif_sol.f90:10-88)  */
 /* ---------------------------------------------------------------------- */
