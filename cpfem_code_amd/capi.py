@@ -161,6 +161,30 @@ class CpfemMesh(C.Structure):
                 ("tiem", _P), ("x_spec", _P), ("f_spec", _P)]
 
 
+class CpfemMaterialsR4(C.Structure):
+    """The FAS_CPL material arrays (REAL(4), leading dimension maxss = 60) of cpfem_create_r4."""
+    _fields_ = [("nmatl", C.c_int32), ("twinlim", C.c_int32), ("twinprob", C.c_float), ("nss", _P),
+                ("e", _P), ("ssang", _P), ("ab", _P), ("an", _P), ("ampar", _P), ("smatrx", _P)]
+
+
+def materials_r4(materials):
+    """Module-layout REAL(4) arrays of the material sets (C order: [set][system][component])."""
+    nm = 12
+    a = dict(nss=np.zeros(nm, np.int32), e=np.zeros((nm, 9), np.float32), ssang=np.zeros((nm, MAXSS), np.float32),
+             ab=np.zeros((nm, MAXSS, 3), np.float32), an=np.zeros((nm, MAXSS, 3), np.float32),
+             ampar=np.zeros((nm, MAXSS, 10), np.float32), smatrx=np.zeros((nm, MAXSS, MAXSS), np.float32))
+    for k, m in enumerate(materials):
+        n = int(m.nss)
+        a["nss"][k] = n
+        a["e"][k] = m.e[:9]
+        a["ssang"][k, :n] = m.ssang[:n]
+        a["ab"][k, :n] = m.ab[:n]
+        a["an"][k, :n] = m.an[:n]
+        a["ampar"][k, :n] = m.ampar[:n, :10]
+        a["smatrx"][k, :n, :n] = m.smatrx[:n, :n]
+    return a
+
+
 class CpfemStats(C.Structure):
     _fields_ = [("nit", C.c_int32), ("sor_sweeps", C.c_int32), ("sor_sweeps_total", C.c_int32), ("err", C.c_int32),
                 ("sdx", C.c_double), ("sfc", C.c_double), ("sumcpiter", C.c_double), ("alp2", C.c_double),
@@ -188,6 +212,10 @@ def _bind_mesh_api(L):
     L.cpfem_destroy.argtypes = [_CTX]
     L.cpfem_set_state.argtypes = [_CTX] + [_P] * 6 + [C.c_int32] + [_P] * 8
     L.cpfem_get_state.argtypes = [_CTX] + [_P] * 7 + [C.c_int32] + [_P] * 10
+    L.cpfem_create_r4.argtypes = [C.POINTER(CpfemMesh), C.POINTER(CpfemMaterialsR4), C.POINTER(CpfemControl),
+                                  C.POINTER(_CTX)]
+    L.cpfem_set_state_r4.argtypes = L.cpfem_set_state.argtypes
+    L.cpfem_get_state_r4.argtypes = L.cpfem_get_state.argtypes
     L.cpfem_update_node.argtypes = [_CTX]
     L.cpfem_elvals.argtypes = [_CTX, C.c_int32, _P, _P]
     L.cpfem_suvals.argtypes = [_CTX, C.c_int32, _P]
@@ -254,7 +282,10 @@ class Solver:
     UPDATE_GPTV and DoReorient (Fasolt3.f90:38-46).
     """
 
-    def __init__(self, prob, precision: int = 0, control=None):
+    def __init__(self, prob, precision: int = 0, control=None, real4: bool = False):
+        """real4: create and fill the context through the REAL(4) entry points
+        (cpfem_create_r4, cpfem_set_state_r4) from single-precision copies of
+        the problem's arrays, as the reference's module arrays are."""
         L = lib()
         if not hasattr(L, "_mesh_bound"):
             _bind_mesh_api(L)
@@ -279,11 +310,25 @@ class Solver:
         m.x_spec = arr("x_spec", p.x_spec, np.float64)
         m.f_spec = arr("f_spec", p.f_spec, np.float64)
         self._mesh = m
-        self._mats = (CpfemMaterial * len(p.materials))(*[material_struct(mt) for mt in p.materials])
         self._ctl = control_struct(control or p.control, precision)
         h = _CTX()
-        _check(L.cpfem_create(C.byref(m), self._mats, len(p.materials), C.byref(self._ctl), C.byref(h)),
-               "cpfem_create")
+        self.real4 = real4
+        if real4:
+            m.tiem = arr("tiem4", p.tiem, np.float32)
+            m.x_spec = arr("x_spec4", p.x_spec, np.float32)
+            m.f_spec = arr("f_spec4", p.f_spec, np.float32)
+            mt = materials_r4(p.materials)
+            t = CpfemMaterialsR4()
+            t.nmatl, t.twinlim, t.twinprob = len(p.materials), int(p.materials[-1].twinlim), \
+                float(p.materials[-1].twinprob)
+            for name in ("nss", "e", "ssang", "ab", "an", "ampar", "smatrx"):
+                setattr(t, name, arr("m4_" + name, mt[name], mt[name].dtype))
+            self._mats = t
+            _check(L.cpfem_create_r4(C.byref(m), C.byref(t), C.byref(self._ctl), C.byref(h)), "cpfem_create_r4")
+        else:
+            self._mats = (CpfemMaterial * len(p.materials))(*[material_struct(mt) for mt in p.materials])
+            _check(L.cpfem_create(C.byref(m), self._mats, len(p.materials), C.byref(self._ctl), C.byref(h)),
+                   "cpfem_create")
         self.h = h
         st = p.zero_state()
         st["x"], st["dx"] = p.x, p.dx
@@ -301,25 +346,32 @@ class Solver:
             pass
 
     def set_state(self, s):
-        f = lambda k, dt=np.float64: None if k not in s else np.ascontiguousarray(s[k], dt)  # noqa: E731
+        ft = np.float32 if self.real4 else np.float64
+        f = lambda k, dt=ft: None if k not in s else np.ascontiguousarray(s[k], dt)  # noqa: E731
         a = {k: f(k) for k in ("x", "dx", "eps", "dep", "sv", "dsv", "ep", "de", "st", "ds", "rt", "dr")}
         a["twinsys"] = f("twinsys", np.int32)
         a["timestwinned"] = f("timestwinned", np.int32)
         ldsv = a["sv"].shape[1] if a["sv"] is not None else MAXSV
-        _check(self.L.cpfem_set_state(self.h, _ptr(a["x"]), _ptr(a["dx"]), _ptr(a["eps"]), _ptr(a["dep"]),
+        fn = self.L.cpfem_set_state_r4 if self.real4 else self.L.cpfem_set_state
+        _check(fn(self.h, _ptr(a["x"]), _ptr(a["dx"]), _ptr(a["eps"]), _ptr(a["dep"]),
                                       _ptr(a["sv"]), _ptr(a["dsv"]), ldsv, _ptr(a["ep"]), _ptr(a["de"]),
                                       _ptr(a["st"]), _ptr(a["ds"]), _ptr(a["rt"]), _ptr(a["dr"]),
                                       _ptr(a["twinsys"]), _ptr(a["timestwinned"])), "cpfem_set_state")
 
-    def get_state(self):
+    def get_state(self, real4=None):
+        """The state in the reference's per-point layout; real4 (default: as
+        the context was created) reads it through cpfem_get_state_r4 into
+        single-precision arrays."""
+        real4 = self.real4 if real4 is None else real4
+        ft = np.float32 if real4 else np.float64
         p = self.prob
         g, d = p.ngps, p.ndof
-        s = dict(x=np.zeros(d), dx=np.zeros(d), fc=np.zeros(d), eps=np.zeros(g), dep=np.zeros(g),
-                 sv=np.zeros((g, MAXSV)), dsv=np.zeros((g, MAXSV)), ep=np.zeros((g, 6)), de=np.zeros((g, 6)),
-                 st=np.zeros((g, 6)), ds=np.zeros((g, 6)), rt=np.zeros((g, 3)), dr=np.zeros((g, 3)),
-                 srate=np.zeros((g, MAXSS)), cumslip=np.zeros((g, MAXSS)),
+        z = lambda *shape: np.zeros(shape, ft)  # noqa: E731
+        s = dict(x=z(d), dx=z(d), fc=z(d), eps=z(g), dep=z(g), sv=z(g, MAXSV), dsv=z(g, MAXSV), ep=z(g, 6),
+                 de=z(g, 6), st=z(g, 6), ds=z(g, 6), rt=z(g, 3), dr=z(g, 3), srate=z(g, MAXSS), cumslip=z(g, MAXSS),
                  twinsys=np.zeros(g, np.int32), timestwinned=np.zeros(g, np.int32))
-        _check(self.L.cpfem_get_state(self.h, _ptr(s["x"]), _ptr(s["dx"]), _ptr(s["fc"]), _ptr(s["eps"]),
+        fn = self.L.cpfem_get_state_r4 if real4 else self.L.cpfem_get_state
+        _check(fn(self.h, _ptr(s["x"]), _ptr(s["dx"]), _ptr(s["fc"]), _ptr(s["eps"]),
                                       _ptr(s["dep"]), _ptr(s["sv"]), _ptr(s["dsv"]), MAXSV, _ptr(s["ep"]),
                                       _ptr(s["de"]), _ptr(s["st"]), _ptr(s["ds"]), _ptr(s["rt"]), _ptr(s["dr"]),
                                       _ptr(s["srate"]), _ptr(s["cumslip"]), _ptr(s["twinsys"]),

@@ -1550,6 +1550,125 @@ extern "C" int cpfem_get_state(cpfem_ctx* c, double* x, double* dx, double* fc, 
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// REAL(4) binding of the FAS_COM / FAS_CPL arrays (fas-com-mod.f90:13-80):
+// the reference's single-precision module arrays pass as they are; the
+// library widens them to its FP64 device state (exactly) and narrows results
+// to single precision with round-to-nearest, as the reference's own REAL(4)
+// stores do.
+// ---------------------------------------------------------------------------
+namespace {
+// FP64 copy of a single-precision host array (empty for a null pointer)
+struct Wide {
+    std::vector<double> v;
+    Wide(const float* p, size_t n) {
+        if (p) v.assign(p, p + n);
+    }
+    double* get() { return v.empty() ? nullptr : v.data(); }
+    void narrow(float* p) const {
+        if (p)
+            for (size_t i = 0; i < v.size(); ++i) p[i] = (float)v[i];
+    }
+};
+}  // namespace
+
+extern "C" int cpfem_create_r4(const cpfem_mesh_r4* m, const cpfem_materials_r4* mt, const cpfem_control* ctl,
+                               cpfem_ctx** out) {
+    if (!m || !mt || !ctl || !out || mt->nmatl < 1 || mt->nmatl > CPFEM_MAXMDS || !mt->nss || !mt->e ||
+        !mt->ssang || !mt->ab || !mt->an || !mt->ampar || !mt->smatrx || m->ninc1 < 1 || m->nxs < 0 ||
+        m->nxs > 99 || m->nfs < 0 || m->nfs > 99)
+        return CPFEM_E_ARG;
+    const size_t ninc1 = (size_t)m->ninc1;
+    Wide tiem(m->tiem, ninc1), xs(m->x_spec, ninc1 * 99 * 15), fs(m->f_spec, ninc1 * 99 * 3);
+    cpfem_mesh md;
+    md.nlmnt = m->nlmnt;
+    md.nnod = m->nnod;
+    md.ngps = m->ngps;
+    md.nsv = m->nsv;
+    md.le = m->le;
+    md.lt = m->lt;
+    md.nd = m->nd;
+    md.np = m->np;
+    md.ln = m->ln;
+    md.lg = m->lg;
+    md.ib = m->ib;
+    md.ninc1 = m->ninc1;
+    md.nxs = m->nxs;
+    md.nfs = m->nfs;
+    md.tiem = tiem.get();
+    md.x_spec = xs.get();
+    md.f_spec = fs.get();
+    // material set k at its slice of the module arrays: e(9,k), ssang(60,k),
+    // ab/an(3,60,k), ampar(10,60,k), smatrx(60,60,k); twinlim and twinprob are
+    // module scalars shared by every set (fas-com-mod.f90:59-60)
+    std::vector<cpfem_material> mats((size_t)mt->nmatl);
+    for (int k = 0; k < mt->nmatl; ++k) {
+        cpfem_material& d = mats[(size_t)k];
+        memset(&d, 0, sizeof(d));
+        d.nss = mt->nss[k];
+        if (d.nss < 1 || d.nss > CPFEM_MAXSS) return CPFEM_E_NSS;
+        d.twinlim = mt->twinlim;
+        d.twinprob = mt->twinprob;
+        for (int i = 0; i < 9; ++i) d.e[i] = mt->e[9 * k + i];
+        const size_t b = (size_t)k * CPFEM_MAXSS;
+        for (int i = 0; i < CPFEM_MAXSS; ++i) {
+            d.ssang[i] = mt->ssang[b + i];
+            for (int q = 0; q < 3; ++q) {
+                d.ab[i][q] = mt->ab[3 * (b + i) + q];
+                d.an[i][q] = mt->an[3 * (b + i) + q];
+            }
+            for (int q = 0; q < 10; ++q) d.ampar[i][q] = mt->ampar[10 * (b + i) + q];
+            for (int j = 0; j < CPFEM_MAXSS; ++j)
+                d.smatrx[i][j] = mt->smatrx[(size_t)k * CPFEM_MAXSS * CPFEM_MAXSS + (size_t)i * CPFEM_MAXSS + j];
+        }
+    }
+    return cpfem_create(&md, mats.data(), mt->nmatl, ctl, out);
+}
+
+extern "C" int cpfem_set_state_r4(cpfem_ctx* c, const float* x, const float* dx, const float* eps,
+                                  const float* dep, const float* sv, const float* dsv, int32_t ldsv, const float* ep,
+                                  const float* de, const float* st, const float* ds, const float* rt, const float* dr,
+                                  const int32_t* twinsys, const int32_t* timestwinned) {
+    if (!c || ldsv < c->nsv) return CPFEM_E_ARG;
+    const size_t G = c->ngps, D = 3 * (size_t)c->nnod;
+    Wide wx(x, D), wdx(dx, D), weps(eps, G), wdep(dep, G), wsv(sv, G * ldsv), wdsv(dsv, G * ldsv);
+    Wide wep(ep, 6 * G), wde(de, 6 * G), wst(st, 6 * G), wds(ds, 6 * G), wrt(rt, 3 * G), wdr(dr, 3 * G);
+    return cpfem_set_state(c, wx.get(), wdx.get(), weps.get(), wdep.get(), wsv.get(), wdsv.get(), ldsv, wep.get(),
+                           wde.get(), wst.get(), wds.get(), wrt.get(), wdr.get(), twinsys, timestwinned);
+}
+
+extern "C" int cpfem_get_state_r4(cpfem_ctx* c, float* x, float* dx, float* fc, float* eps, float* dep, float* sv,
+                                  float* dsv, int32_t ldsv, float* ep, float* de, float* st, float* ds, float* rt,
+                                  float* dr, float* srate, float* cumslip, int32_t* twinsys, int32_t* timestwinned) {
+    if (!c || ldsv < c->nsv) return CPFEM_E_ARG;
+    const size_t G = c->ngps, D = 3 * (size_t)c->nnod;
+    // entries the library does not write (state variables past nsv, slip
+    // columns past nss) keep the caller's values: the FP64 buffers start from them
+    Wide wx(x, D), wdx(dx, D), wfc(fc, D), weps(eps, G), wdep(dep, G), wsv(sv, G * ldsv), wdsv(dsv, G * ldsv);
+    Wide wep(ep, 6 * G), wde(de, 6 * G), wst(st, 6 * G), wds(ds, 6 * G), wrt(rt, 3 * G), wdr(dr, 3 * G);
+    Wide wsr(srate, G * CPFEM_MAXSS), wcs(cumslip, G * CPFEM_MAXSS);
+    const int rc = cpfem_get_state(c, wx.get(), wdx.get(), wfc.get(), weps.get(), wdep.get(), wsv.get(), wdsv.get(),
+                                   ldsv, wep.get(), wde.get(), wst.get(), wds.get(), wrt.get(), wdr.get(), wsr.get(),
+                                   wcs.get(), twinsys, timestwinned);
+    if (rc) return rc;
+    wx.narrow(x);
+    wdx.narrow(dx);
+    wfc.narrow(fc);
+    weps.narrow(eps);
+    wdep.narrow(dep);
+    wsv.narrow(sv);
+    wdsv.narrow(dsv);
+    wep.narrow(ep);
+    wde.narrow(de);
+    wst.narrow(st);
+    wds.narrow(ds);
+    wrt.narrow(rt);
+    wdr.narrow(dr);
+    wsr.narrow(srate);
+    wcs.narrow(cumslip);
+    return 0;
+}
+
 extern "C" int cpfem_update_node(cpfem_ctx* c) {
     if (!c) return CPFEM_E_ARG;
     const int D = 3 * c->nnod;

@@ -39,6 +39,22 @@ MODULE CPFEM_CAPI
         TYPE(C_PTR) :: tiem, x_spec, f_spec
     END TYPE cpfem_mesh
 
+    ! REAL(4) module arrays of the reference (fas-com-mod.f90:13-80)
+    TYPE, BIND(C) :: cpfem_mesh_r4
+        INTEGER(C_INT32_T) :: nlmnt, nnod, ngps, nsv
+        TYPE(C_PTR) :: le, lt, nd, np, ln, lg, ib
+        INTEGER(C_INT32_T) :: ninc1, nxs, nfs
+        TYPE(C_PTR) :: tiem, x_spec, f_spec          ! REAL(4)
+    END TYPE cpfem_mesh_r4
+
+    TYPE, BIND(C) :: cpfem_materials_r4
+        INTEGER(C_INT32_T) :: nmatl, twinlim
+        REAL(C_FLOAT) :: twinprob
+        TYPE(C_PTR) :: nss                           ! nss(maxmds)
+        TYPE(C_PTR) :: e, ssang, ab, an, ampar, smatrx   ! e(9,maxmds), ssang(60,maxmds), ab/an(3,60,maxmds),
+                                                     ! ampar(10,60,maxmds), smatrx(60,60,maxmds)
+    END TYPE cpfem_materials_r4
+
     TYPE, BIND(C) :: cpfem_stats
         INTEGER(C_INT32_T) :: nit, sor_sweeps, sor_sweeps_total, err
         REAL(C_DOUBLE) :: sdx, sfc, sumcpiter, alp2, sor_error
@@ -61,6 +77,35 @@ MODULE CPFEM_CAPI
             INTEGER(C_INT32_T), VALUE :: nmat
             TYPE(cpfem_control), INTENT(IN) :: ctl
             TYPE(C_PTR), INTENT(OUT) :: ctx
+        END FUNCTION
+
+        INTEGER(C_INT) FUNCTION cpfem_create_r4(mesh, mats, ctl, ctx) BIND(C, name='cpfem_create_r4')
+            IMPORT :: C_INT, C_PTR, cpfem_mesh_r4, cpfem_materials_r4, cpfem_control
+            TYPE(cpfem_mesh_r4), INTENT(IN) :: mesh
+            TYPE(cpfem_materials_r4), INTENT(IN) :: mats
+            TYPE(cpfem_control), INTENT(IN) :: ctl
+            TYPE(C_PTR), INTENT(OUT) :: ctx
+        END FUNCTION
+
+        INTEGER(C_INT) FUNCTION cpfem_set_state_r4(ctx, x, dx, eps, dep, sv, dsv, ldsv, ep, de, st, ds, rt, dr, &
+                                                   twinsys, timestwinned) BIND(C, name='cpfem_set_state_r4')
+            IMPORT :: C_INT, C_INT32_T, C_PTR, C_FLOAT
+            TYPE(C_PTR), VALUE :: ctx
+            REAL(C_FLOAT), INTENT(IN) :: x(*), dx(*), eps(*), dep(*), sv(*), dsv(*)
+            INTEGER(C_INT32_T), VALUE :: ldsv
+            REAL(C_FLOAT), INTENT(IN) :: ep(*), de(*), st(*), ds(*), rt(*), dr(*)
+            INTEGER(C_INT32_T), INTENT(IN) :: twinsys(*), timestwinned(*)
+        END FUNCTION
+
+        INTEGER(C_INT) FUNCTION cpfem_get_state_r4(ctx, x, dx, fc, eps, dep, sv, dsv, ldsv, ep, de, st, ds, rt, dr, &
+                                                   srate, cumslip, twinsys, timestwinned) &
+                                                   BIND(C, name='cpfem_get_state_r4')
+            IMPORT :: C_INT, C_INT32_T, C_PTR, C_FLOAT
+            TYPE(C_PTR), VALUE :: ctx
+            REAL(C_FLOAT), INTENT(INOUT) :: x(*), dx(*), fc(*), eps(*), dep(*), sv(*), dsv(*)
+            INTEGER(C_INT32_T), VALUE :: ldsv
+            REAL(C_FLOAT), INTENT(INOUT) :: ep(*), de(*), st(*), ds(*), rt(*), dr(*), srate(*), cumslip(*)
+            INTEGER(C_INT32_T), INTENT(INOUT) :: twinsys(*), timestwinned(*)
         END FUNCTION
 
         INTEGER(C_INT) FUNCTION cpfem_destroy(ctx) BIND(C, name='cpfem_destroy')

@@ -28,19 +28,23 @@ MODULE HOST_STATE
     USE ISO_C_BINDING
     USE CPFEM_CAPI
     IMPLICIT NONE
-    INTEGER, PARAMETER :: maxsvs = 63, maxss = 60
+    INTEGER, PARAMETER :: maxsvs = 63, maxss = 60, maxmds = 12
     INTEGER(C_INT32_T) :: nlmnt, nnod, ngps, nsv, ninc, ninc1, nxs, nfs, nmatl, nmpar, maxit, inc
     INTEGER(C_INT32_T) :: last_tflag = 0
     INTEGER(C_INT32_T), ALLOCATABLE, TARGET :: le(:), lt(:), nd(:), np(:), ln(:,:), lg(:,:), ib(:)
     INTEGER(C_INT32_T), ALLOCATABLE :: twinsys(:), timestwinned(:)
-    REAL(C_DOUBLE), ALLOCATABLE, TARGET :: x(:), dx(:), fc(:), eps(:), dep(:), sv(:,:), dsv(:,:)
-    REAL(C_DOUBLE), ALLOCATABLE :: ep(:,:), de(:,:), st(:,:), ds(:,:), rt(:,:), dr(:,:)
-    REAL(C_DOUBLE), ALLOCATABLE :: srate(:,:), cumslip(:,:)
-    REAL(C_DOUBLE), ALLOCATABLE, TARGET :: tiem(:), x_spec(:,:,:), f_spec(:,:,:)
-    TYPE(cpfem_material), ALLOCATABLE :: mats(:)
-    REAL(4) :: amu, anu, aga, rtwinprob
+    ! the reference's REAL(4) module arrays (FAS_COM / FAS_CPL, fas-com-mod.f90:13-80),
+    ! passed to the library as they are (cpfem_create_r4, cpfem_set_state_r4, cpfem_get_state_r4)
+    REAL(4), ALLOCATABLE, TARGET :: x(:), dx(:), fc(:), eps(:), dep(:), sv(:,:), dsv(:,:)
+    REAL(4), ALLOCATABLE :: ep(:,:), de(:,:), st(:,:), ds(:,:), rt(:,:), dr(:,:)
+    REAL(4), ALLOCATABLE :: srate(:,:), cumslip(:,:)
+    REAL(4), ALLOCATABLE, TARGET :: tiem(:), x_spec(:,:,:), f_spec(:,:,:)
+    INTEGER(4), TARGET :: nss(maxmds)
+    REAL(4), TARGET :: e(9, maxmds), ssang(maxss, maxmds), ab(3, maxss, maxmds), an(3, maxss, maxmds)
+    REAL(4), TARGET :: ampar(10, maxss, maxmds), smatrx(maxss, maxss, maxmds)
+    REAL(4) :: amu, anu, aga, twinprob
     REAL(4) :: toler, alpha, afactor, efix, smalp, smalf, smald, smale, smalr
-    INTEGER(4) :: itwinlim
+    INTEGER(4) :: twinlim
     TYPE(cpfem_control) :: ctl
     TYPE(C_PTR) :: ctx = C_NULL_PTR
     CHARACTER(len=100) :: dir = './'
@@ -56,19 +60,24 @@ PROGRAM FASOLT_GPU
     LOGICAL :: calib
     REAL(C_DOUBLE) :: cpar(5), closs, cgrad(5)
     REAL(C_DOUBLE), ALLOCATABLE :: ctarget(:), ccurve(:)
-    TYPE(cpfem_mesh) :: mesh
+    TYPE(cpfem_mesh_r4) :: mesh
+    TYPE(cpfem_materials_r4) :: mtab
 
     CALL FSINIT
     CALL RDDATA(f_name)
 
+    ! SETPTA (Fasolt3.f90:24) and the upload of the module arrays
     mesh%nlmnt = nlmnt; mesh%nnod = nnod; mesh%ngps = ngps; mesh%nsv = nsv
     mesh%le = C_LOC(le); mesh%lt = C_LOC(lt); mesh%nd = C_LOC(nd); mesh%np = C_LOC(np)
     mesh%ln = C_LOC(ln); mesh%lg = C_LOC(lg); mesh%ib = C_LOC(ib)
     mesh%ninc1 = ninc1; mesh%nxs = nxs; mesh%nfs = nfs
     mesh%tiem = C_LOC(tiem); mesh%x_spec = C_LOC(x_spec); mesh%f_spec = C_LOC(f_spec)
-    rc = cpfem_create(mesh, mats, nmatl, ctl, ctx)
+    mtab%nmatl = nmatl; mtab%twinlim = twinlim; mtab%twinprob = twinprob
+    mtab%nss = C_LOC(nss); mtab%e = C_LOC(e); mtab%ssang = C_LOC(ssang)
+    mtab%ab = C_LOC(ab); mtab%an = C_LOC(an); mtab%ampar = C_LOC(ampar); mtab%smatrx = C_LOC(smatrx)
+    rc = cpfem_create_r4(mesh, mtab, ctl, ctx)
     IF (rc .NE. 0) CALL FAFERR(rc)
-    rc = cpfem_set_state(ctx, x, dx, eps, dep, sv, dsv, maxsvs, ep, de, st, ds, rt, dr, twinsys, timestwinned)
+    rc = cpfem_set_state_r4(ctx, x, dx, eps, dep, sv, dsv, maxsvs, ep, de, st, ds, rt, dr, twinsys, timestwinned)
     IF (rc .NE. 0) CALL FAFERR(rc)
 
     WRITE(*, '(15X,''Number of increments is: '',I4)') ninc
@@ -187,8 +196,7 @@ SUBROUTINE RDDATA(f_name)
     CHARACTER(len=100) :: dir1, dir2, dir3, fpath
     INTEGER :: ierr, i, j, k, m, nde, npe, n
     INTEGER(4) :: i4a, i4b, i4c, i4d
-    REAL(4) :: r6(6), r3(3), r15(15), r9(9), rbuf(maxsvs)
-    REAL(4), ALLOCATABLE :: rrow(:)
+    REAL(4) :: r6(6), r3(3), r15(15), rbuf(maxsvs)
     LOGICAL :: here
     dir3 = ''
     OPEN(ichan, FILE='dirname', STATUS='OLD', IOSTAT=ierr)
@@ -219,7 +227,7 @@ SUBROUTINE RDDATA(f_name)
     ALLOCATE(eps(ngps), dep(ngps), sv(maxsvs, ngps), dsv(maxsvs, ngps))
     ALLOCATE(ep(6, ngps), de(6, ngps), st(6, ngps), ds(6, ngps), rt(3, ngps), dr(3, ngps))
     ALLOCATE(srate(maxss, ngps), cumslip(maxss, ngps), twinsys(ngps), timestwinned(ngps))
-    sv = 0.d0; dsv = 0.d0; srate = 0.d0; cumslip = 0.d0; twinsys = 0; timestwinned = 0
+    sv = 0.0; dsv = 0.0; srate = 0.0; cumslip = 0.0; twinsys = 0; timestwinned = 0
     DO i = 1, nlmnt
         READ(ichan) i4a, i4b, nde, npe
         le(i) = i4a; lt(i) = i4b; nd(i) = nde; np(i) = npe
@@ -250,41 +258,31 @@ SUBROUTINE RDDATA(f_name)
     END DO
     READ(ichan) nmatl, nmpar
     IF (nmatl .GT. 12) CALL FAFERR(601)
-    ALLOCATE(mats(nmatl))
+    nss = 0; e = 0.0; ssang = 0.0; ab = 0.0; an = 0.0; ampar = 0.0; smatrx = 0.0
     DO m = 1, nmatl
-        mats(m)%e = 0.d0; mats(m)%ssang = 0.d0; mats(m)%ab = 0.d0; mats(m)%an = 0.d0
-        mats(m)%ampar = 0.d0; mats(m)%smatrx = 0.d0
         READ(ichan) n
         IF (n .GT. maxss) CALL FAFERR(602)
         IF (n .GT. maxsvs - 3) CALL FAFERR(603)
-        mats(m)%nss = n
-        READ(ichan) itwinlim
-        READ(ichan) rtwinprob
-        mats(m)%twinlim = itwinlim
-        mats(m)%twinprob = rtwinprob
-        READ(ichan) r9
-        mats(m)%e = r9
-        ALLOCATE(rrow(max(n, nmpar, 6)))
-        READ(ichan) rrow(1:n)
-        mats(m)%ssang(1:n) = rrow(1:n)
+        nss(m) = n
+        READ(ichan) twinlim          ! module scalars: the last set's values hold (Faslt_io_nowin.f90:83-86)
+        READ(ichan) twinprob
+        READ(ichan) e(:, m)
+        READ(ichan) ssang(1:n, m)
         DO i = 1, n
             READ(ichan) r6
-            mats(m)%ab(:, i) = r6(1:3); mats(m)%an(:, i) = r6(4:6)
+            ab(:, i, m) = r6(1:3); an(:, i, m) = r6(4:6)
         END DO
         DO i = 1, n
-            READ(ichan) rrow(1:nmpar)
-            mats(m)%ampar(1:nmpar, i) = rrow(1:nmpar)
+            READ(ichan) ampar(1:nmpar, i, m)
         END DO
         DO i = 1, n
-            READ(ichan) rrow(1:n)
-            mats(m)%smatrx(1:n, i) = rrow(1:n)
+            READ(ichan) smatrx(1:n, i, m)
         END DO
-        DEALLOCATE(rrow)
     END DO
     READ(ichan) amu, anu, aga
     READ(ichan) ninc1, nxs, nfs
     ALLOCATE(tiem(ninc1), x_spec(15, 99, ninc1), f_spec(3, 99, ninc1))
-    x_spec = 0.d0; f_spec = 0.d0
+    x_spec = 0.0; f_spec = 0.0
     DO i = 1, ninc1
         READ(ichan) r3(1)
         tiem(i) = r3(1)
@@ -362,7 +360,7 @@ SUBROUTINE REORIENT_LOG(ntw)
     IMPLICIT NONE
     INTEGER(4) :: ntw, rc, j
     INTEGER(C_INT32_T), ALLOCATABLE :: tws(:)
-    REAL(C_DOUBLE), ALLOCATABLE :: e0(:,:)
+    REAL(4), ALLOCATABLE :: e0(:,:)
     ntw = 0
     ! tflag of the last pass (Fasolt3.f90:43): pull the state only when twins are pending
     IF (last_tflag .EQ. 0) RETURN
@@ -374,7 +372,7 @@ SUBROUTINE REORIENT_LOG(ntw)
     IF (rc .NE. 0) CALL FAFERR(rc)
     CALL PULL_STATE
     DO j = 1, ngps
-        IF (tws(j) .NE. 0) WRITE(55, '(I8, I8, I8, 9999ES15.5)') inc, j, tws(j), REAL(e0(:, j), 4), REAL(sv(1:3, j), 4)
+        IF (tws(j) .NE. 0) WRITE(55, '(I8, I8, I8, 9999ES15.5)') inc, j, tws(j), e0(:, j), sv(1:3, j)
     END DO
     DEALLOCATE(tws, e0)
     last_tflag = 0
@@ -384,8 +382,8 @@ SUBROUTINE PULL_STATE
     USE HOST_STATE
     IMPLICIT NONE
     INTEGER(4) :: rc
-    rc = cpfem_get_state(ctx, x, dx, fc, eps, dep, sv, dsv, maxsvs, ep, de, st, ds, rt, dr, srate, cumslip, &
-                         twinsys, timestwinned)
+    rc = cpfem_get_state_r4(ctx, x, dx, fc, eps, dep, sv, dsv, maxsvs, ep, de, st, ds, rt, dr, srate, cumslip, &
+                            twinsys, timestwinned)
     IF (rc .NE. 0) CALL FAFERR(rc)
 END SUBROUTINE PULL_STATE
 
@@ -409,47 +407,47 @@ SUBROUTINE WTDATA(f_name)
     END DO
     DO i = 1, nnod
         WRITE(ochan) ib(i)
-        WRITE(ochan) REAL(x(3*i-2:3*i), 4), REAL(dx(3*i-2:3*i), 4)
-        WRITE(ochan) REAL(fc(3*i-2:3*i), 4)
+        WRITE(ochan) x(3*i-2:3*i), dx(3*i-2:3*i)
+        WRITE(ochan) fc(3*i-2:3*i)
     END DO
     DO i = 1, ngps
-        WRITE(ochan) REAL(eps(i), 4), REAL(dep(i), 4)
-        WRITE(ochan) REAL(sv(1:nsv, i), 4)
-        WRITE(ochan) REAL(dsv(1:nsv, i), 4)
-        WRITE(ochan) REAL(ep(:, i), 4)
-        WRITE(ochan) REAL(de(:, i), 4)
-        WRITE(ochan) REAL(st(:, i), 4)
-        WRITE(ochan) REAL(ds(:, i), 4)
-        WRITE(ochan) REAL(rt(:, i), 4)
-        WRITE(ochan) REAL(dr(:, i), 4)
+        WRITE(ochan) eps(i), dep(i)
+        WRITE(ochan) sv(1:nsv, i)
+        WRITE(ochan) dsv(1:nsv, i)
+        WRITE(ochan) ep(:, i)
+        WRITE(ochan) de(:, i)
+        WRITE(ochan) st(:, i)
+        WRITE(ochan) ds(:, i)
+        WRITE(ochan) rt(:, i)
+        WRITE(ochan) dr(:, i)
     END DO
     WRITE(ochan) nmatl, nmpar
     DO m = 1, nmatl
-        n = mats(m)%nss
+        n = nss(m)
         WRITE(ochan) n
-        WRITE(ochan) mats(m)%twinlim
-        WRITE(ochan) REAL(mats(m)%twinprob, 4)
-        WRITE(ochan) REAL(mats(m)%e, 4)
-        WRITE(ochan) REAL(mats(m)%ssang(1:n), 4)
+        WRITE(ochan) twinlim
+        WRITE(ochan) twinprob
+        WRITE(ochan) e(:, m)
+        WRITE(ochan) ssang(1:n, m)
         DO i = 1, n
-            WRITE(ochan) REAL(mats(m)%ab(:, i), 4), REAL(mats(m)%an(:, i), 4)
+            WRITE(ochan) ab(:, i, m), an(:, i, m)
         END DO
         DO i = 1, n
-            WRITE(ochan) REAL(mats(m)%ampar(1:nmpar, i), 4)
+            WRITE(ochan) ampar(1:nmpar, i, m)
         END DO
         DO i = 1, n
-            WRITE(ochan) REAL(mats(m)%smatrx(1:n, i), 4)
+            WRITE(ochan) smatrx(1:n, i, m)
         END DO
     END DO
     WRITE(ochan) amu, anu, aga
     WRITE(ochan) ninc1, nxs, nfs
     DO i = 1, ninc1
-        WRITE(ochan) REAL(tiem(i), 4)
+        WRITE(ochan) tiem(i)
         DO j = 1, nxs
-            WRITE(ochan) REAL(x_spec(:, j, i), 4)
+            WRITE(ochan) x_spec(:, j, i)
         END DO
         DO j = 1, nfs
-            WRITE(ochan) REAL(f_spec(:, j, i), 4)
+            WRITE(ochan) f_spec(:, j, i)
         END DO
     END DO
     CLOSE(ochan)
@@ -461,11 +459,11 @@ SUBROUTINE WRITESLIPS(f_name)
     IMPLICIT NONE
     CHARACTER(len=*) :: f_name
     INTEGER :: i, j, n
-    n = mats(1)%nss
+    n = nss(1)
     WRITE(*, *) f_name
     OPEN(unit=54, file=f_name, action="write", status="replace")
     DO i = 1, ngps
-        WRITE(54, '(9999ES15.5)') (REAL(cumslip(j, i), 4), j = 1, n), (REAL(srate(j, i), 4), j = 1, n)
+        WRITE(54, '(9999ES15.5)') (cumslip(j, i), j = 1, n), (srate(j, i), j = 1, n)
     END DO
     CLOSE(54)
 END SUBROUTINE WRITESLIPS

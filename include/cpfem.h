@@ -197,6 +197,48 @@ int cpfem_get_state(cpfem_ctx *ctx, double *x, double *dx, double *fc, double *e
                     double *dsv, int32_t ldsv, double *ep, double *de, double *st, double *ds, double *rt,
                     double *dr, double *srate, double *cumslip, int32_t *twinsys, int32_t *timestwinned);
 
+/* REAL(4) binding: the reference keeps its state in single-precision module
+ * arrays (FAS_COM / FAS_CPL, fas-com-mod.f90:13-80).  These entry points take
+ * those arrays as they are, so a Fortran host passes its module variables
+ * with no copies of its own; the library widens them to its FP64 state and
+ * narrows results with round-to-nearest.  Layouts as in the module: x(3*nnod),
+ * sv(ldsv, ngps) with ldsv = maxsvs = 63, st(6, ngps), srate(60, ngps), ... */
+#define CPFEM_MAXMDS 12 /* fas-com-mod.f90:57 maxmds */
+typedef struct {
+    int32_t nlmnt, nnod, ngps, nsv;
+    const int32_t *le, *lt, *nd, *np;
+    const int32_t *ln;       /* ln(1:20, nlmnt), 1-based */
+    const int32_t *lg;       /* lg(1:8, nlmnt), 1-based */
+    const int32_t *ib;       /* ib(nnod) */
+    int32_t ninc1, nxs, nfs;
+    const float *tiem;       /* tiem(ninc1) */
+    const float *x_spec;     /* x_spec(15, 99, ninc1) */
+    const float *f_spec;     /* f_spec(3, 99, ninc1) */
+} cpfem_mesh_r4;
+/* The FAS_CPL material arrays of sets 1..nmatl, leading dimensions maxss = 60:
+ * nss(k), e(9,k), ssang(60,k), ab/an(3,60,k), ampar(10,60,k), smatrx(60,60,k);
+ * twinlim and twinprob are the module's scalars (one value for every set). */
+typedef struct {
+    int32_t nmatl;
+    int32_t twinlim;
+    float twinprob;
+    const int32_t *nss;
+    const float *e, *ssang, *ab, *an, *ampar, *smatrx;
+} cpfem_materials_r4;
+/* cpfem_create on the module arrays (RDDATA's, Faslt_io_nowin.f90:29-128). */
+int cpfem_create_r4(const cpfem_mesh_r4 *mesh, const cpfem_materials_r4 *mats, const cpfem_control *ctl,
+                    cpfem_ctx **out);
+/* cpfem_set_state / cpfem_get_state on REAL(4) arrays (the .fsd / .frs point
+ * and node records, Faslt_io_nowin.f90:45-65,207-240).  get: entries the
+ * library does not own (sv past nsv, srate/cumslip past nss) are left as they are. */
+int cpfem_set_state_r4(cpfem_ctx *ctx, const float *x, const float *dx, const float *eps, const float *dep,
+                       const float *sv, const float *dsv, int32_t ldsv, const float *ep, const float *de,
+                       const float *st, const float *ds, const float *rt, const float *dr,
+                       const int32_t *twinsys, const int32_t *timestwinned);
+int cpfem_get_state_r4(cpfem_ctx *ctx, float *x, float *dx, float *fc, float *eps, float *dep, float *sv,
+                       float *dsv, int32_t ldsv, float *ep, float *de, float *st, float *ds, float *rt,
+                       float *dr, float *srate, float *cumslip, int32_t *twinsys, int32_t *timestwinned);
+
 /* UPDATE_NODE (Fasolt3.f90:231): x += dx. */
 int cpfem_update_node(cpfem_ctx *ctx);
 /* fc = 0, ss = 0 and ELVALS (Stif_sol.f90:38-42, Elvals.f90:11): every point's
