@@ -152,7 +152,7 @@ def run_path(prob, precision, warmup, steps, warm_s, timed_s, capi, trace=False,
         log(f"[bench] {label} inc {inc}: nit {st['nit']} sweeps {st['sor_sweeps_total']} total "
             f"{st['t_total_ms']:.1f} ms const {st['t_const_ms']:.1f} asm {st['t_asm_ms']:.1f} sor "
             f"{st['t_sor_ms']:.1f} code {st['code']} 987-points {st['pts_987_last']} (updates {st['pts_987']}) "
-            f"pts {st['gp_updates']} continued {st['pts_continued']} wide {st['pts_wide']}")
+            f"pts {st['gp_updates']} continued {st['pts_continued']} wide {st['pts_wide']} omega {st['sor_omega']:.4g}")
         if time.perf_counter() - t0 > timed_s:
             break
     wall = time.perf_counter() - t0
@@ -191,6 +191,11 @@ def leg_summary(recs, wall, warm, first_inc):
         "points_987_final_iteration_each": [r["pts_987_last"] for r in recs],
         "point_updates_987_each": [r["pts_987"] for r in recs],
         "warmup_increments": len(warm),
+        "warmup_s_each": [r["t_total_ms"] * 1e-3 for r in warm],
+        "warmup_nr_iterations_each": [r["nit"] for r in warm],
+        "warmup_sweeps_each": [r["sor_sweeps_total"] for r in warm],
+        "warmup_constitutive_s_each": [r["t_const_ms"] * 1e-3 for r in warm],
+        "warmup_points_987_each": [r["pts_987_last"] for r in warm],
     }
 
 

@@ -232,6 +232,7 @@ def _bind_mesh_api(L):
     L.cpfem_get_matrix.argtypes = [_CTX, _P, _P, _P, _P]
     L.cpfem_get_stats.argtypes = [_CTX, C.POINTER(CpfemStats)]
     L.cpfem_autotune.argtypes = [_CTX, C.c_int32, _P, _P]
+    L.cpfem_autotune_points.argtypes = [_CTX, C.c_int32, _P, _P]
     L.cpfem_calib_begin.argtypes = [_CTX, C.c_int32, C.c_int32, C.c_double, C.c_int32]
     L.cpfem_calib_record.argtypes = [_CTX, C.c_int32]
     L.cpfem_calib_history.argtypes = [_CTX, C.c_int32, _P]
@@ -407,6 +408,14 @@ class Solver:
         flags = (TUNE_LAUNCH if launch else 0) | (TUNE_OMEGA if omega else 0)
         _check(self.L.cpfem_autotune(self.h, flags, _ptr(om), _ptr(wpb)), "cpfem_autotune")
         return float(om[0]), int(wpb[0])
+
+    def autotune_points(self, inc=0, first_iters=0, elems_per_block=0):
+        """Constitutive first-pass budget and assembly elements per workgroup:
+        timed on ELVALS of increment inc (> 0) or set (inc = 0).  Returns both."""
+        fi = np.array([first_iters], np.int32)
+        ep = np.array([elems_per_block], np.int32)
+        _check(self.L.cpfem_autotune_points(self.h, int(inc), _ptr(fi), _ptr(ep)), "cpfem_autotune_points")
+        return int(fi[0]), int(ep[0])
 
     def set_sweeps(self, omega=0.0, waves=0):
         """Apply a relaxation factor and/or waves per workgroup (0 keeps it)."""

@@ -658,7 +658,6 @@ constexpr int kAmax = CPFEM_AMAX;
 constexpr int kAmaxMid = 8;
 constexpr int kAmaxWide = 12;
 constexpr int kAmaxGeneric = kConstMaxWidth;   // fourth tier, generic system counts only
-constexpr int kFirstPassIters = 8;
 
 __global__ void k_zero_n(int* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -683,14 +682,18 @@ __global__ void k_set1(int* p, int v) {
 constexpr int kSortBins = 1024;
 
 // Sort key of a point: with flops (the work of the point's previous update,
-// for the pass over every point) an eighth-octave bin of it, otherwise the
-// mean slip-Newton iterations per outer iteration so far in eighths.  Larger
-// work sorts first.
+// for the pass over every point) an eighth-octave bin of it, which is also
+// saved in save_bin; with prev_bin that saved bin; otherwise the mean
+// slip-Newton iterations per outer iteration so far in eighths.  Larger work
+// sorts first.
 struct SortKey {
     const int* ck_inner;
     const int* ck_iter;
     const long long* flops;
+    const int* prev_bin;
+    int* save_bin;
     __device__ __forceinline__ int bin(int g) const {
+        if (prev_bin) return prev_bin[g];
         int m;
         if (flops) {
             const unsigned long long f = flops[g] > 0 ? (unsigned long long)flops[g] : 1ull;
@@ -750,6 +753,7 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const int* __restrict__ li
     if (live) {
         g = list ? list[i] : i;
         key = K.bin(g);
+        if (K.save_bin) K.save_bin[g] = key;
         atomicAdd(&cnt[key], 1);
     }
     __syncthreads();
@@ -762,10 +766,12 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const int* __restrict__ li
 }
 
 // Sorted copy of the n-point list (null: points 0..n-1) into S.sorted, keyed
-// by the previous updates' work (flops non-null) or by the checkpoints.
-static hipError_t sort_points(ConstSchedule& S, const int* list, int n, const long long* flops, hipStream_t stream) {
+// by the previous updates' work (flops non-null; the bins are kept in
+// S.prev_bin), by those kept bins (by_prev) or by the checkpoints.
+static hipError_t sort_points(ConstSchedule& S, const int* list, int n, const long long* flops, hipStream_t stream,
+                              bool by_prev = false) {
     const int nb = (n + 255) / 256;
-    const SortKey K{S.ck_inner, S.ck_iter, flops};
+    const SortKey K{S.ck_inner, S.ck_iter, flops, by_prev ? S.prev_bin : nullptr, flops ? S.prev_bin : nullptr};
     hipLaunchKernelGGL(k_zero_n, dim3(kSortBins / 256), dim3(256), 0, stream, S.bins, kSortBins);
     hipLaunchKernelGGL(k_sort_count, dim3(nb), dim3(256), 0, stream, list, n, K, S.bins);
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kSortBins), 0, stream, S.bins);
@@ -803,7 +809,7 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
     A.ck_kerr = S.ck_kerr;
     A.el = S.el;
     A.gp = S.gp;
-    // tier 1, pass 1: every point, at most kFirstPassIters outer iterations;
+    // tier 1, pass 1: every point, at most S.first_iters outer iterations;
     // on a mesh, in the order of the work of each point's previous update
     A.first = 1;
     A.in_list = nullptr;
@@ -816,7 +822,8 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
     A.cont_count = S.counts + 0;
     A.ovf_list = S.list_ovf;
     A.ovf_count = S.counts + 2;
-    A.iter_limit = kFirstPassIters < C.maxit ? kFirstPassIters : C.maxit;
+    const int fi = S.first_iters > 0 ? S.first_iters : 8;
+    A.iter_limit = fi < C.maxit ? fi : C.maxit;
     for (int q = 0; q < 4; ++q) S.pass_ms[q] = 0.0;
     const bool timed = S.ev[0] != nullptr;
     auto mark = [&](int k) {
@@ -853,7 +860,11 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
         hipLaunchKernelGGL(k_zero4, dim3(1), dim3(64), 0, stream, S.counts);  // list_a reused below
         // the overflow count was read on the host; keep it on the device in slot 3
         hipLaunchKernelGGL(k_set1, dim3(1), dim3(64), 0, stream, S.counts + 3, S.host_counts[2]);
-        if ((e = sort_points(S, S.list_ovf, S.host_counts[2], nullptr, stream)) != hipSuccess) return e;
+        // the points of the wider tier mostly outgrow the first kernel's width at
+        // their first outer iteration and keep the wider set; what they will cost
+        // is best told by their previous update's work, so on a mesh they run in
+        // the order of the bins the first pass sorted by
+        if ((e = sort_points(S, S.list_ovf, S.host_counts[2], nullptr, stream, A.mesh != 0)) != hipSuccess) return e;
         A.in_list = S.sorted;
         A.in_count = S.counts + 3;
         A.cont_list = S.list_b;
@@ -928,6 +939,7 @@ hipError_t alloc_schedule(ConstSchedule& S, int n) {
     if ((e = hipMalloc((void**)&S.ck_kerr, sizeof(int) * (size_t)n)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&S.ck_inner, sizeof(int) * (size_t)n)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&S.sorted, sizeof(int) * (size_t)n)) != hipSuccess) return e;
+    if ((e = hipMalloc((void**)&S.prev_bin, sizeof(int) * (size_t)n)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&S.bins, sizeof(int) * kSortBins)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&S.list_a, sizeof(int) * (size_t)n)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&S.list_b, sizeof(int) * (size_t)n)) != hipSuccess) return e;
@@ -942,7 +954,7 @@ hipError_t alloc_schedule(ConstSchedule& S, int n) {
 }
 
 void free_schedule(ConstSchedule& S) {
-    void* p[] = {S.ck_s, S.ck_iter, S.ck_kerr, S.ck_inner, S.sorted, S.bins, S.list_a,
+    void* p[] = {S.ck_s, S.ck_iter, S.ck_kerr, S.ck_inner, S.sorted, S.prev_bin, S.bins, S.list_a,
                  S.list_b, S.list_ovf, S.el, S.gp, S.counts};
     for (void* q : p)
         if (q) (void)hipFree(q);

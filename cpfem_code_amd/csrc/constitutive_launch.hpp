@@ -20,6 +20,7 @@ struct ConstSchedule {
     int* ck_kerr = nullptr;    // [n]
     int* ck_inner = nullptr;   // [n] slip-Newton iterations so far (ordering key)
     int* sorted = nullptr;     // [n] a point list in resume order
+    int* prev_bin = nullptr;   // [n] sort bin of each point's previous-update work (mesh mode)
     int* bins = nullptr;       // [1024] counting-sort bins
     int* list_a = nullptr;     // [n] unfinished after the first pass
     int* list_b = nullptr;     // [n] spare
@@ -29,6 +30,7 @@ struct ConstSchedule {
     int* counts = nullptr;     // [4] device
     int* host_counts = nullptr;// [4] pinned
     hipEvent_t ev[5] = {};     // around the passes of the last run (timing)
+    int first_iters = 8;       // outer iterations of the first pass over every point (cpfem_autotune)
     double pass_ms[4] = {};    // device time of the last run: pass 1, pass 2, 8-wide, 12(+)-wide tiers
 };
 

@@ -124,14 +124,14 @@ struct AsmArgs {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_assemble(const AsmArgs A, T* __restrict__ vals) {
-    constexpr int EPB = 4;  // elements per block
+    constexpr int EPB = 4;  // elements per block at most (blockDim.x / 64 of the launch: 1, 2 or 4)
     __shared__ T sh_dn[EPB][8][8][3];
     __shared__ T sh_w[EPB][8];
     __shared__ T sh_d[EPB][8][36];
     __shared__ T sh_s[EPB][8][6];
     __shared__ double sh_x[EPB][8][3];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int k = blockIdx.x * EPB + wave;
+    const int k = blockIdx.x * (int)(blockDim.x >> 6) + wave;
     const bool live = k < A.nel_color;
     const int e = live ? A.elems[k] : 0;
     const int ng = live ? A.elem_np[e] : 0;
@@ -1772,11 +1772,12 @@ static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
             AA.nel_color = cnt;
             AA.elems = c->elem_by_color + b0;
             if (q == kElem8) {
-                const int nb = (cnt + 3) / 4;
+                const int epb = c->asm_epb;
+                const int nb = (cnt + epb - 1) / epb;
                 if (f32)
-                    hipLaunchKernelGGL(k_assemble<float>, dim3(nb), dim3(256), 0, s, AA, (float*)c->A.vals);
+                    hipLaunchKernelGGL(k_assemble<float>, dim3(nb), dim3(64 * epb), 0, s, AA, (float*)c->A.vals);
                 else
-                    hipLaunchKernelGGL(k_assemble<double>, dim3(nb), dim3(256), 0, s, AA, (double*)c->A.vals);
+                    hipLaunchKernelGGL(k_assemble<double>, dim3(nb), dim3(64 * epb), 0, s, AA, (double*)c->A.vals);
             } else if (q == kElem20) {
                 if (f32)
                     hipLaunchKernelGGL((k_assemble_tab<float, 20>), dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
@@ -2020,6 +2021,53 @@ static double residual(Ctx* c, const double* b, const double* y, double* res) {
 
 static int autotune_impl(Ctx* c, int flags);
 
+// Relaxation factor that follows the solves.  The ladder tuning of the first
+// solve sees the first (elastic) system; the stiffness softens as the points
+// yield and the best factor moves.  Each residual-rule solve measures its
+// asymptotic rate (log residual reduction per sweep over its last two checks),
+// and a coordinate search moves omega: after a solve at the best factor, the
+// next runs at best + dir*step; a trial that converges faster per sweep becomes
+// the best and the search goes on in that direction, one that does not turns
+// it round, halves the step (not below 0.0025) and returns to the best factor,
+// whose rate the next solve measures again (the systems drift).  Any factor
+// in (0, 2) converges on these SPD systems, so a trial costs sweeps, never the
+// solution.  The rates are deterministic reductions: so is the sequence.
+static void drop_graphs(Ctx* c);
+static void refine_omega(Ctx* c, double rate) {
+    if (!c->om_adapt || !(rate < 0.0) || !std::isfinite(rate)) return;
+    const double lo = 1.0, hi = 1.995, min_step = 0.0025;
+    double next;
+    if (!c->om_trial) {
+        c->om_rate = rate;
+        next = c->om_best + c->om_dir * c->om_step;
+        if (next < lo || next > hi) {
+            c->om_dir = -c->om_dir;
+            next = c->om_best + c->om_dir * c->om_step;
+        }
+        c->om_trial = true;
+    } else if (rate < c->om_rate) {
+        c->om_best = c->ctl.sor_omega;
+        c->om_rate = rate;
+        next = c->om_best + c->om_dir * c->om_step;
+        if (next < lo || next > hi) {
+            c->om_dir = -c->om_dir;
+            c->om_step = std::max(0.5 * c->om_step, min_step);
+            next = c->om_best;
+            c->om_trial = false;
+        }
+    } else {
+        c->om_dir = -c->om_dir;
+        c->om_step = std::max(0.5 * c->om_step, min_step);
+        next = c->om_best;
+        c->om_trial = false;
+    }
+    next = std::min(hi, std::max(lo, next));
+    if (next != c->ctl.sor_omega) {
+        c->ctl.sor_omega = next;
+        drop_graphs(c);
+    }
+}
+
 // FACSLS: solve A ddx = rhs from a zero start.
 // warm: start from the ddx in place instead of zero (the reference starts from
 // zero, Stif_sol.f90:179; the calibration's solves of consecutive increments
@@ -2030,10 +2078,17 @@ static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out, bool warm = fal
         const int rc = autotune_impl(c, CPFEM_TUNE_LAUNCH | CPFEM_TUNE_OMEGA);
         if (rc) return rc;
         c->omega_auto = false;
+        c->om_adapt = true;
+        c->om_best = c->ctl.sor_omega;
+        c->om_trial = false;
+        c->om_step = 0.01;
+        c->om_dir = 1;
     }
     hipStream_t s = c->stream;
     const int D = 3 * c->nnod;
     const int maxit = c->ctl.sor_maxit;
+    const double omega_used = c->ctl.sor_omega;
+    double final_rate = 0.0;
     CPFEM_HIP(hipEventRecord(c->ev[5], s));
     if (!warm) hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, s, c->ddx, (size_t)D, 0.0);
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, c->flags + 2, (size_t)2, 0);
@@ -2086,6 +2141,9 @@ static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out, bool warm = fal
             rel = residual(c, c->rhs, c->ddx, mixed ? c->xold : nullptr);
             c->stats.t_sweep_ms += elapsed(c->ev[7], c->ev[8]);
             c->stats.sweep_launches += (int64_t)batches * kSweepBatch * c->A.ncolor;
+            // asymptotic rate of the solve: the latest stretch of at least four batches
+            if (prev_rel > 0.0 && rel > 0.0 && rel < prev_rel && sweeps - prev_sweeps >= 4 * kSweepBatch)
+                final_rate = log(rel / prev_rel) / (double)(sweeps - prev_sweeps);
             if (rel < c->ctl.sor_tol) break;
             batches = 1;
             if (prev_rel > 0.0 && rel < prev_rel) {
@@ -2105,8 +2163,9 @@ static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out, bool warm = fal
     c->stats.sor_sweeps = sweeps;
     c->stats.sor_sweeps_total += sweeps;
     c->stats.sor_error = err;
-    c->stats.sor_omega = c->ctl.sor_omega;
+    c->stats.sor_omega = omega_used;
     c->stats.sor_wpb = c->sor_wpb;
+    if (code == 0 && !warm) refine_omega(c, final_rate);
     if (sweeps_out) *sweeps_out = sweeps;
     if (err_out) *err_out = err;
     if (code > c->stats.err) c->stats.err = code;
@@ -2283,15 +2342,92 @@ extern "C" int cpfem_autotune(cpfem_ctx* c, int32_t flags, double* omega, int32_
         if (om > 0.0) {
             c->ctl.sor_omega = om;
             c->omega_auto = false;
+            c->om_adapt = false;
         }
         drop_graphs(c);
     } else {
         const int rc = autotune_impl(c, flags);
         if (rc) return rc;
+        if ((flags & CPFEM_TUNE_OMEGA) && c->om_adapt) {
+            c->om_best = c->ctl.sor_omega;
+            c->om_trial = false;
+        }
     }
     if (flags & CPFEM_TUNE_OMEGA) c->omega_auto = false;
     if (omega) *omega = c->ctl.sor_omega;
     if (waves_per_block) *waves_per_block = c->sor_wpb;
+    return 0;
+}
+
+// Launch shapes of the point kernels: the outer iterations every point gets in
+// the constitutive kernel's first pass (the rest resume in later passes) and
+// the elements per workgroup of the 8-node assembly.  Neither changes a
+// result (every point and every element is computed by the same code), only
+// how the work is packed, so they are chosen by timing ELVALS on the current
+// state: four runs with first-pass budgets 4, 8, 16, 32 and 1, 2, 4, 4
+// elements per workgroup, each timed by its events, the state ELVALS
+// overwrites restored after each run.
+static int autotune_points_impl(Ctx* c, int inc) {
+    hipStream_t s = c->stream;
+    const size_t G = c->ngps;
+    struct Keep { void* d; size_t bytes; void* saved; } keep[] = {
+        {c->ds, 6 * G * 8, nullptr}, {c->dsv, (size_t)c->nsv * G * 8, nullptr},
+        {c->srate, (size_t)c->nss * G * 8, nullptr}, {c->flops, G * 8, nullptr},
+        {c->twinsys, G * 4, nullptr}, {c->flags, 2 * 4, nullptr}};
+    int rc = 0;
+    for (auto& k : keep)
+        if (hipMalloc(&k.saved, k.bytes) != hipSuccess) rc = CPFEM_E_HIP;
+    if (!rc)
+        for (auto& k : keep) (void)hipMemcpyAsync(k.saved, k.d, k.bytes, hipMemcpyDeviceToDevice, s);
+    const uint64_t epoch = c->rng_epoch;
+    const cpfem_stats st0 = c->stats;
+    const int fis[4] = {4, 8, 16, 32}, epbs[4] = {1, 2, 4, 4};
+    double tc[4] = {0, 0, 0, 0}, ta[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 4 && !rc; ++k) {
+        c->sched.first_iters = fis[k];
+        c->asm_epb = epbs[k];
+        c->rng_epoch = epoch;
+        memset(&c->stats, 0, sizeof(c->stats));
+        const int code = elvals_impl(c, inc, nullptr, nullptr);
+        if (code < 0) rc = code;
+        tc[k] = c->stats.t_const_ms;
+        ta[k] = c->stats.t_asm_ms;
+        for (auto& q : keep) (void)hipMemcpyAsync(q.d, q.saved, q.bytes, hipMemcpyDeviceToDevice, s);
+    }
+    if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = CPFEM_E_HIP;
+    for (auto& k : keep)
+        if (k.saved) (void)hipFree(k.saved);
+    c->rng_epoch = epoch;
+    c->stats = st0;
+    c->assembled = false;   // fc and the matrix hold a candidate's ELVALS: the caller runs ELVALS again
+    if (rc) {
+        c->sched.first_iters = 8;
+        c->asm_epb = 4;
+        return rc;
+    }
+    int bf = 0;
+    for (int k = 1; k < 4; ++k)
+        if (tc[k] < tc[bf]) bf = k;
+    c->sched.first_iters = fis[bf];
+    const double a1 = ta[0], a2 = ta[1], a4 = 0.5 * (ta[2] + ta[3]);
+    c->asm_epb = (a1 <= a2 && a1 <= a4) ? 1 : (a2 <= a4 ? 2 : 4);
+    return 0;
+}
+
+extern "C" int cpfem_autotune_points(cpfem_ctx* c, int32_t inc, int32_t* first_iters, int32_t* elems_per_block) {
+    if (!c) return CPFEM_E_ARG;
+    if (inc == 0) {
+        const int fi = first_iters ? *first_iters : 0, ep = elems_per_block ? *elems_per_block : 0;
+        if (fi < 0 || fi > 1000 || (ep != 0 && ep != 1 && ep != 2 && ep != 4)) return CPFEM_E_ARG;
+        if (fi > 0) c->sched.first_iters = fi;
+        if (ep > 0) c->asm_epb = ep;
+    } else {
+        if (inc < 1 || inc > c->ninc1) return CPFEM_E_ARG;
+        const int rc = autotune_points_impl(c, inc);
+        if (rc) return rc;
+    }
+    if (first_iters) *first_iters = c->sched.first_iters;
+    if (elems_per_block) *elems_per_block = c->asm_epb;
     return 0;
 }
 

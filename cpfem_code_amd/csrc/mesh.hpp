@@ -125,7 +125,14 @@ struct Ctx {
     hipGraph_t graph_src[3] = {nullptr, nullptr, nullptr};   // their captured graphs
     bool use_graphs = true;      // CPFEM_SOR_GRAPHS=0: launch the batches kernel by kernel
     int sor_wpb = 4;             // waves per workgroup of the colour sweeps (cpfem_autotune)
+    int asm_epb = 4;             // elements (waves) per workgroup of the 8-node assembly (cpfem_autotune)
     bool omega_auto = false;     // sor_omega <= 0 at creation: tuned at the first solve
+    // sor_omega <= 0 at creation: after the first solve's tuning, omega follows
+    // the convergence rates of the solves (coordinate search, see refine_omega)
+    bool om_adapt = false;
+    bool om_trial = false;       // the last solve ran at a trial factor
+    double om_best = 0.0, om_rate = 0.0, om_step = 0.01;
+    int om_dir = 1;
     bool assembled = false;      // A and rhs hold a system (set by SUVALS)
     bool vals32_fresh = false;   // precision 2: A.vals32 holds the rounded copy of the current A.vals
     // timing and counters of the last calls

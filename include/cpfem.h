@@ -302,10 +302,26 @@ int cpfem_get_stats(cpfem_ctx *ctx, cpfem_stats *stats);
  * the settings in force.  With flags 0 nothing is tuned and no system is
  * needed: a positive *omega (< 2) and a *waves_per_block of 1, 2 or 4 are
  * applied as given (0 keeps the setting).  A context created with
- * sor_omega <= 0 runs both tunings at its first solve. */
+ * sor_omega <= 0 runs both tunings at its first solve and afterwards moves
+ * omega by a coordinate search on the measured convergence rate of each
+ * residual-rule solve (a trial factor best +- step on alternate solves, kept
+ * when it converges faster per sweep; step 0.01 halving to 0.0025, omega in
+ * [1, 1.995]); setting omega through this call ends that. */
 #define CPFEM_TUNE_LAUNCH 1
 #define CPFEM_TUNE_OMEGA 2
 int cpfem_autotune(cpfem_ctx *ctx, int32_t flags, double *omega, int32_t *waves_per_block);
+
+/* Launch shapes of the point kernels.  No counterpart in the reference.  The
+ * constitutive kernel runs every point for a first budget of outer CPSTRS
+ * iterations, then resumes the unfinished points in later passes sorted by
+ * their work; the 8-node assembly packs 1, 2 or 4 elements (one wave each)
+ * per workgroup.  Neither changes a result.  With inc > 0 both are timed on
+ * ELVALS of increment inc on the current state (budgets 4, 8, 16, 32; the
+ * state ELVALS overwrites is restored, the matrix is not: call ELVALS again
+ * before the next solve) and the fastest are kept; with inc == 0 the given
+ * values (> 0; elements per workgroup 1, 2 or 4) are applied.  Both return
+ * the settings in force (defaults 8 and 4). */
+int cpfem_autotune_points(cpfem_ctx *ctx, int32_t inc, int32_t *first_iters, int32_t *elems_per_block);
 
 /* Calibration: replay of a recorded load path and the adjoint of its
  * constitutive updates.  No counterpart in the reference, which has no

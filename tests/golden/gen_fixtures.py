@@ -23,7 +23,9 @@ Fixtures:
                   with the .fmt and .ori inputs they read
   pipeline.npz    the reference chain on answer set b8_x2: FS3PRE's .fsd, then
                   fasolt3 (three increments, reference solver settings, the
-                  sample fafcdf.dat) and CSVMKR's element table of the last .frs
+                  sample fafcdf.dat, its fafrep.txt and .frs after increments 2
+                  and 3) and CSVMKR's element table of the last .frs; then
+                  fasolt3 resumed from the increment-2 .frs (fafrep.txt, .frs)
   csvmkr.npz      CSVMKR tables (point and element mode) of two .frs files: the
                   reference driver's state after increment 2 of a 2^3 one-point
                   FCC tension run, and a synthetic state on a 2^3 mesh of
@@ -258,8 +260,26 @@ def pipeline_fixture():
         refio.fasolt3(td, "b8_x2", 2)
         with open(os.path.join(td, "b8_x2-3.frs"), "rb") as fh:
             frs = np.frombuffer(fh.read(), np.uint8)
+        with open(os.path.join(td, "b8_x2-2.frs"), "rb") as fh:
+            frs2 = np.frombuffer(fh.read(), np.uint8)
+        with open(os.path.join(td, "fafrep.txt")) as fh:
+            fafrep = fh.read()
         el = refio.csvmkr(td, "b8_x2-3", True, "el")
-    np.savez_compressed(os.path.join(HERE, "pipeline.npz"), frs=frs, el=np.array(el), fafcdf=np.array(fafcdf))
+        # resume: the saved .frs read back as an input file.  Its first record
+        # holds the increment it was saved at, which RDDATA reads as the number
+        # of increments to run (Faslt_io_nowin.f90:29 vs :207), so the run
+        # repeats increments 1..2 of the load tables from the saved state.
+        with open(os.path.join(td, "resume.fsd"), "wb") as fh:
+            fh.write(frs2.tobytes())
+        refio.fasolt3(td, "resume", 1)
+        with open(os.path.join(td, "resume-2.frs"), "rb") as fh:
+            frs_resume = np.frombuffer(fh.read(), np.uint8)
+        resume_files = sorted(f for f in os.listdir(td) if f.startswith("resume-"))
+        with open(os.path.join(td, "fafrep.txt")) as fh:
+            fafrep_resume = fh.read()
+    np.savez_compressed(os.path.join(HERE, "pipeline.npz"), frs=frs, el=np.array(el), fafcdf=np.array(fafcdf),
+                        frs2=frs2, fafrep=np.array(fafrep), frs_resume=frs_resume,
+                        resume_files=np.array(resume_files), fafrep_resume=np.array(fafrep_resume))
 
 
 if __name__ == "__main__":

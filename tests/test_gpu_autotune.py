@@ -130,3 +130,28 @@ def test_autotune_arguments():
     bad = dataclasses.replace(prob, control=dataclasses.replace(prob.control, sor_omega=2.0))
     with pytest.raises(RuntimeError):
         capi.Solver(bad, precision=0)
+
+
+def test_point_kernel_shapes_keep_results():
+    """The constitutive first-pass budget and the assembly's elements per
+    workgroup only pack the work: increments are bitwise the same at every
+    setting, and tuning them on a state (cpfem_autotune_points) leaves that
+    state as it was."""
+    prob = fcc_plane_strain(4, ninc=3, seed=2, control=Control(sor_kind=1, sor_tol=1e-10, sor_omega=1.6,
+                                                                sor_maxit=50000))
+    runs = []
+    for mode in ("default", (3, 1), (32, 2), "tuned"):
+        sv = capi.Solver(prob, precision=0)
+        if isinstance(mode, tuple):
+            assert sv.autotune_points(0, *mode) == mode
+        sv.increment(1)
+        if mode == "tuned":
+            fi, ep = sv.autotune_points(2)
+            print(f"autotune points: first-pass budget {fi}, elements per workgroup {ep}", flush=True)
+            assert fi in (4, 8, 16, 32) and ep in (1, 2, 4)
+        sv.increment(2)
+        runs.append(sv.get_state())
+        sv.close()
+    for r in runs[1:]:
+        for k in ("x", "dx", "st", "sv", "ep", "srate", "cumslip"):
+            assert np.array_equal(r[k], runs[0][k]), k

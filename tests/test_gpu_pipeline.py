@@ -75,3 +75,93 @@ def test_prepro_host_post_chain_matches_reference(tmp_path):
             qa = post.quaternion(*mine_t[r_, lo:lo + 3])
             qb = post.quaternion(*ref_t[r_, lo:lo + 3])
             assert 2.0 * np.arccos(min(1.0, abs(float(qa @ qb)))) < 1e-3
+
+
+def _iteration_lines(text):
+    """(increment, [(nit, sdx, sfc, cper, alp2)], [power]) blocks of a fafrep.txt,
+    parsed by the columns of ITHEAD/ITEREP/POWER (Faslt_io_nowin.f90:308-366):
+    '(12X,I3,4X,5(1X,E9.2))' for an iteration line."""
+    incs, cur = [], None
+    for ln in text.splitlines():
+        if "Increment number" in ln:
+            cur = [int(ln.split()[-1]), [], []]
+            incs.append(cur)
+        elif ln.startswith("     Power:"):
+            cur[2].append(float(ln.split(":")[1]))
+        elif cur is not None and len(ln) >= 59 and ln[:12] == " " * 12 and ln[12:15].strip().isdigit():
+            assert ln[15:19] == "    ", ln
+            vals = [ln[19 + 10 * k:29 + 10 * k] for k in range(4)]
+            for v in vals:                      # 1X,E9.2: a blank, then d.ddE+dd right-aligned in 9
+                assert v[0] == " " and len(v.strip()) == 8 and v.strip()[4] == "E", (ln, v)
+            cur[1].append((int(ln[12:15]),) + tuple(float(v) for v in vals))
+    return incs
+
+
+def _run_host(d, stdin):
+    r = subprocess.run([build.HOST_EXE], input=stdin, text=True, cwd=d, capture_output=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return r
+
+
+def _setup_b8_x2(tmp_path):
+    g = np.load(os.path.join(GOLD, "fs3pre.npz"))
+    pz = np.load(os.path.join(GOLD, "pipeline.npz"))
+    d = str(tmp_path)
+    for k in g.keys():
+        if k.startswith("file_"):
+            (tmp_path / k[5:]).write_text(str(g[k]))
+    (tmp_path / "dirname").write_text("'./' './' './' './' './'\n")
+    (tmp_path / "fafcdf.dat").write_text(str(pz["fafcdf"]))
+    prepro.main(str(g["b8_x2_answers"]).splitlines(), workdir=d)
+    return d, pz
+
+
+def test_fafrep_lines_match_reference(tmp_path):
+    """fafrep.txt of the Fortran host on the b8_x2 chain against the reference
+    program's own: the same blocks and header text, every iteration line in
+    ITEREP's fixed columns, the increment's first report line (before any
+    solve) identical as text, iteration counts within two and the power
+    within 1e-3 (the two runs solve with different sweep orders and
+    precisions to the loose relative-update rule)."""
+    d, pz = _setup_b8_x2(tmp_path)
+    _run_host(d, "b8_x2\n1\n2\n")
+    mine = (tmp_path / "fafrep.txt").read_text()
+    ref = str(pz["fafrep"])
+    heads = lambda t: [ln for ln in t.splitlines() if "Increment number" in ln or "Iteration" in ln]  # noqa: E731
+    assert heads(mine) == heads(ref)
+    a, b = _iteration_lines(mine), _iteration_lines(ref)
+    assert [x[0] for x in a] == [x[0] for x in b] == [1, 2, 3]
+    first = lambda t: [ln for ln in t.splitlines() if ln[12:15] == "  1"][0]  # noqa: E731
+    assert first(mine) == first(ref)
+    for (ia, la, pa), (ib, lb, pb) in zip(a, b):
+        print(f"fafrep inc {ia}: iterations {len(la)} vs reference {len(lb)}, power {pa} vs {pb}", flush=True)
+        assert [r[0] for r in la] == list(range(1, len(la) + 1))
+        assert abs(len(la) - len(lb)) <= 2
+        assert len(pa) == len(pb) == 1 and abs(pa[0] - pb[0]) <= 1e-3 * abs(pb[0])
+        assert la[-1][1] < 1e-3 and lb[-1][1] < 1e-3          # both stop on Error(X) < toler
+
+
+def test_resume_from_frs_matches_reference(tmp_path):
+    """A saved .frs read back as the input file: its first record holds the
+    increment it was saved at, which RDDATA reads as the number of increments
+    (Faslt_io_nowin.f90:29 vs :207), so the run repeats increments 1..2 of the
+    load tables from the saved state.  The host is given the reference's own
+    increment-2 .frs and must write the same files as the reference resumed
+    from it, with the same state at the pipeline test's bars."""
+    d, pz = _setup_b8_x2(tmp_path)
+    (tmp_path / "resume.fsd").write_bytes(pz["frs2"].tobytes())
+    _run_host(d, "resume\n1\n1\n")
+    files = sorted(f for f in os.listdir(d) if f.startswith("resume-"))
+    assert files == [str(f) for f in pz["resume_files"]]
+    mine = fsd.read_frs(os.path.join(d, "resume-2.frs"))
+    (tmp_path / "ref_resume.frs").write_bytes(pz["frs_resume"].tobytes())
+    ref = fsd.read_frs(str(tmp_path / "ref_resume.frs"))
+    # the final save's first record holds the DO variable after the loop,
+    # ninc + 1 (Fasolt3.f90:38-66), in both programs
+    assert mine["inc"] == ref["inc"] == 3
+    dx = np.abs(mine["x"] - ref["x"]).max() / np.abs(ref["x"]).max()
+    dst = np.abs(mine["st"] - ref["st"]).max() / np.abs(ref["st"]).max()
+    print(f"resume: |dx|/|x| {dx:.3e}  |dst|/|st| {dst:.3e}", flush=True)
+    assert dx < 5e-3 and dst < 1e-3
+    a, b = _iteration_lines((tmp_path / "fafrep.txt").read_text()), _iteration_lines(str(pz["fafrep_resume"]))
+    assert [x[0] for x in a] == [x[0] for x in b] == [1, 2]
