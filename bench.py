@@ -122,7 +122,7 @@ def traced_increment(sv, inc, ctl):
     return rec
 
 
-def run_path(prob, precision, warmup, steps, warm_s, timed_s, capi, trace=False, label=""):
+def run_path(prob, precision, warmup, steps, warm_s, timed_s, capi, trace=False, label="", tune_points=False):
     """Warm-up increments (at least one, at most `warmup`, stopping once warm_s
     is spent), then timed increments (at least one, at most `steps`, stopping
     once timed_s is spent).  Returns the timed records, their wall time, the
@@ -143,6 +143,12 @@ def run_path(prob, precision, warmup, steps, warm_s, timed_s, capi, trace=False,
             f"{st['sor_omega']:.3g} waves/block {st['sor_wpb']} tune {st['t_tune_ms']:.1f} ms")
         if time.perf_counter() - t_w > warm_s:
             break
+    shapes = None
+    if tune_points:
+        t_tp = time.perf_counter()
+        shapes = sv.autotune_points(inc + 1)
+        log(f"[bench] {label}: point-kernel shapes tuned on increment {inc + 1}: first-pass budget {shapes[0]}, "
+            f"elements per workgroup {shapes[1]} ({time.perf_counter() - t_tp:.1f} s)")
     recs = []
     t0 = time.perf_counter()
     while len(recs) < max(steps, 1) and inc < prob.ninc1:
@@ -157,6 +163,8 @@ def run_path(prob, precision, warmup, steps, warm_s, timed_s, capi, trace=False,
             break
     wall = time.perf_counter() - t0
     sv.close()
+    if recs:
+        recs[0]["point_shapes"] = shapes
     return recs, wall, warm, inc
 
 
@@ -191,6 +199,7 @@ def leg_summary(recs, wall, warm, first_inc):
         "points_987_final_iteration_each": [r["pts_987_last"] for r in recs],
         "point_updates_987_each": [r["pts_987"] for r in recs],
         "warmup_increments": len(warm),
+        "point_kernel_shapes": recs[0].get("point_shapes"),
         "warmup_s_each": [r["t_total_ms"] * 1e-3 for r in warm],
         "warmup_nr_iterations_each": [r["nit"] for r in warm],
         "warmup_sweeps_each": [r["sor_sweeps_total"] for r in warm],
@@ -380,6 +389,9 @@ def main():
     ap.add_argument("--stall-tol", type=float, default=0.0,
                     help="> 0: CPSTRS also stops at a stationary stress (opt-in; the reference runs to maxit)")
     ap.add_argument("--accuracy", type=int, default=1, help="0 skips leg 4 (one-system accuracy check)")
+    ap.add_argument("--tune-points", type=int, default=1,
+                    help="1: tune the constitutive first-pass budget and the assembly's elements per workgroup "
+                         "after the warm-up (cpfem_autotune_points, untimed); 0: the defaults (8, 4)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample; 0 skips it")
     ap.add_argument("--calibration", type=int, default=0, help="mesh size of an opt-in calibration leg (0: none)")
     ap.add_argument("--trace-iterations", action="store_true",
@@ -425,11 +437,11 @@ def main():
     traffic = load_traffic()
     # 1-2: the headline paths
     recs64, wall64, warm64, _ = run_path(p_head, 0, args.warmup, args.steps, 0.08 * B, 0.22 * B, capi,
-                                         trace=args.trace_iterations, label="fp64")
+                                         trace=args.trace_iterations, label="fp64", tune_points=bool(args.tune_points))
     first = len(warm64) + 1
     legs["fp64"] = leg_summary(recs64, wall64, warm64, first)
     recsm, wallm, warmm, _ = run_path(p_head, args.mixed_precision, args.warmup, args.steps, 0.08 * B, 0.18 * B,
-                                      capi, trace=args.trace_iterations, label="mixed")
+                                      capi, trace=args.trace_iterations, label="mixed", tune_points=bool(args.tune_points))
     legs["mixed"] = leg_summary(recsm, wallm, warmm, len(warmm) + 1)
     # 3: the configured relaxation factor, one timed increment each (increment 2)
     if args.configured_omega > 0 and time.time() - t_start < 0.75 * B:

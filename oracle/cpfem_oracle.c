@@ -20,8 +20,12 @@
  *   ELVALS/SUVALS/FACSLS/NRITER: compiled reference on 1-GP bricks, symmetry,
  *   rigid-body null space, direct-solve agreement.
  *   Twin draws with twinprob < 1: the reference uses the compiler's
- *   RANDOM_NUMBER, which is not restated — parity unpinned (twinprob = 1 is
- *   pinned: the draw is then always below the probability).
+ *   RANDOM_NUMBER afresh at every check; its stream is not reproducible, so
+ *   the draw here is the build's own counter-based one (one uniform number
+ *   per check), pinned by what the reference's model fixes: uniformity and
+ *   independence of the draws, and Wald's identity (twinned points =
+ *   twinprob x checks) on large batches (tests/test_twin_draw.py).
+ *   twinprob = 1 never consults the draw.
  */
 #include "cpfem_oracle.h"
 
