@@ -115,8 +115,8 @@ def traced_increment(sv, inc, ctl):
     keep = ("sor_omega", "sor_wpb", "pts_987_last")
     rec = {k: (after[k] - before[k]) if not isinstance(after[k], list) else
            [a - b for a, b in zip(after[k], before[k])]
-           for k in after if k not in ("nit", "sor_sweeps", "err", "sdx", "sfc", "sumcpiter", "alp2", "sor_error",
-                                       "pad_") + keep}
+           for k in after if k not in ("nit", "sor_sweeps", "err", "sdx", "sfc", "sumcpiter", "alp2", "sor_error")
+           + keep}
     rec.update({k: after[k] for k in keep})
     rec.update(nit=nit, code=code, t_total_ms=1e3 * (time.perf_counter() - t0))
     return rec

@@ -195,7 +195,8 @@ class CpfemStats(C.Structure):
                 ("t_sweep_ms", C.c_double), ("sweep_launches", C.c_int64),
                 ("sor_omega", C.c_double), ("t_tune_ms", C.c_double), ("sor_wpb", C.c_int32),
                 ("tune_sweeps", C.c_int32), ("pts_987", C.c_int64), ("pts_987_last", C.c_int32),
-                ("pad_", C.c_int32), ("t_const_pass_ms", C.c_double * 4)]
+                ("graph_launches", C.c_int32), ("t_const_pass_ms", C.c_double * 4),
+                ("t_graph_ms", C.c_double)]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
@@ -222,6 +223,7 @@ def _bind_mesh_api(L):
     L.cpfem_facsls.argtypes = [_CTX, _P, _P]
     L.cpfem_nr_update.argtypes = [_CTX, _P, _P]
     L.cpfem_nriter.argtypes = [_CTX, C.c_int32, C.POINTER(CpfemStats)]
+    L.cpfem_set_graph_mode.argtypes = [_CTX, C.c_int32]
     L.cpfem_update_gptv.argtypes = [_CTX]
     L.cpfem_reorient.argtypes = [_CTX, _P]
     L.cpfem_set_tflag.argtypes = [_CTX, C.c_int32]
@@ -429,6 +431,11 @@ class Solver:
         alp2 = np.zeros(1)
         _check(self.L.cpfem_nr_update(self.h, _ptr(sdx), _ptr(alp2)), "cpfem_nr_update")
         return float(sdx[0]), float(alp2[0])
+
+    def set_graph_mode(self, on=True):
+        """Run each Newton iteration of nriter/increment as one graph launch
+        (residual-rule solves only)."""
+        _check(self.L.cpfem_set_graph_mode(self.h, 1 if on else 0), "cpfem_set_graph_mode")
 
     def nriter(self, inc):
         st = CpfemStats()

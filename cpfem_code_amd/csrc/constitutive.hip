@@ -189,7 +189,8 @@ __device__ __forceinline__ int slpfta_c(const DevMaterial<N>& M, const DevContro
 // The per-point kernel.  One lane per point, 64 lanes per block.
 template <int N, int AMAX>
 __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__ Mp, const DevControl C,
-                                              const double dt, const ConstArgs A) {
+                                              const double dt_arg, const ConstArgs A) {
+    const double dt = A.dt_dev ? *A.dt_dev : dt_arg;
     // LDS: the Schmid columns of the active systems (also the staging area of
     // the active-set compaction) and the active system numbers, interleaved by
     // lane; the compliance and the last slips live in per-point work arrays
@@ -305,7 +306,7 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
             }
             int twinsys = A.twinsys[g];
             const int timestwinned = A.timestwinned[g];
-            const uint64_t rng_key = (A.rng_epoch << 32) ^ (uint64_t)g;
+            const uint64_t rng_key = ((A.rng_epoch_dev ? *A.rng_epoch_dev : A.rng_epoch) << 32) ^ (uint64_t)g;
             int err = kerr, tflag = 0;
             Compact<AMAX> S;
             double djac[6][6], epl[6];
@@ -711,11 +712,12 @@ struct SortKey {
 // Bin counts: per block in LDS, then one device atomic per occupied bin
 // (most points share a few bins, whose device counters would otherwise take
 // one atomic per point).
-__global__ __launch_bounds__(256) void k_sort_count(const int* __restrict__ list, int count, const SortKey K,
-                                                    int* bins) {
+__global__ __launch_bounds__(256) void k_sort_count(const int* __restrict__ list, int count, const int* count_dev,
+                                                    const SortKey K, int* bins) {
     __shared__ int sh[kSortBins];
     for (int b = threadIdx.x; b < kSortBins; b += blockDim.x) sh[b] = 0;
     __syncthreads();
+    if (count_dev) count = *count_dev;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) atomicAdd(&sh[K.bin(list ? list[i] : i)], 1);
     __syncthreads();
@@ -741,12 +743,13 @@ __global__ __launch_bounds__(kSortBins) void k_sort_scan(int* bins) {
 
 // Scatter into bin order: each block reserves its run of every occupied bin
 // with one device atomic, then ranks its points within the run in LDS.
-__global__ __launch_bounds__(256) void k_sort_scatter(const int* __restrict__ list, int count, const SortKey K,
-                                                      int* bins, int* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_sort_scatter(const int* __restrict__ list, int count, const int* count_dev,
+                                                      const SortKey K, int* bins, int* __restrict__ out) {
     __shared__ int cnt[kSortBins];
     __shared__ int base[kSortBins];
     for (int b = threadIdx.x; b < kSortBins; b += blockDim.x) cnt[b] = 0;
     __syncthreads();
+    if (count_dev) count = *count_dev;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = i < count;
     int g = 0, key = 0;
@@ -768,14 +771,15 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const int* __restrict__ li
 // Sorted copy of the n-point list (null: points 0..n-1) into S.sorted, keyed
 // by the previous updates' work (flops non-null; the bins are kept in
 // S.prev_bin), by those kept bins (by_prev) or by the checkpoints.
+// count_dev: the list's length read on the device (n is then the largest it can be).
 static hipError_t sort_points(ConstSchedule& S, const int* list, int n, const long long* flops, hipStream_t stream,
-                              bool by_prev = false) {
+                              bool by_prev = false, const int* count_dev = nullptr) {
     const int nb = (n + 255) / 256;
     const SortKey K{S.ck_inner, S.ck_iter, flops, by_prev ? S.prev_bin : nullptr, flops ? S.prev_bin : nullptr};
     hipLaunchKernelGGL(k_zero_n, dim3(kSortBins / 256), dim3(256), 0, stream, S.bins, kSortBins);
-    hipLaunchKernelGGL(k_sort_count, dim3(nb), dim3(256), 0, stream, list, n, K, S.bins);
+    hipLaunchKernelGGL(k_sort_count, dim3(nb), dim3(256), 0, stream, list, n, count_dev, K, S.bins);
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kSortBins), 0, stream, S.bins);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(nb), dim3(256), 0, stream, list, n, K, S.bins, S.sorted);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(nb), dim3(256), 0, stream, list, n, count_dev, K, S.bins, S.sorted);
     return hipGetLastError();
 }
 
@@ -919,6 +923,108 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
         }
     }
     return hipGetLastError();
+}
+
+// The schedule above with every decision on the device, for capture in a
+// graph: no count is read back; each pass and tier is launched over the
+// largest list it can get (its threads past the device count return at
+// once) and the sorts read their list lengths on the device.  report[0] and
+// report[1] receive the points continued after the first pass and those
+// needing a wider tier, report[2] the points wider than the widest tier.
+__global__ void k_tier_prep(int* counts, int* report) {
+    if (threadIdx.x == 0) {
+        report[0] = counts[0];
+        report[1] = counts[2];
+        counts[3] = counts[2];
+        counts[0] = 0;
+        counts[1] = 0;
+        counts[2] = 0;
+    }
+}
+__global__ void k_tier4_prep(int* counts) {
+    if (threadIdx.x == 0) {
+        counts[3] = counts[2];
+        counts[0] = 0;
+    }
+}
+__global__ void k_width_report(const int* counts, int slot, int* report) {
+    if (threadIdx.x == 0) report[2] = counts[slot];
+}
+
+template <int N>
+static hipError_t schedule_dev(const void* dmat, const DevControl& C, double dt, ConstArgs A, ConstSchedule& S,
+                               hipStream_t stream, int* report) {
+    const int n = A.n;
+    hipLaunchKernelGGL(k_zero4, dim3(1), dim3(64), 0, stream, S.counts);
+    A.ck_s = S.ck_s;
+    A.ck_iter = S.ck_iter;
+    A.ck_inner = S.ck_inner;
+    A.ck_kerr = S.ck_kerr;
+    A.el = S.el;
+    A.gp = S.gp;
+    A.first = 1;
+    A.in_list = nullptr;
+    A.in_count = nullptr;
+    hipError_t e;
+    if (A.mesh) {
+        if ((e = sort_points(S, nullptr, n, A.flops, stream)) != hipSuccess) return e;
+        A.in_list = S.sorted;
+    }
+    A.cont_list = S.list_a;
+    A.cont_count = S.counts + 0;
+    A.ovf_list = S.list_ovf;
+    A.ovf_count = S.counts + 2;
+    const int fi = S.first_iters > 0 ? S.first_iters : 8;
+    A.iter_limit = fi < C.maxit ? fi : C.maxit;
+    launch_one<N, kAmax>(dmat, C, dt, A, n, stream);
+    // pass 2: the unfinished points
+    A.first = 0;
+    if ((e = sort_points(S, S.list_a, n, nullptr, stream, false, S.counts + 0)) != hipSuccess) return e;
+    A.in_list = S.sorted;
+    A.in_count = S.counts + 0;
+    A.cont_list = S.list_b;
+    A.cont_count = S.counts + 1;
+    A.iter_limit = C.maxit;
+    launch_one<N, kAmax>(dmat, C, dt, A, n, stream);
+    // tier 2: the overflow of tier 1, by the previous update's work on a mesh
+    hipLaunchKernelGGL(k_tier_prep, dim3(1), dim3(64), 0, stream, S.counts, report);
+    if ((e = sort_points(S, S.list_ovf, n, nullptr, stream, A.mesh != 0, S.counts + 3)) != hipSuccess) return e;
+    A.in_list = S.sorted;
+    A.in_count = S.counts + 3;
+    A.cont_list = S.list_b;
+    A.cont_count = S.counts + 1;
+    A.ovf_list = S.list_a;
+    A.ovf_count = S.counts + 0;
+    launch_one<N, kAmaxMid>(dmat, C, dt, A, n, stream);
+    // tier 3: the overflow of tier 2
+    A.in_list = S.list_a;
+    A.in_count = S.counts + 0;
+    A.ovf_list = S.list_ovf;
+    A.ovf_count = S.counts + 2;
+    launch_one<N, kAmaxWide>(dmat, C, dt, A, n, stream);
+    if constexpr (N == kNGeneric) {
+        hipLaunchKernelGGL(k_tier4_prep, dim3(1), dim3(64), 0, stream, S.counts);
+        A.in_list = S.list_ovf;
+        A.in_count = S.counts + 3;
+        A.ovf_list = S.list_a;
+        A.ovf_count = S.counts + 0;
+        launch_one<N, kAmaxGeneric>(dmat, C, dt, A, n, stream);
+        hipLaunchKernelGGL(k_width_report, dim3(1), dim3(64), 0, stream, (const int*)S.counts, 0, report);
+    } else {
+        hipLaunchKernelGGL(k_width_report, dim3(1), dim3(64), 0, stream, (const int*)S.counts, 2, report);
+    }
+    return hipGetLastError();
+}
+
+hipError_t run_constitutive_dev(int nss, const void* dmat, const DevControl& C, double dt, const ConstArgs& A,
+                                ConstSchedule& S, hipStream_t stream, int* report) {
+    switch (nss) {
+        case 12: return schedule_dev<12>(dmat, C, dt, A, S, stream, report);
+        case 18: return schedule_dev<18>(dmat, C, dt, A, S, stream, report);
+        default:
+            if (nss >= 1 && nss <= kNGeneric) return schedule_dev<kNGeneric>(dmat, C, dt, A, S, stream, report);
+            return hipErrorInvalidValue;
+    }
 }
 
 hipError_t run_constitutive(int nss, const void* dmat, const DevControl& C, double dt, const ConstArgs& A,

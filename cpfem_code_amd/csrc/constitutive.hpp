@@ -319,6 +319,8 @@ struct ConstArgs {
     int* tflag;                // scalar
     int* errmax;               // scalar
     uint64_t rng_epoch;
+    const uint64_t* rng_epoch_dev;   // non-null: the epoch read on the device (graph capture)
+    const double* dt_dev;            // non-null: the time step read on the device (graph capture)
     // schedule
     double* el;                // [21][n] compliance in specimen axes (work array)
     double* gp;                // [width][n] slips of the last hardening update (work array)

@@ -43,6 +43,10 @@ void free_schedule(ConstSchedule& S);
 // after the first pass, counts[1] = points needing the full-width kernel.
 hipError_t run_constitutive(int nss, const void* dmat, const DevControl& C, double dt, const ConstArgs& A,
                             ConstSchedule& S, hipStream_t stream, int counts[4]);
+// The same with every decision on the device (capturable in a graph); report
+// [0] continued points, [1] points of the wider tiers, [2] points too wide.
+hipError_t run_constitutive_dev(int nss, const void* dmat, const DevControl& C, double dt, const ConstArgs& A,
+                                ConstSchedule& S, hipStream_t stream, int* report);
 // Stream epoch of the twin draws of the next cpfem_dstrss_batch call.
 void set_batch_epoch(uint64_t next);
 

@@ -57,7 +57,9 @@ __device__ __forceinline__ void block_partials(double v[Q], double* partial, int
     }
 }
 
-__global__ void k_reduce_final(const double* __restrict__ partial, int nblocks, double* __restrict__ out) {
+// Sums of the nblocks partial slots in a fixed order, by one block of kBlock
+// threads; thread 0 receives them in out.
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ partial, int nblocks, double out[4]) {
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
 #pragma unroll
@@ -73,6 +75,13 @@ __global__ void k_reduce_final(const double* __restrict__ partial, int nblocks, 
             out[q] = s;
         }
     }
+}
+
+__global__ void k_reduce_final(const double* __restrict__ partial, int nblocks, double* __restrict__ out) {
+    double r[4];
+    reduce_partials(partial, nblocks, r);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; ++q) out[q] = r[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -699,12 +708,18 @@ struct BcArgs {
     const double* fs;   // f_spec of this increment [99][3]
     double efix;
     double* rhs;
+    const int* inc_dev;   // non-null: xs, fs are the whole tables and the increment is read here
 };
 
 template <typename T>
-__global__ void k_suvals(const BcArgs A, T* __restrict__ vals, double* partial) {
+__global__ void k_suvals(BcArgs A, T* __restrict__ vals, double* partial) {
     const int nd = blockIdx.x * blockDim.x + threadIdx.x;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (A.inc_dev) {
+        const size_t k = (size_t)(*A.inc_dev - 1);
+        A.xs += k * 99 * 15;
+        A.fs += k * 99 * 3;
+    }
     if (nd < A.nnod) {
         const int idof = 3 * nd;
         double r[3] = {A.fc[idof], A.fc[idof + 1], A.fc[idof + 2]};
@@ -768,6 +783,8 @@ struct SorArgs {
     const int* cols;     // [group][P][64]
     double omega;
     const int* stop;     // device flag: nonzero -> skip the sweep
+    const double* omega_dev = nullptr;   // non-null: the relaxation factor read on the device
+    const int* gate = nullptr;           // non-null: the residual runs only while *gate != 0
 };
 
 constexpr int kWavesPerBlock = kBlock / 64;
@@ -851,11 +868,12 @@ template <typename T, typename V, bool ACC, int PC>
 __global__ __launch_bounds__(kBlock) void k_sor_color(const SorArgs A, const T* __restrict__ vals,
                                                       const V* __restrict__ b, V* __restrict__ y, double* partial,
                                                       int pslot0) {
+    if (*A.stop) return;   // uniform: the flag is written by earlier kernels only
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     const int lane = threadIdx.x & 63;
     const int w = xcd_block(A.per_xcd) * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
     const int l = lane / kGroupRows, i = lane - l * kGroupRows;
-    const bool live = (l < kRowLanes) && (w * kGroupRows + i < A.nrows) && (*A.stop == 0);
+    const bool live = (l < kRowLanes) && (w * kGroupRows + i < A.nrows);
     const size_t g = (size_t)A.gbase + w;
     T sum[3] = {(T)0, (T)0, (T)0};
     T d[9];
@@ -866,7 +884,7 @@ __global__ __launch_bounds__(kBlock) void k_sor_color(const SorArgs A, const T* 
     row_combine<T>(sum, lane);
     if (live && l == 0) {
         T yn[3] = {(T)y[3 * nd], (T)y[3 * nd + 1], (T)y[3 * nd + 2]};
-        const T om = (T)A.omega;
+        const T om = (T)(A.omega_dev ? *A.omega_dev : A.omega);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             T s = sum[a];
@@ -898,6 +916,7 @@ template <typename T, int PC>
 __global__ __launch_bounds__(kBlock) void k_residual(const SorArgs A, const T* __restrict__ vals,
                                                      const double* __restrict__ b, const double* __restrict__ y,
                                                      double* __restrict__ res, double* partial) {
+    if (A.gate && *A.gate == 0) return;   // uniform over the grid
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     const int lane = threadIdx.x & 63;
     const int w = xcd_block(A.per_xcd) * kWavesPerBlock + (threadIdx.x >> 6);
@@ -1009,6 +1028,191 @@ __global__ void k_dot(const double* __restrict__ a, const double* __restrict__ b
 }
 
 // UPDATE_GPTV (Fasolt3.f90:254-284) on field-major arrays.
+// ---------------------------------------------------------------------------
+// Newton iteration on the device (NrDev): the residual-rule solve's stopping
+// decisions and the damped update, so that a whole iteration can run as one
+// graph.  Every kernel below that takes the record returns at once when its
+// condition does not hold (a uniform read), which is how one captured
+// sequence serves solves of any length.
+// ---------------------------------------------------------------------------
+// Batches before the next residual check, from the residuals at the last two
+// checks and the sweeps between them: at most 80 % of the remainder that the
+// observed geometric rate predicts (so the check after them is passed by at
+// most one batch), at most kMaxChecks; one when the residual did not fall.
+__device__ __forceinline__ int check_batches(double rel, double prev_rel, int sweeps, int prev_sweeps, double tol,
+                                             int batch, int max_batches) {
+#pragma clang fp contract(off)
+    if (!(prev_rel > 0.0 && rel < prev_rel)) return 1;
+    const double rate = log(rel / prev_rel) / (double)(sweeps - prev_sweeps);
+    const double sw = 0.8 * (log(tol / rel) / rate);
+    if (!(sw < (double)(max_batches * batch))) return max_batches;
+    return (int)sw / batch;
+}
+
+__global__ void k_set_omega(NrDev* nd, double omega) {
+    if (threadIdx.x == 0) nd->omega = omega;
+}
+
+// Inputs of a Newton iteration run as a graph (written before its launch).
+__global__ void k_nr_set(NrDev* nd, int inc, double dt, double omega, uint64_t epoch) {
+    if (threadIdx.x == 0) {
+        nd->inc = inc;
+        nd->dt = dt;
+        nd->omega = omega;
+        nd->epoch = epoch;
+    }
+}
+
+__global__ void k_epoch_inc(NrDev* nd) {
+    if (threadIdx.x == 0) nd->epoch += 1;
+}
+
+__global__ void k_flags_to_nr(const int* flags, NrDev* nd) {
+    if (threadIdx.x == 0) {
+        nd->flags[0] = flags[0];
+        nd->flags[1] = flags[1];
+    }
+}
+
+// Start of a residual-rule solve (Stif_sol.f90:179 starts from zero; the
+// caller clears ddx).  flags non-null: the ELVALS code in flags[1] decides
+// whether the iteration goes on (codes above 500 other than 987 and 910 end
+// NRITER before the solve).
+__global__ void k_solve_init(NrDev* nd, const int* flags, int maxit) {
+    if (threadIdx.x != 0) return;
+    int fatal = 0;
+    if (flags) {
+        const int e = flags[1];
+        fatal = e < 0 || (e > 500 && e != 987 && e != 910);
+    }
+    nd->fatal = fatal;
+    nd->updated = fatal;
+    nd->stop = fatal || maxit <= 0;
+    nd->sweeps = 0;
+    nd->left = 1;
+    nd->check_now = 0;
+    nd->cycle = 1;
+    nd->prev_sweeps = 0;
+    nd->rel = 1.0;
+    nd->prev_rel = -1.0;
+    nd->final_rate = 0.0;
+}
+
+__global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// Mixed precision, first batch of a stretch: the FP32 correction equation
+// A d = r from d = 0, r the FP64 residual of the previous check.
+__global__ void k_cycle_start(const NrDev* nd, const double* __restrict__ r, float* __restrict__ r32,
+                              float* __restrict__ d32, int n) {
+    if (nd->stop || !nd->cycle) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        r32[i] = (float)r[i];
+        d32[i] = 0.0f;
+    }
+}
+
+__global__ void k_solve_after(NrDev* nd, int batch) {
+    if (threadIdx.x != 0 || nd->stop) return;
+    nd->sweeps += batch;
+    nd->left -= 1;
+    nd->cycle = 0;
+    nd->check_now = nd->left <= 0;
+}
+
+// Mixed precision, end of a stretch: ddx += d.
+__global__ void k_cycle_end(const NrDev* nd, double* __restrict__ y, const float* __restrict__ d, int n) {
+    if (!nd->check_now) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = y[i] + (double)d[i];
+}
+
+// Residual check after a stretch: relative Jacobi-scaled residual from the
+// residual kernel's partials; stop below tol or at maxit sweeps, otherwise
+// the next stretch's length.  The solve's asymptotic rate (log residual
+// reduction per sweep) is kept from the latest stretch of at least four
+// batches.
+__global__ __launch_bounds__(kBlock) void k_solve_check(const double* __restrict__ partial, int nblocks, NrDev* nd,
+                                                        double tol, int maxit, int batch, int max_batches) {
+#pragma clang fp contract(off)
+    if (!nd->check_now) return;
+    double r[4];
+    reduce_partials(partial, nblocks, r);
+    if (threadIdx.x != 0) return;
+    const double rel = r[1] > 0.0 ? sqrt(r[0] / r[1]) : 0.0;
+    const int sweeps = nd->sweeps;
+    const double prev_rel = nd->prev_rel;
+    const int prev_sweeps = nd->prev_sweeps;
+    if (prev_rel > 0.0 && rel > 0.0 && rel < prev_rel && sweeps - prev_sweeps >= 4 * batch)
+        nd->final_rate = log(rel / prev_rel) / (double)(sweeps - prev_sweeps);
+    nd->rel = rel;
+    nd->check_now = 0;
+    if (rel < tol) {
+        nd->stop = 1;
+        return;
+    }
+    int b = check_batches(rel, prev_rel, sweeps, prev_sweeps, tol, batch, max_batches);
+    nd->prev_rel = rel;
+    nd->prev_sweeps = sweeps;
+    if (sweeps >= maxit) {
+        nd->stop = 1;
+        return;
+    }
+    const int cap = (maxit - sweeps) / batch;
+    b = b < cap ? b : cap;
+    nd->left = b > 1 ? b : 1;
+    nd->cycle = 1;
+}
+
+// The Newton update of Stif_sol.f90:66-74 on the device.  gate non-null: run
+// only once the solve has stopped and before the update has run.
+__device__ __forceinline__ bool update_gate(const NrDev* g) { return !g || (g->stop && !g->updated); }
+
+__global__ void k_norms2_g(const double* __restrict__ a, const double* __restrict__ b, int n, double* partial,
+                           const NrDev* gate) {
+    if (!update_gate(gate)) return;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        v[0] += a[i] * a[i];
+        v[1] += b[i] * b[i];
+    }
+    block_partials<4>(v, partial, blockIdx.x);
+}
+
+// sdx = |ddx| / |dx| (toler when dx = 0) and the damping alp2 = 1 / (1 + alpha sdx).
+__global__ __launch_bounds__(kBlock) void k_nr_alpha(const double* __restrict__ partial, int nblocks, NrDev* nd,
+                                                     double toler, double alpha, const NrDev* gate) {
+    // the host's arithmetic (no fused multiply-add), as the reference evaluates it
+#pragma clang fp contract(off)
+    if (!update_gate(gate)) return;
+    double r[4];
+    reduce_partials(partial, nblocks, r);
+    if (threadIdx.x != 0) return;
+    for (int q = 0; q < 4; ++q) nd->nrm[q] = r[q];
+    const double errdx = sqrt(r[0]), erddx = sqrt(r[1]);
+    const double sdx = errdx == 0.0 ? toler : erddx / errdx;
+    nd->sdx = sdx;
+    nd->alp2 = 1.0 / (1.0 + alpha * sdx);
+}
+
+__global__ void k_nr_update_g(double* __restrict__ x, double* __restrict__ dx, const double* __restrict__ ddx,
+                              const NrDev* nd, int n, const NrDev* gate) {
+    if (!update_gate(gate)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double xc = nd->alp2 * ddx[i];
+        x[i] = x[i] - xc;
+        dx[i] = dx[i] - xc;
+    }
+}
+
+__global__ void k_nr_mark(NrDev* nd) {
+    if (threadIdx.x == 0 && nd->stop && !nd->updated) nd->updated = 1;
+}
+
 __global__ void k_commit(int n, int nsv, double* eps, const double* dep, double* rt, const double* dr, double* ep,
                          const double* de, double* st, const double* ds, double* sv, const double* dsv) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1444,6 +1648,8 @@ extern "C" int cpfem_create(const cpfem_mesh* m, const cpfem_material* mats, int
         TRY(dalloc(c, &c->red.partial, (size_t)nbr * 4));
         TRY(dalloc(c, &c->red.result, 8));
         TRY(hipHostMalloc((void**)&c->red.host, 8 * sizeof(double), 0));
+        TRY(dalloc(c, &c->nrdev, 1));
+        TRY(hipHostMalloc((void**)&c->nrhost, sizeof(NrDev), 0));
         // material
         {
             // one device record per material, indexed by |le| - 1
@@ -1469,6 +1675,7 @@ fail:
 }
 
 static void drop_graphs(Ctx* c);
+static void drop_solve_graphs(Ctx* c);
 
 extern "C" int cpfem_destroy(cpfem_ctx* c) {
     if (!c) return 0;
@@ -1476,8 +1683,10 @@ extern "C" int cpfem_destroy(cpfem_ctx* c) {
     calib_free(c);
     free_schedule(c->sched);
     drop_graphs(c);
+    drop_solve_graphs(c);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->red.host) (void)hipHostFree(c->red.host);
+    if (c->nrhost) (void)hipHostFree(c->nrhost);
     for (int i = 0; i < Ctx::kEvents; ++i)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1831,6 +2040,159 @@ static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
     return fl[1];
 }
 
+// ELVALS enqueued with every decision and every report value on the device
+// (graph capture): the twin-draw epoch is advanced and read on the device, the
+// constitutive schedule reads its counts there, the sums land in nd.
+static int elvals_enqueue_dev(Ctx* c, NrDev* nd) {
+    hipStream_t s = c->stream;
+    hipLaunchKernelGGL(k_epoch_inc, dim3(1), dim3(64), 0, s, nd);
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, c->flags + 1, (size_t)1, 0);
+    // point kinematics (strain and rotation increments, element checks)
+    KinArgs K;
+    K.n = c->ngps;
+    K.gp_elem = c->gp_elem;
+    K.gp_local = c->gp_local;
+    K.elem_nodes = c->elem_nodes;
+    K.elem_lt = c->elem_lt;
+    K.elem_nd = c->elem_nd;
+    K.x = c->x;
+    K.dx = c->dx;
+    K.smale = c->ctl.smale;
+    K.smalr = c->ctl.smalr;
+    K.de = c->de;
+    K.r = c->dr;
+    K.kerr = c->sched.ck_kerr;
+    if (c->npts8 > 0) hipLaunchKernelGGL(k_kin<8>, dim3(nblk(c->ngps)), dim3(kBlock), 0, s, K);
+    if (c->npts20 > 0) hipLaunchKernelGGL(k_kin<20>, dim3(nblk(c->ngps)), dim3(kBlock), 0, s, K);
+    if (c->npts10 > 0) hipLaunchKernelGGL(k_kin<10>, dim3(nblk(c->ngps)), dim3(kBlock), 0, s, K);
+    if (c->npts15 > 0) hipLaunchKernelGGL(k_kin<15>, dim3(nblk(c->ngps)), dim3(kBlock), 0, s, K);
+    ConstArgs A;
+    memset(&A, 0, sizeof(A));
+    A.n = c->ngps;
+    A.mesh = 1;
+    A.gp_elem = c->gp_elem;
+    A.elem_le = c->elem_le;
+    A.de = c->de;
+    A.r = c->dr;
+    A.dep = c->dep;
+    A.st = c->st;
+    A.ds = c->ds;
+    A.sv = c->sv;
+    A.dsv = c->dsv;
+    A.t = c->t;
+    A.chi = c->chi;
+    A.erepmd = c->erepmd;
+    A.srate = c->srate;
+    A.twinsys = c->twinsys;
+    A.timestwinned = c->timestwinned;
+    A.flops = c->flops;
+    A.transc = c->transc;
+    A.err = c->gperr;
+    A.tflag = c->flags;
+    A.errmax = c->flags + 1;
+    A.rng_epoch_dev = &nd->epoch;
+    A.dt_dev = &nd->dt;
+    const DevControl C = make_control(c->ctl);
+    CPFEM_HIP(run_constitutive_dev(c->knss, c->dmat, C, 0.0, A, c->sched, s, nd->report));
+    // zero fc and the matrix values
+    const size_t D = 3 * (size_t)c->nnod, nv = c->A.nvals();
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, s, c->fc, D, 0.0);
+    if (c->precision == 1) {
+        hipLaunchKernelGGL(k_fill<float>, dim3(std::min(nblk(nv), 65536)), dim3(kBlock), 0, s, (float*)c->A.vals, nv,
+                           0.0f);
+    } else {
+        hipLaunchKernelGGL(k_fill<double>, dim3(std::min(nblk(nv), 65536)), dim3(kBlock), 0, s, (double*)c->A.vals, nv,
+                           0.0);
+    }
+    AsmArgs AA;
+    AA.elem_nodes = c->elem_nodes;
+    AA.elem_gps = c->elem_gps;
+    AA.elem_lt = c->elem_lt;
+    AA.elem_np = c->elem_np;
+    AA.scatter = c->A.scatter;
+    AA.scat_start = c->A.scat_start;
+    AA.x = c->x;
+    AA.chi = c->chi;
+    AA.t = c->t;
+    AA.ngps = c->ngps;
+    AA.fc = c->fc;
+    const bool f32 = c->precision == 1;
+    const int ncol_el = ((int)c->ekstart.size() - 1) / kKinds;
+    for (int k = 0; k < ncol_el; ++k) {
+        for (int q = 0; q < kKinds; ++q) {
+            const int b0 = c->ekstart[kKinds * k + q], cnt = c->ekstart[kKinds * k + q + 1] - b0;
+            if (cnt == 0) continue;
+            AA.nel_color = cnt;
+            AA.elems = c->elem_by_color + b0;
+            if (q == kElem8) {
+                const int epb = c->asm_epb;
+                const int nb = (cnt + epb - 1) / epb;
+                if (f32)
+                    hipLaunchKernelGGL(k_assemble<float>, dim3(nb), dim3(64 * epb), 0, s, AA, (float*)c->A.vals);
+                else
+                    hipLaunchKernelGGL(k_assemble<double>, dim3(nb), dim3(64 * epb), 0, s, AA, (double*)c->A.vals);
+            } else if (q == kElem20) {
+                if (f32)
+                    hipLaunchKernelGGL((k_assemble_tab<float, 20>), dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
+                else
+                    hipLaunchKernelGGL((k_assemble_tab<double, 20>), dim3(cnt), dim3(256), 0, s, AA,
+                                       (double*)c->A.vals);
+            } else if (q == kElem10) {
+                if (f32)
+                    hipLaunchKernelGGL((k_assemble_tab<float, 10>), dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
+                else
+                    hipLaunchKernelGGL((k_assemble_tab<double, 10>), dim3(cnt), dim3(256), 0, s, AA,
+                                       (double*)c->A.vals);
+            } else if (q == kElem15) {
+                if (f32)
+                    hipLaunchKernelGGL((k_assemble_tab<float, 15>), dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
+                else
+                    hipLaunchKernelGGL((k_assemble_tab<double, 15>), dim3(cnt), dim3(256), 0, s, AA,
+                                       (double*)c->A.vals);
+            } else {
+                if (f32)
+                    hipLaunchKernelGGL(k_ties<float>, dim3(nblk(cnt)), dim3(kBlock), 0, s, cnt, AA.elems, c->elem_lt,
+                                       c->A.scatter, c->A.scat_start, c->ctl.efix, (float*)c->A.vals);
+                else
+                    hipLaunchKernelGGL(k_ties<double>, dim3(nblk(cnt)), dim3(kBlock), 0, s, cnt, AA.elems, c->elem_lt,
+                                       c->A.scatter, c->A.scat_start, c->ctl.efix, (double*)c->A.vals);
+            }
+        }
+    }
+    CPFEM_HIP(hipGetLastError());
+    // reductions for the report, into nd
+    const int nb = std::min(nblk(c->ngps), c->red.nblocks);
+    hipLaunchKernelGGL(k_point_sums, dim3(nb), dim3(kBlock), 0, s, (const double*)c->erepmd,
+                       (const long long*)c->flops, (const long long*)c->transc, (const int*)c->gperr, c->ngps,
+                       c->red.partial);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, s, (const double*)c->red.partial, nb, nd->sums);
+    hipLaunchKernelGGL(k_flags_to_nr, dim3(1), dim3(64), 0, s, (const int*)c->flags, nd);
+    return hipGetLastError() == hipSuccess ? 0 : CPFEM_E_HIP;
+}
+
+// SUVALS enqueued with the increment read on the device and the sums into nd.
+static void suvals_enqueue_dev(Ctx* c, NrDev* nd) {
+    hipStream_t s = c->stream;
+    BcArgs B;
+    B.nnod = c->nnod;
+    B.ib = c->ib;
+    B.diag_off = c->A.diag_off;
+    B.x = c->x;
+    B.dx = c->dx;
+    B.fc = c->fc;
+    B.xs = c->x_spec;
+    B.fs = c->f_spec;
+    B.efix = c->ctl.efix;
+    B.rhs = c->rhs;
+    B.inc_dev = &nd->inc;
+    const int nb = nblk(c->nnod);
+    if (c->precision == 1)
+        hipLaunchKernelGGL(k_suvals<float>, dim3(nb), dim3(kBlock), 0, s, B, (float*)c->A.vals, c->red.partial);
+    else
+        hipLaunchKernelGGL(k_suvals<double>, dim3(nb), dim3(kBlock), 0, s, B, (double*)c->A.vals, c->red.partial);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, s, (const double*)c->red.partial, nb, nd->suv);
+}
+
 extern "C" int cpfem_elvals(cpfem_ctx* c, int32_t inc, double* sumcpiter, int32_t* tflag) {
     if (!c) return CPFEM_E_ARG;
     return elvals_impl(c, inc, sumcpiter, tflag);
@@ -1851,6 +2213,7 @@ static int suvals_impl(Ctx* c, int inc, double* sfc) {
     B.fs = c->f_spec + (size_t)(inc - 1) * 99 * 3;
     B.efix = c->ctl.efix;
     B.rhs = c->rhs;
+    B.inc_dev = nullptr;
     const int nb = nblk(c->nnod);
     if (c->precision == 1)
         hipLaunchKernelGGL(k_suvals<float>, dim3(nb), dim3(kBlock), 0, s, B, (float*)c->A.vals, c->red.partial);
@@ -1883,12 +2246,14 @@ static const int kMaxCheckBatches = 64;
 static int group_per_xcd(int ng, int wpb = kWavesPerBlock) { return ((ng + wpb - 1) / wpb + kXcd - 1) / kXcd; }
 
 template <typename T, typename V, bool ACC>
-static void sor_sweep(Ctx* c, const T* vals, const V* b, V* y, double* partial) {
+static void sor_sweep(Ctx* c, const T* vals, const V* b, V* y, double* partial, const int* stop = nullptr,
+                      const double* omega_dev = nullptr) {
     SorArgs S;
     S.P = c->A.P;
     S.cols = c->A.cols;
     S.omega = c->ctl.sor_omega;
-    S.stop = c->flags + 2;
+    S.stop = stop ? stop : c->flags + 2;
+    S.omega_dev = omega_dev;
     int slot = 0;
     for (int k = 0; k < c->A.ncolor; ++k) {
         S.nrows = c->A.cstart[k + 1] - c->A.cstart[k];
@@ -1989,7 +2354,10 @@ static hipError_t run_batch(Ctx* c, int which) {
     return hipGraphLaunch(gx, c->stream);
 }
 
-static double residual(Ctx* c, const double* b, const double* y, double* res) {
+// r = b - A y in FP64 (res non-null: r kept there) with the partial sums of
+// the Jacobi-scaled norms in red.partial; gate non-null: only while *gate.
+// Returns the number of partial slots.
+static int residual_enqueue(Ctx* c, const double* b, const double* y, double* res, const int* gate = nullptr) {
     SorArgs S;
     S.nrows = c->A.ngroups;
     S.gbase = 0;
@@ -1997,6 +2365,7 @@ static double residual(Ctx* c, const double* b, const double* y, double* res) {
     S.cols = c->A.cols;
     S.omega = c->ctl.sor_omega;
     S.stop = nullptr;
+    S.gate = gate;
     S.per_xcd = group_per_xcd(c->A.ngroups);
     const int nb = S.per_xcd * kXcd;
     if (c->precision == 1) {
@@ -2014,9 +2383,120 @@ static double residual(Ctx* c, const double* b, const double* y, double* res) {
             hipLaunchKernelGGL((k_residual<double, 0>), dim3(nb), dim3(kBlock), 0, c->stream, S,
                                (const double*)c->A.vals, b, y, res, c->red.partial);
     }
+    return nb;
+}
+
+static double residual(Ctx* c, const double* b, const double* y, double* res) {
+    const int nb = residual_enqueue(c, b, y, res);
     double r[4];
     reduce_now(c, nb, r);
     return r[1] > 0.0 ? sqrt(r[0] / r[1]) : 0.0;
+}
+
+// One unit of a residual-rule solve: a batch of kSweepBatch sweeps and the
+// stopping-rule kernels after it (NrDev).  The sweeps read the relaxation
+// factor from the record.  FP64: the sweeps relax A ddx = rhs.  Mixed: a
+// stretch between two checks is one refinement cycle, the FP32 sweeps relax
+// A d = r from d = 0 with r the FP64 residual in xold, and ddx += d at its end.
+static void enqueue_unit(Ctx* c) {
+    hipStream_t s = c->stream;
+    NrDev* nd = c->nrdev;
+    const int D = 3 * c->nnod;
+    const bool mixed = c->precision != 0;
+    if (mixed)
+        hipLaunchKernelGGL(k_cycle_start, dim3(nblk(D)), dim3(kBlock), 0, s, (const NrDev*)nd,
+                           (const double*)c->xold, c->rhs32, c->del32, D);
+    for (int k = 0; k < kSweepBatch; ++k) {
+        if (mixed)
+            sor_sweep<float, float, false>(c, sweep_vals32(c), c->rhs32, c->del32, c->red.partial, &nd->stop,
+                                           &nd->omega);
+        else
+            sor_sweep<double, double, false>(c, (const double*)c->A.vals, c->rhs, c->ddx, c->red.partial, &nd->stop,
+                                             &nd->omega);
+    }
+    hipLaunchKernelGGL(k_solve_after, dim3(1), dim3(64), 0, s, nd, kSweepBatch);
+    if (mixed)
+        hipLaunchKernelGGL(k_cycle_end, dim3(nblk(D)), dim3(kBlock), 0, s, (const NrDev*)nd, c->ddx,
+                           (const float*)c->del32, D);
+    const int nb = residual_enqueue(c, c->rhs, c->ddx, mixed ? c->xold : nullptr, &nd->check_now);
+    hipLaunchKernelGGL(k_solve_check, dim3(1), dim3(kBlock), 0, s, (const double*)c->red.partial, nb, nd,
+                       c->ctl.sor_tol, c->ctl.sor_maxit, kSweepBatch, kMaxCheckBatches);
+}
+
+// Start of a residual-rule solve: ddx = 0 (unless warm), the record's
+// stopping-rule state, the relaxation factor, and (mixed) the first
+// refinement cycle's residual r = rhs.  flags: see k_solve_init.
+static void enqueue_solve_init(Ctx* c, bool warm, const int* flags) {
+    hipStream_t s = c->stream;
+    const int D = 3 * c->nnod;
+    if (!warm) hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, s, c->ddx, (size_t)D, 0.0);
+    hipLaunchKernelGGL(k_solve_init, dim3(1), dim3(64), 0, s, c->nrdev, flags, c->ctl.sor_maxit);
+    if (c->precision != 0)
+        hipLaunchKernelGGL(k_copy, dim3(nblk(D)), dim3(kBlock), 0, s, c->xold, (const double*)c->rhs, D);
+}
+
+// The damped Newton update; gated: only after the solve has stopped and once.
+static void enqueue_update(Ctx* c, bool gated) {
+    hipStream_t s = c->stream;
+    NrDev* nd = c->nrdev;
+    const NrDev* gate = gated ? nd : nullptr;
+    const int D = 3 * c->nnod;
+    const int nb = std::min(nblk(D), c->red.nblocks);
+    hipLaunchKernelGGL(k_norms2_g, dim3(nb), dim3(kBlock), 0, s, (const double*)c->dx, (const double*)c->ddx, D,
+                       c->red.partial, gate);
+    hipLaunchKernelGGL(k_nr_alpha, dim3(1), dim3(kBlock), 0, s, (const double*)c->red.partial, nb, nd,
+                       c->ctl.toler, c->ctl.alpha, gate);
+    hipLaunchKernelGGL(k_nr_update_g, dim3(nblk(D)), dim3(kBlock), 0, s, c->x, c->dx, (const double*)c->ddx,
+                       (const NrDev*)nd, D, gate);
+    if (gated) hipLaunchKernelGGL(k_nr_mark, dim3(1), dim3(64), 0, s, nd);
+}
+
+// Capture of the kernels enqueue(c) puts on the context's stream as an
+// executable graph (kernel nodes only).
+template <typename F>
+static hipError_t capture(Ctx* c, F enqueue, hipGraph_t* src, hipGraphExec_t* exec) {
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    const int rc = enqueue();
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(c->stream, &g);
+    if (e == hipSuccess && rc != 0) e = hipErrorInvalidValue;
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return e;
+    }
+    e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        *exec = nullptr;
+        return e;
+    }
+    *src = g;
+    return hipSuccess;
+}
+
+static void check_solve_graphs(Ctx* c);
+
+// One unit: a replay of its graph or (CPFEM_SOR_GRAPHS=0) its kernels.
+static hipError_t run_unit(Ctx* c) {
+    if (!c->use_graphs) {
+        enqueue_unit(c);
+        return hipGetLastError();
+    }
+    check_solve_graphs(c);
+    if (!c->g_unit) {
+        hipError_t e = capture(
+            c, [&]() { enqueue_unit(c); return hipGetLastError() == hipSuccess ? 0 : 1; }, &c->g_unit_src,
+            &c->g_unit);
+        if (e != hipSuccess) return e;
+    }
+    return hipGraphLaunch(c->g_unit, c->stream);
+}
+
+static hipError_t read_nrdev(Ctx* c) {
+    hipError_t e = hipMemcpyAsync(c->nrhost, c->nrdev, sizeof(NrDev), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e;
 }
 
 static int autotune_impl(Ctx* c, int flags);
@@ -2092,6 +2572,7 @@ static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out, bool warm = fal
     CPFEM_HIP(hipEventRecord(c->ev[5], s));
     if (!warm) hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, s, c->ddx, (size_t)D, 0.0);
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, c->flags + 2, (size_t)2, 0);
+    hipLaunchKernelGGL(k_set_omega, dim3(1), dim3(64), 0, s, c->nrdev, c->ctl.sor_omega);
     int sweeps = 0, code = 0;
     double err = 0.0;
     if (c->precision == 0 && c->ctl.sor_kind == 0) {
@@ -2119,43 +2600,28 @@ static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out, bool warm = fal
         // more than one batch).  In mixed precision each stretch between two
         // residuals is one refinement cycle: the sweeps run in FP32 on the
         // correction equation A d = r from d = 0, r formed in FP64 by the
-        // previous residual, then ddx += d.
-        const bool mixed = c->precision != 0;
-        double rel = 1.0, prev_rel = -1.0;
-        int prev_sweeps = 0, batches = 1;
-        if (mixed) CPFEM_HIP(hipMemcpyAsync(c->xold, c->rhs, D * sizeof(double), hipMemcpyDeviceToDevice, s));
-        const int which = mixed ? 2 : 1;
-        while (sweeps < maxit) {
-            batches = std::max(1, std::min(batches, (maxit - sweeps) / kSweepBatch));
-            if (mixed) {
-                hipLaunchKernelGGL(k_to_float, dim3(nblk(D)), dim3(kBlock), 0, s, (const double*)c->xold, c->rhs32,
-                                   D);
-                hipLaunchKernelGGL(k_fill<float>, dim3(nblk(D)), dim3(kBlock), 0, s, c->del32, (size_t)D, 0.0f);
-            }
+        // previous residual, then ddx += d.  The decisions are taken on the
+        // device (enqueue_unit); the host replays the stretch's units and
+        // reads the record after each check.
+        enqueue_solve_init(c, true, nullptr);
+        CPFEM_HIP(hipGetLastError());
+        const NrDev& h = *c->nrhost;
+        int left = 1;
+        while (true) {
             CPFEM_HIP(hipEventRecord(c->ev[7], s));
-            for (int k = 0; k < batches; ++k) CPFEM_HIP(run_batch(c, which));
+            for (int k = 0; k < left; ++k) CPFEM_HIP(run_unit(c));
             CPFEM_HIP(hipEventRecord(c->ev[8], s));
-            if (mixed)
-                hipLaunchKernelGGL(k_add_float, dim3(nblk(D)), dim3(kBlock), 0, s, c->ddx, (const float*)c->del32, D);
-            sweeps += batches * kSweepBatch;
-            rel = residual(c, c->rhs, c->ddx, mixed ? c->xold : nullptr);
+            CPFEM_HIP(read_nrdev(c));
             c->stats.t_sweep_ms += elapsed(c->ev[7], c->ev[8]);
-            c->stats.sweep_launches += (int64_t)batches * kSweepBatch * c->A.ncolor;
-            // asymptotic rate of the solve: the latest stretch of at least four batches
-            if (prev_rel > 0.0 && rel > 0.0 && rel < prev_rel && sweeps - prev_sweeps >= 4 * kSweepBatch)
-                final_rate = log(rel / prev_rel) / (double)(sweeps - prev_sweeps);
-            if (rel < c->ctl.sor_tol) break;
-            batches = 1;
-            if (prev_rel > 0.0 && rel < prev_rel) {
-                const double rate = log(rel / prev_rel) / (double)(sweeps - prev_sweeps);
-                const double need = log(c->ctl.sor_tol / rel) / rate;
-                batches = std::min((int)(0.8 * need) / kSweepBatch, kMaxCheckBatches);
-            }
-            prev_rel = rel;
-            prev_sweeps = sweeps;
+            c->stats.sweep_launches += (int64_t)left * kSweepBatch * c->A.ncolor;
+            if (h.stop) break;
+            left = h.left;
         }
-        err = rel;
-        if (rel > 10.0 * c->ctl.sor_tol) code = 904;
+        sweeps = h.sweeps;
+        err = h.rel;
+        final_rate = h.final_rate;
+        c->last_batches = sweeps / kSweepBatch;
+        if (err > 10.0 * c->ctl.sor_tol) code = 904;
     }
     CPFEM_HIP(hipEventRecord(c->ev[6], s));
     CPFEM_HIP(hipEventSynchronize(c->ev[6]));
@@ -2193,6 +2659,27 @@ static void drop_graphs(Ctx* c) {
             (void)hipGraphDestroy(c->graph_src[i]);
             c->graph_src[i] = nullptr;
         }
+    }
+}
+
+static void drop_solve_graphs(Ctx* c) {
+    hipGraphExec_t* ex[3] = {&c->g_unit, &c->g_it, &c->g_more};
+    hipGraph_t* src[3] = {&c->g_unit_src, &c->g_it_src, &c->g_more_src};
+    for (int i = 0; i < 3; ++i) {
+        if (*ex[i]) (void)hipGraphExecDestroy(*ex[i]);
+        if (*src[i]) (void)hipGraphDestroy(*src[i]);
+        *ex[i] = nullptr;
+        *src[i] = nullptr;
+    }
+}
+
+// The unit and iteration graphs bake in the launch shapes: captured again
+// when one of them has changed since.
+static void check_solve_graphs(Ctx* c) {
+    const int key[4] = {c->sor_wpb, c->asm_epb, c->sched.first_iters, c->A.ncolor};
+    if (memcmp(key, c->g_key, sizeof(key)) != 0) {
+        drop_solve_graphs(c);
+        memcpy(c->g_key, key, sizeof(key));
     }
 }
 
@@ -2438,25 +2925,16 @@ extern "C" int cpfem_facsls(cpfem_ctx* c, int32_t* sweeps, double* error) {
 
 static int nr_update_impl(Ctx* c, double* sdx_out, double* alp2_out) {
     hipStream_t s = c->stream;
-    const int D = 3 * c->nnod;
     CPFEM_HIP(hipEventRecord(c->ev[5], s));
-    const int nb = std::min(nblk(D), c->red.nblocks);
-    hipLaunchKernelGGL(k_norms2, dim3(nb), dim3(kBlock), 0, s, (const double*)c->dx, (const double*)c->ddx, D,
-                       c->red.partial);
-    double r[4];
-    reduce_now(c, nb, r);
-    const double errdx = sqrt(r[0]), erddx = sqrt(r[1]);
-    const double sdx = (errdx == 0.0) ? c->ctl.toler : erddx / errdx;
-    const double alp2 = 1.0 / (1.0 + c->ctl.alpha * sdx);
-    hipLaunchKernelGGL(k_nr_update, dim3(nblk(D)), dim3(kBlock), 0, s, c->x, c->dx, (const double*)c->ddx, alp2, D);
+    enqueue_update(c, false);
     CPFEM_HIP(hipGetLastError());
     CPFEM_HIP(hipEventRecord(c->ev[6], s));
-    CPFEM_HIP(hipEventSynchronize(c->ev[6]));
+    CPFEM_HIP(read_nrdev(c));
     c->stats.t_update_ms += elapsed(c->ev[5], c->ev[6]);
-    c->stats.sdx = sdx;
-    c->stats.alp2 = alp2;
-    if (sdx_out) *sdx_out = sdx;
-    if (alp2_out) *alp2_out = alp2;
+    c->stats.sdx = c->nrhost->sdx;
+    c->stats.alp2 = c->nrhost->alp2;
+    if (sdx_out) *sdx_out = c->stats.sdx;
+    if (alp2_out) *alp2_out = c->stats.alp2;
     return 0;
 }
 
@@ -2465,14 +2943,125 @@ extern "C" int cpfem_nr_update(cpfem_ctx* c, double* sdx, double* alp2) {
     return nr_update_impl(c, sdx, alp2);
 }
 
+// Graph mode.  One Newton iteration is one launch of the iteration graph:
+// ELVALS (kinematics, the constitutive passes and tiers with their counts on
+// the device, assembly, point sums), SUVALS, precision 2's rounded matrix,
+// the solve's start, kItUnits units and the gated update.  A solve that needs
+// more batches continues in replays of the continuation graph (kMoreUnits
+// units and the gated update), as many as the last solve's length predicts
+// queued at once, then one at a time; the host reads the record once they
+// have run.  Units after the solve's stop are skipped kernel by kernel.  The
+// kernels and their order are those of the call-by-call path, so both give
+// the same iterates.
+static const int kItUnits = 2;
+static const int kMoreUnits = 4;
+
+static int enqueue_iteration(Ctx* c) {
+    NrDev* nd = c->nrdev;
+    const int rc = elvals_enqueue_dev(c, nd);
+    if (rc) return rc;
+    suvals_enqueue_dev(c, nd);
+    if (c->precision == 2) {
+        const size_t nv = c->A.nvals();
+        hipLaunchKernelGGL(k_round_vals, dim3((unsigned)std::min<size_t>((nv + kBlock - 1) / kBlock, 65536)),
+                           dim3(kBlock), 0, c->stream, (const double*)c->A.vals, c->A.vals32, nv);
+    }
+    enqueue_solve_init(c, false, c->flags);
+    for (int u = 0; u < kItUnits; ++u) enqueue_unit(c);
+    enqueue_update(c, true);
+    return hipGetLastError() == hipSuccess ? 0 : CPFEM_E_HIP;
+}
+
+static int enqueue_more(Ctx* c) {
+    for (int u = 0; u < kMoreUnits; ++u) enqueue_unit(c);
+    enqueue_update(c, true);
+    return hipGetLastError() == hipSuccess ? 0 : CPFEM_E_HIP;
+}
+
+// One Newton iteration in graph mode; the report values go to the stats.
+// Returns the iteration's code as the call-by-call sequence would.
+static int graph_iteration(Ctx* c, int inc, double* sdx) {
+    hipStream_t s = c->stream;
+    check_solve_graphs(c);
+    if (!c->g_it) CPFEM_HIP(capture(c, [&]() { return enqueue_iteration(c); }, &c->g_it_src, &c->g_it));
+    if (!c->g_more) CPFEM_HIP(capture(c, [&]() { return enqueue_more(c); }, &c->g_more_src, &c->g_more));
+    hipLaunchKernelGGL(k_nr_set, dim3(1), dim3(64), 0, s, c->nrdev, inc, c->tiem[inc - 1], c->ctl.sor_omega,
+                       c->rng_epoch);
+    const double omega_used = c->ctl.sor_omega;
+    CPFEM_HIP(hipEventRecord(c->ev[0], s));
+    CPFEM_HIP(hipGraphLaunch(c->g_it, s));
+    int launches = 1;
+    const int predicted = (c->last_batches - kItUnits + kMoreUnits - 1) / kMoreUnits;
+    for (int k = 0; k < predicted - 1; ++k, ++launches) CPFEM_HIP(hipGraphLaunch(c->g_more, s));
+    CPFEM_HIP(read_nrdev(c));
+    const NrDev& h = *c->nrhost;
+    while (!h.updated) {
+        CPFEM_HIP(hipGraphLaunch(c->g_more, s));
+        ++launches;
+        CPFEM_HIP(read_nrdev(c));
+    }
+    CPFEM_HIP(hipEventRecord(c->ev[1], s));
+    CPFEM_HIP(hipEventSynchronize(c->ev[1]));
+    c->rng_epoch += 1;
+    c->assembled = !h.fatal;
+    c->vals32_fresh = c->precision == 2 && !h.fatal;
+    cpfem_stats& S = c->stats;
+    S.graph_launches += launches;
+    S.t_graph_ms += elapsed(c->ev[0], c->ev[1]);
+    S.sumcpiter = h.sums[0] / (double)c->ngps;
+    S.flops_const += (int64_t)llround(h.sums[1]);
+    S.transc_const += (int64_t)llround(h.sums[2]);
+    S.gp_updates += c->ngps;
+    S.pts_987 += (int64_t)llround(h.sums[3]);
+    S.pts_987_last = (int32_t)llround(h.sums[3]);
+    S.pts_continued += h.report[0];
+    S.pts_wide += h.report[1];
+    if (h.report[2] > 0) return CPFEM_E_WIDTH;
+    const int ecode = h.flags[1];
+    if (ecode > S.err) S.err = ecode;
+    if (h.fatal) return ecode;
+    S.sfc = sqrt(h.suv[0]);
+    S.sor_sweeps = h.sweeps;
+    S.sor_sweeps_total += h.sweeps;
+    S.sor_error = h.rel;
+    S.sor_omega = omega_used;
+    S.sor_wpb = c->sor_wpb;
+    S.sweep_launches += (int64_t)h.sweeps * c->A.ncolor;
+    c->last_batches = h.sweeps / kSweepBatch;
+    int code = 0;
+    if (h.rel > 10.0 * c->ctl.sor_tol) code = 904;
+    if (code > S.err) S.err = code;
+    if (code == 0) refine_omega(c, h.final_rate);
+    S.sdx = h.sdx;
+    S.alp2 = h.alp2;
+    *sdx = h.sdx;
+    return code ? code : ecode;
+}
+
+extern "C" int cpfem_set_graph_mode(cpfem_ctx* c, int32_t on) {
+    if (!c) return CPFEM_E_ARG;
+    if (on && (c->ctl.sor_kind != 1)) return CPFEM_E_ARG;
+    c->graph_mode = on != 0;
+    return 0;
+}
+
 extern "C" int cpfem_nriter(cpfem_ctx* c, int32_t inc, cpfem_stats* st) {
     if (!c) return CPFEM_E_ARG;
+    if (inc < 1 || inc > c->ninc1) return CPFEM_E_ARG;
     const auto t0 = std::chrono::steady_clock::now();
     memset(&c->stats, 0, sizeof(c->stats));
     int nit = 0, code = 0;
     bool more = true;
     while (more) {
         ++nit;
+        if (c->graph_mode && c->ctl.sor_kind == 1 && !c->omega_auto) {
+            // the first solve of an automatic-omega context tunes on the host path
+            double sdx = 0.0;
+            code = graph_iteration(c, inc, &sdx);
+            if (code < 0 || (code > 500 && code != 987 && code != 910 && code != 904)) break;
+            more = (sdx >= c->ctl.toler) && (nit < c->ctl.maxit);
+            continue;
+        }
         double sumcp = 0.0, sfc = 0.0, sdx = 0.0, alp2 = 0.0;
         int tflag = 0, sweeps = 0;
         double serr = 0.0;

@@ -77,6 +77,34 @@ struct Matrix {
     size_t nvals() const { return (size_t)ngroups * P * 576; }
 };
 
+// Device record of the Newton iterations: the solve's stopping-rule state
+// (every decision of the residual-rule SOR is taken on the device, so a
+// solve runs the same kernels whether the host replays them batch by batch
+// or a whole Newton iteration is one graph launch), the Newton update's
+// norms and damping, and the report values the host reads after an
+// iteration.  Written by the kernels of mesh.hip; the host reads it back.
+struct NrDev {
+    int stop;          // nonzero: the sweeps and the checks are skipped
+    int sweeps;        // sweeps of the solve so far
+    int left;          // batches left before the next residual check
+    int check_now;     // the batch just run ends a stretch: refine, residual, check
+    int cycle;         // the next batch starts a stretch (mixed: r -> FP32, d = 0)
+    int prev_sweeps;   // sweeps at the previous check
+    int updated;       // the iteration's Newton update has run (or is skipped)
+    int fatal;         // the iteration's ELVALS code stops NRITER: no solve, no update
+    int inc;           // increment of the iteration (load tables)
+    int flags[2];      // tflag, largest point code of the ELVALS
+    int report[3];     // constitutive: continued points, wider tiers, too wide
+    uint64_t epoch;    // twin-draw stream epoch of the ELVALS
+    double dt;         // time step of the increment
+    double omega;      // relaxation factor the sweeps read
+    double rel, prev_rel, final_rate;
+    double sdx, alp2;
+    double sums[4];    // ELVALS point sums: sumcpiter, flops, transcendentals, 987 points
+    double suv[4];     // SUVALS sums
+    double nrm[4];     // |dx|^2, |ddx|^2
+};
+
 struct Ctx {
     int nlmnt = 0, nnod = 0, ngps = 0, nsv = 0, nss = 0, ninc1 = 0, nmat = 0;   // nss: largest of the materials
     int precision = 0;           // cpfem_control::precision (0 FP64, 1 FP32 matrix, 2 FP32 sweeps on an FP64 matrix)
@@ -124,6 +152,18 @@ struct Ctx {
     hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};  // sweep batches: FP64 update test, FP64, FP32
     hipGraph_t graph_src[3] = {nullptr, nullptr, nullptr};   // their captured graphs
     bool use_graphs = true;      // CPFEM_SOR_GRAPHS=0: launch the batches kernel by kernel
+    // residual-rule solves: the batch of sweeps with its stopping-rule kernels
+    // (one unit) as a graph; graph mode: each Newton iteration as one graph
+    // (ELVALS, SUVALS, solve start, it_units units, gated update) and its
+    // continuation (more_units units, gated update)
+    hipGraphExec_t g_unit = nullptr, g_it = nullptr, g_more = nullptr;
+    hipGraph_t g_unit_src = nullptr, g_it_src = nullptr, g_more_src = nullptr;
+    int it_units = 0, more_units = 0;
+    int g_key[4] = {0, 0, 0, 0};  // launch shapes the iteration graphs were captured with
+    bool graph_mode = false;     // cpfem_set_graph_mode
+    int last_batches = 0;        // batches of the last residual-rule solve
+    NrDev* nrdev = nullptr;      // device record
+    NrDev* nrhost = nullptr;     // pinned host copy
     int sor_wpb = 4;             // waves per workgroup of the colour sweeps (cpfem_autotune)
     int asm_epb = 4;             // elements (waves) per workgroup of the 8-node assembly (cpfem_autotune)
     bool omega_auto = false;     // sor_omega <= 0 at creation: tuned at the first solve

@@ -65,8 +65,9 @@ MODULE CPFEM_CAPI
         REAL(C_DOUBLE) :: sor_omega, t_tune_ms
         INTEGER(C_INT32_T) :: sor_wpb, tune_sweeps
         INTEGER(C_INT64_T) :: pts_987
-        INTEGER(C_INT32_T) :: pts_987_last, pad_
+        INTEGER(C_INT32_T) :: pts_987_last, graph_launches
         REAL(C_DOUBLE) :: t_const_pass_ms(4)
+        REAL(C_DOUBLE) :: t_graph_ms
     END TYPE cpfem_stats
 
     INTERFACE

@@ -172,10 +172,13 @@ typedef struct {
     int64_t pts_987;          /* point updates whose slip Newton ended above the 0.5 residual (FAFERR 987,
                                  crys-cons.f90:108), summed over the ELVALS calls */
     int32_t pts_987_last;     /* such points in the last ELVALS (for NRITER: its final iteration) */
-    int32_t pad_;
+    int32_t graph_launches;   /* graph mode: graph launches of the last cpfem_nriter (iteration graphs and their
+                                 continuations) */
     double t_const_pass_ms[4];/* device time of the constitutive passes: every point's first outer iterations,
                                  the continuation of the unfinished ones, the 8-wide and the 12-wide (or wider)
-                                 active-set tiers */
+                                 active-set tiers (call-by-call iterations only) */
+    double t_graph_ms;        /* graph mode: device time of the Newton iterations run as graphs (the per-phase
+                                 times above cover the call-by-call iterations only) */
 } cpfem_stats;
 
 /* Upload mesh, materials and control; build the block sparsity, colourings
@@ -252,6 +255,18 @@ int cpfem_facsls(cpfem_ctx *ctx, int32_t *sweeps, double *error);
 int cpfem_nr_update(cpfem_ctx *ctx, double *sdx, double *alp2);
 /* The whole of NRITER for increment inc (Stif_sol.f90:10-88), cumulative slip included. */
 int cpfem_nriter(cpfem_ctx *ctx, int32_t inc, cpfem_stats *stats);
+/* Graph mode of cpfem_nriter and cpfem_increment (on != 0; needs sor_kind 1,
+ * else CPFEM_E_ARG).  Replaces the host loop of NRITER's iteration body
+ * (Stif_sol.f90:32-76): each Newton iteration runs as one launch of a graph
+ * captured once per context (ELVALS, SUVALS, the solve's start, its first
+ * sweep batches with their residual checks, the damped update), continued by
+ * replays of a second graph of sweep batches when the solve needs more; every
+ * stopping decision of the solve is taken on the device.  Same kernels in the
+ * same order as the call-by-call path: the iterates are the same.  The host
+ * reads one record per iteration (plus one per continuation beyond the
+ * predicted number).  Per-phase times are not taken in graph mode
+ * (cpfem_stats.t_graph_ms holds the iterations' device time). */
+int cpfem_set_graph_mode(cpfem_ctx *ctx, int32_t on);
 /* Cumulative slip at the end of NRITER (Stif_sol.f90:80-84): cumslip += srate*tiem(inc). */
 int cpfem_cumslip(cpfem_ctx *ctx, int32_t inc);
 /* Power of the POWER report (Faslt_io_nowin.f90:353-366): sum(fc*dx). */
