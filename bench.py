@@ -16,7 +16,10 @@ Legs, in this order (each bounded by wall time, see --budget):
   3. The same two paths at the configured omega 1.6, one timed increment each.
   4. One Newton system of the workload solved by each path against a tight
      FP64 solve: the displacement error of each at the 1e-8 residual.
-  5. cpu_baseline: the FP64 oracle (the CPU restatement of the same algorithm,
+  5. Graph mode: FCC tension at 16^3, increments 2-6 call by call and with
+     each Newton iteration as one graph launch; time per increment of each
+     and whether the final states are bit for bit equal.
+  6. cpu_baseline: the FP64 oracle (the CPU restatement of the same algorithm,
      one core) on whole increments of the CPU-runnable 8^3 uniaxial-tension
      case, next to the GPU on the same increments.
 Timed steps stop early when a leg's time is spent; ``steps`` is the number of
@@ -342,6 +345,40 @@ def sweep_probe(n, sweeps, copy_mib):
     print(f"calibration copy: {nbytes} bytes read and {nbytes} bytes written", flush=True)
 
 
+def graph_mode_leg(capi, n, ninc):
+    """The same increments of FCC uniaxial tension (n^3 bricks x 8 points,
+    FP64, omega tuned at the first solve) run call by call and with each Newton
+    iteration as one graph launch (cpfem_set_graph_mode): wall time of
+    increments 2..ninc and whether the two final states are bit for bit equal."""
+    import dataclasses
+    from cpfem_code_amd.problem import Control, fcc_tension
+    ctl = Control(sor_kind=1, sor_tol=1e-8, sor_omega=0.0, sor_maxit=400000)
+    prob = fcc_tension(n, ninc=ninc, seed=0, control=ctl)
+    out, states = {}, {}
+    for mode in ("calls", "graph"):
+        sv = capi.Solver(prob, precision=0)
+        if mode == "graph":
+            sv.set_graph_mode(True)
+        sv.increment(1)
+        t0 = time.perf_counter()
+        launches = 0
+        nit = 0
+        for inc in range(2, ninc + 1):
+            st = sv.increment(inc)
+            launches += st["graph_launches"]
+            nit += st["nit"]
+        wall = time.perf_counter() - t0
+        states[mode] = sv.get_state()
+        sv.close()
+        out[mode] = {"s_per_increment": wall / (ninc - 1), "nr_iterations": nit, "graph_launches": launches}
+        log(f"[bench] graph leg {mode}: {wall / (ninc - 1) * 1e3:.1f} ms per increment, {nit} Newton iterations, "
+            f"{launches} graph launches")
+    out["same_state_bitwise"] = all(np.array_equal(states["calls"][k], states["graph"][k])
+                                    for k in ("x", "dx", "st", "sv"))
+    out["workload"] = f"FCC uniaxial tension, {n}^3 bricks x 8 points, increments 2-{ninc}, FP64"
+    return out
+
+
 def calibration_leg(capi_mod, n, ctl):
     """Gradient of the macroscopic stress-strain curve (FCC tension, 50
     increments, FP64) by the adjoint, next to central differences."""
@@ -393,6 +430,9 @@ def main():
                     help="1: tune the constitutive first-pass budget and the assembly's elements per workgroup "
                          "after the warm-up (cpfem_autotune_points, untimed); 0: the defaults (8, 4)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample; 0 skips it")
+    ap.add_argument("--graph-leg", type=int, default=16,
+                    help="mesh size of the graph-mode comparison leg (each Newton iteration as one graph launch "
+                         "against the call-by-call path); 0 skips it")
     ap.add_argument("--calibration", type=int, default=0, help="mesh size of an opt-in calibration leg (0: none)")
     ap.add_argument("--trace-iterations", action="store_true",
                     help="drive each increment through the per-step calls and print every Newton iteration")
@@ -454,6 +494,7 @@ def main():
     acc = None
     if args.accuracy and args.workload == "plane_strain" and time.time() - t_start < B:
         acc = mixed_accuracy(prob, capi)
+    graph = graph_mode_leg(capi, args.graph_leg, 6) if args.graph_leg > 0 else None
     cpu = cpu_baseline(capi, args.cpu_seconds) if args.cpu_seconds > 0 else None
     h = legs["fp64"]
     roof = roofline_of(prob, recs64, traffic, "k_sor_color<double>", 8)
@@ -483,6 +524,7 @@ def main():
         "cpu_baseline": cpu,
         "legs": legs,
         "mixed_accuracy_one_system": acc,
+        "graph_mode": graph,
         "device": f"{name} ({cus} CUs)",
         "note": ("value: point updates (points x Newton iterations) per second of whole-increment wall time on "
                  "the FP64 path over the timed increments (their indices in legs.fp64.increments_timed); the "
