@@ -828,6 +828,7 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
     A.ovf_count = S.counts + 2;
     const int fi = S.first_iters > 0 ? S.first_iters : 8;
     A.iter_limit = fi < C.maxit ? fi : C.maxit;
+    const int wide_limit = S.wide_iters > 0 ? S.wide_iters : fi;
     for (int q = 0; q < 4; ++q) S.pass_ms[q] = 0.0;
     const bool timed = S.ev[0] != nullptr;
     auto mark = [&](int k) {
@@ -875,9 +876,21 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
         A.cont_count = S.counts + 1;
         A.ovf_list = S.list_a;
         A.ovf_count = S.counts + 0;
-        A.iter_limit = C.maxit;
+        // as in tier 1: at most first_iters outer iterations, then the
+        // unfinished points resume sorted by their inner iterations per outer
+        // one (the split is exact: a point resumes from its checkpoint)
+        A.iter_limit = wide_limit < C.maxit ? wide_limit : C.maxit;
         mark(2);
         launch_one<N, kAmaxMid>(dmat, C, dt, A, S.host_counts[2], stream);
+        if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
+        if (S.host_counts[1] > 0) {
+            if ((e = sort_points(S, S.list_b, S.host_counts[1], nullptr, stream)) != hipSuccess) return e;
+            A.in_list = S.sorted;
+            A.in_count = S.counts + 1;
+            A.cont_count = S.counts + 3;   // no point continues at iter_limit = maxit
+            A.iter_limit = C.maxit;
+            launch_one<N, kAmaxMid>(dmat, C, dt, A, S.host_counts[1], stream);
+        }
         mark(3);
         if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
         span(2, 2, 3);
@@ -995,6 +1008,14 @@ static hipError_t schedule_dev(const void* dmat, const DevControl& C, double dt,
     A.cont_count = S.counts + 1;
     A.ovf_list = S.list_a;
     A.ovf_count = S.counts + 0;
+    const int wide_limit = S.wide_iters > 0 ? S.wide_iters : fi;
+    A.iter_limit = wide_limit < C.maxit ? wide_limit : C.maxit;
+    launch_one<N, kAmaxMid>(dmat, C, dt, A, n, stream);
+    if ((e = sort_points(S, S.list_b, n, nullptr, stream, false, S.counts + 1)) != hipSuccess) return e;
+    A.in_list = S.sorted;
+    A.in_count = S.counts + 1;
+    A.cont_count = S.counts + 3;
+    A.iter_limit = C.maxit;
     launch_one<N, kAmaxMid>(dmat, C, dt, A, n, stream);
     // tier 3: the overflow of tier 2
     A.in_list = S.list_a;

@@ -31,6 +31,8 @@ struct ConstSchedule {
     int* host_counts = nullptr;// [4] pinned
     hipEvent_t ev[5] = {};     // around the passes of the last run (timing)
     int first_iters = 8;       // outer iterations of the first pass over every point (cpfem_autotune)
+    int wide_iters = 0;        // outer iterations (counted from the point's start) before the 8-wide tier's
+                               // unfinished points are compacted and resumed (0: first_iters)
     double pass_ms[4] = {};    // device time of the last run: pass 1, pass 2, 8-wide, 12(+)-wide tiers
 };
 
