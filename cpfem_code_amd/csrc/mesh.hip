@@ -117,6 +117,7 @@ __global__ void k_point_sums(const double* __restrict__ erepmd, const long long*
 // ---------------------------------------------------------------------------
 struct AsmArgs {
     int nel_color;
+    int per_xcd;                 // logical blocks per XCD of the 8-node launch
     const int* elems;            // elements of this colour
     const int* elem_nodes;
     const int* elem_gps;
@@ -140,7 +141,10 @@ __global__ __launch_bounds__(256) void k_assemble(const AsmArgs A, T* __restrict
     __shared__ T sh_s[EPB][8][6];
     __shared__ double sh_x[EPB][8][3];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int k = blockIdx.x * (int)(blockDim.x >> 6) + wave;
+    // consecutive elements of the colour (neighbours in the mesh, whose
+    // blocks share cache lines of the matrix) on one XCD, so that the partial
+    // lines they write meet in that XCD's L2 before they are written back
+    const int k = xcd_block(A.per_xcd) * (int)(blockDim.x >> 6) + wave;
     const bool live = k < A.nel_color;
     const int e = live ? A.elems[k] : 0;
     const int ng = live ? A.elem_np[e] : 0;
@@ -1886,6 +1890,79 @@ extern "C" int cpfem_update_node(cpfem_ctx* c) {
     return 0;
 }
 
+// fc = 0, A = 0 and the element assembly (Elvals.f90:80-221), colour by
+// colour (the elements of one colour share no node), on the context's stream.
+static void enqueue_assembly(Ctx* c) {
+    hipStream_t s = c->stream;
+    // zero fc and the matrix values
+    const size_t D = 3 * (size_t)c->nnod, nv = c->A.nvals();
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, s, c->fc, D, 0.0);
+    if (c->precision == 1) {
+        hipLaunchKernelGGL(k_fill<float>, dim3(std::min(nblk(nv), 65536)), dim3(kBlock), 0, s, (float*)c->A.vals, nv,
+                           0.0f);
+    } else {
+        hipLaunchKernelGGL(k_fill<double>, dim3(std::min(nblk(nv), 65536)), dim3(kBlock), 0, s, (double*)c->A.vals, nv,
+                           0.0);
+    }
+    AsmArgs AA;
+    AA.per_xcd = 0;
+    AA.elem_nodes = c->elem_nodes;
+    AA.elem_gps = c->elem_gps;
+    AA.elem_lt = c->elem_lt;
+    AA.elem_np = c->elem_np;
+    AA.scatter = c->A.scatter;
+    AA.scat_start = c->A.scat_start;
+    AA.x = c->x;
+    AA.chi = c->chi;
+    AA.t = c->t;
+    AA.ngps = c->ngps;
+    AA.fc = c->fc;
+    const bool f32 = c->precision == 1;
+    const int ncol_el = ((int)c->ekstart.size() - 1) / kKinds;
+    for (int k = 0; k < ncol_el; ++k) {
+        for (int q = 0; q < kKinds; ++q) {
+            const int b0 = c->ekstart[kKinds * k + q], cnt = c->ekstart[kKinds * k + q + 1] - b0;
+            if (cnt == 0) continue;
+            AA.nel_color = cnt;
+            AA.elems = c->elem_by_color + b0;
+            if (q == kElem8) {
+                const int epb = c->asm_epb;
+                AA.per_xcd = ((cnt + epb - 1) / epb + kXcd - 1) / kXcd;
+                const int nb = AA.per_xcd * kXcd;
+                if (f32)
+                    hipLaunchKernelGGL(k_assemble<float>, dim3(nb), dim3(64 * epb), 0, s, AA, (float*)c->A.vals);
+                else
+                    hipLaunchKernelGGL(k_assemble<double>, dim3(nb), dim3(64 * epb), 0, s, AA, (double*)c->A.vals);
+            } else if (q == kElem20) {
+                if (f32)
+                    hipLaunchKernelGGL((k_assemble_tab<float, 20>), dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
+                else
+                    hipLaunchKernelGGL((k_assemble_tab<double, 20>), dim3(cnt), dim3(256), 0, s, AA,
+                                       (double*)c->A.vals);
+            } else if (q == kElem10) {
+                if (f32)
+                    hipLaunchKernelGGL((k_assemble_tab<float, 10>), dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
+                else
+                    hipLaunchKernelGGL((k_assemble_tab<double, 10>), dim3(cnt), dim3(256), 0, s, AA,
+                                       (double*)c->A.vals);
+            } else if (q == kElem15) {
+                if (f32)
+                    hipLaunchKernelGGL((k_assemble_tab<float, 15>), dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
+                else
+                    hipLaunchKernelGGL((k_assemble_tab<double, 15>), dim3(cnt), dim3(256), 0, s, AA,
+                                       (double*)c->A.vals);
+            } else {
+                if (f32)
+                    hipLaunchKernelGGL(k_ties<float>, dim3(nblk(cnt)), dim3(kBlock), 0, s, cnt, AA.elems, c->elem_lt,
+                                       c->A.scatter, c->A.scat_start, c->ctl.efix, (float*)c->A.vals);
+                else
+                    hipLaunchKernelGGL(k_ties<double>, dim3(nblk(cnt)), dim3(kBlock), 0, s, cnt, AA.elems, c->elem_lt,
+                                       c->A.scatter, c->A.scat_start, c->ctl.efix, (double*)c->A.vals);
+            }
+        }
+    }
+}
+
 // ELVALS: point kernel, then zero fc/matrix and the colour-ordered assembly.
 static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
     if (inc < 1 || inc > c->ninc1) return CPFEM_E_ARG;
@@ -1950,71 +2027,7 @@ static int elvals_impl(Ctx* c, int inc, double* sumcpiter, int32_t* tflag) {
     c->stats.pts_wide += counts[1];
     for (int q = 0; q < 4; ++q) c->stats.t_const_pass_ms[q] += c->sched.pass_ms[q];
     CPFEM_HIP(hipEventRecord(c->ev[1], s));
-    // zero fc and the matrix values
-    const size_t D = 3 * (size_t)c->nnod, nv = c->A.nvals();
-    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, s, c->fc, D, 0.0);
-    if (c->precision == 1) {
-        hipLaunchKernelGGL(k_fill<float>, dim3(std::min(nblk(nv), 65536)), dim3(kBlock), 0, s, (float*)c->A.vals, nv,
-                           0.0f);
-    } else {
-        hipLaunchKernelGGL(k_fill<double>, dim3(std::min(nblk(nv), 65536)), dim3(kBlock), 0, s, (double*)c->A.vals, nv,
-                           0.0);
-    }
-    AsmArgs AA;
-    AA.elem_nodes = c->elem_nodes;
-    AA.elem_gps = c->elem_gps;
-    AA.elem_lt = c->elem_lt;
-    AA.elem_np = c->elem_np;
-    AA.scatter = c->A.scatter;
-    AA.scat_start = c->A.scat_start;
-    AA.x = c->x;
-    AA.chi = c->chi;
-    AA.t = c->t;
-    AA.ngps = c->ngps;
-    AA.fc = c->fc;
-    const bool f32 = c->precision == 1;
-    const int ncol_el = ((int)c->ekstart.size() - 1) / kKinds;
-    for (int k = 0; k < ncol_el; ++k) {
-        for (int q = 0; q < kKinds; ++q) {
-            const int b0 = c->ekstart[kKinds * k + q], cnt = c->ekstart[kKinds * k + q + 1] - b0;
-            if (cnt == 0) continue;
-            AA.nel_color = cnt;
-            AA.elems = c->elem_by_color + b0;
-            if (q == kElem8) {
-                const int epb = c->asm_epb;
-                const int nb = (cnt + epb - 1) / epb;
-                if (f32)
-                    hipLaunchKernelGGL(k_assemble<float>, dim3(nb), dim3(64 * epb), 0, s, AA, (float*)c->A.vals);
-                else
-                    hipLaunchKernelGGL(k_assemble<double>, dim3(nb), dim3(64 * epb), 0, s, AA, (double*)c->A.vals);
-            } else if (q == kElem20) {
-                if (f32)
-                    hipLaunchKernelGGL((k_assemble_tab<float, 20>), dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
-                else
-                    hipLaunchKernelGGL((k_assemble_tab<double, 20>), dim3(cnt), dim3(256), 0, s, AA,
-                                       (double*)c->A.vals);
-            } else if (q == kElem10) {
-                if (f32)
-                    hipLaunchKernelGGL((k_assemble_tab<float, 10>), dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
-                else
-                    hipLaunchKernelGGL((k_assemble_tab<double, 10>), dim3(cnt), dim3(256), 0, s, AA,
-                                       (double*)c->A.vals);
-            } else if (q == kElem15) {
-                if (f32)
-                    hipLaunchKernelGGL((k_assemble_tab<float, 15>), dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
-                else
-                    hipLaunchKernelGGL((k_assemble_tab<double, 15>), dim3(cnt), dim3(256), 0, s, AA,
-                                       (double*)c->A.vals);
-            } else {
-                if (f32)
-                    hipLaunchKernelGGL(k_ties<float>, dim3(nblk(cnt)), dim3(kBlock), 0, s, cnt, AA.elems, c->elem_lt,
-                                       c->A.scatter, c->A.scat_start, c->ctl.efix, (float*)c->A.vals);
-                else
-                    hipLaunchKernelGGL(k_ties<double>, dim3(nblk(cnt)), dim3(kBlock), 0, s, cnt, AA.elems, c->elem_lt,
-                                       c->A.scatter, c->A.scat_start, c->ctl.efix, (double*)c->A.vals);
-            }
-        }
-    }
+    enqueue_assembly(c);
     CPFEM_HIP(hipGetLastError());
     CPFEM_HIP(hipEventRecord(c->ev[2], s));
     // reductions for the report
@@ -2094,71 +2107,7 @@ static int elvals_enqueue_dev(Ctx* c, NrDev* nd) {
     A.dt_dev = &nd->dt;
     const DevControl C = make_control(c->ctl);
     CPFEM_HIP(run_constitutive_dev(c->knss, c->dmat, C, 0.0, A, c->sched, s, nd->report));
-    // zero fc and the matrix values
-    const size_t D = 3 * (size_t)c->nnod, nv = c->A.nvals();
-    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, s, c->fc, D, 0.0);
-    if (c->precision == 1) {
-        hipLaunchKernelGGL(k_fill<float>, dim3(std::min(nblk(nv), 65536)), dim3(kBlock), 0, s, (float*)c->A.vals, nv,
-                           0.0f);
-    } else {
-        hipLaunchKernelGGL(k_fill<double>, dim3(std::min(nblk(nv), 65536)), dim3(kBlock), 0, s, (double*)c->A.vals, nv,
-                           0.0);
-    }
-    AsmArgs AA;
-    AA.elem_nodes = c->elem_nodes;
-    AA.elem_gps = c->elem_gps;
-    AA.elem_lt = c->elem_lt;
-    AA.elem_np = c->elem_np;
-    AA.scatter = c->A.scatter;
-    AA.scat_start = c->A.scat_start;
-    AA.x = c->x;
-    AA.chi = c->chi;
-    AA.t = c->t;
-    AA.ngps = c->ngps;
-    AA.fc = c->fc;
-    const bool f32 = c->precision == 1;
-    const int ncol_el = ((int)c->ekstart.size() - 1) / kKinds;
-    for (int k = 0; k < ncol_el; ++k) {
-        for (int q = 0; q < kKinds; ++q) {
-            const int b0 = c->ekstart[kKinds * k + q], cnt = c->ekstart[kKinds * k + q + 1] - b0;
-            if (cnt == 0) continue;
-            AA.nel_color = cnt;
-            AA.elems = c->elem_by_color + b0;
-            if (q == kElem8) {
-                const int epb = c->asm_epb;
-                const int nb = (cnt + epb - 1) / epb;
-                if (f32)
-                    hipLaunchKernelGGL(k_assemble<float>, dim3(nb), dim3(64 * epb), 0, s, AA, (float*)c->A.vals);
-                else
-                    hipLaunchKernelGGL(k_assemble<double>, dim3(nb), dim3(64 * epb), 0, s, AA, (double*)c->A.vals);
-            } else if (q == kElem20) {
-                if (f32)
-                    hipLaunchKernelGGL((k_assemble_tab<float, 20>), dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
-                else
-                    hipLaunchKernelGGL((k_assemble_tab<double, 20>), dim3(cnt), dim3(256), 0, s, AA,
-                                       (double*)c->A.vals);
-            } else if (q == kElem10) {
-                if (f32)
-                    hipLaunchKernelGGL((k_assemble_tab<float, 10>), dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
-                else
-                    hipLaunchKernelGGL((k_assemble_tab<double, 10>), dim3(cnt), dim3(256), 0, s, AA,
-                                       (double*)c->A.vals);
-            } else if (q == kElem15) {
-                if (f32)
-                    hipLaunchKernelGGL((k_assemble_tab<float, 15>), dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
-                else
-                    hipLaunchKernelGGL((k_assemble_tab<double, 15>), dim3(cnt), dim3(256), 0, s, AA,
-                                       (double*)c->A.vals);
-            } else {
-                if (f32)
-                    hipLaunchKernelGGL(k_ties<float>, dim3(nblk(cnt)), dim3(kBlock), 0, s, cnt, AA.elems, c->elem_lt,
-                                       c->A.scatter, c->A.scat_start, c->ctl.efix, (float*)c->A.vals);
-                else
-                    hipLaunchKernelGGL(k_ties<double>, dim3(nblk(cnt)), dim3(kBlock), 0, s, cnt, AA.elems, c->elem_lt,
-                                       c->A.scatter, c->A.scat_start, c->ctl.efix, (double*)c->A.vals);
-            }
-        }
-    }
+    enqueue_assembly(c);
     CPFEM_HIP(hipGetLastError());
     // reductions for the report, into nd
     const int nb = std::min(nblk(c->ngps), c->red.nblocks);
