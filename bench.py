@@ -34,6 +34,7 @@ the committed counter summary (profiles/pmc_traffic.json).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 64] [--budget S]
     python bench.py --sweep-probe 64     # fixed-length SOR run for profiler passes
+    python bench.py --const-probe 262144 # constitutive kernels alone for counter passes
 """
 import argparse
 import json
@@ -335,14 +336,49 @@ def sweep_probe(n, sweeps, copy_mib):
         print(f"precision {prec}: {nsw} sweeps, {st['sweep_launches']} colour launches, {per:.2f} us per launch "
               f"(events), facsls wall {1e3 * wall:.1f} ms, residual {err:.3e}", flush=True)
         sv.close()
-    import torch
-    nbytes = copy_mib << 20
-    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    b.copy_(a)
-    torch.cuda.synchronize()
+    device_copy(copy_mib << 20)
+
+
+def device_copy(nbytes):
+    """A device-to-device copy of nbytes (the runtime's copy kernel), whose
+    FETCH_SIZE / WRITE_SIZE readings calibrate the counters.  Called through
+    the HIP runtime directly: no torch in a profiled process (torch's own
+    kernel launches end the process with SIGSEGV under rocprofv3 here)."""
+    import ctypes as C
+    hip = C.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+    a, b = C.c_void_p(), C.c_void_p()
+    for p_ in (a, b):
+        if hip.hipMalloc(C.byref(p_), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, C.c_size_t(nbytes)) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemset failed")
+        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemcpy failed")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
     print(f"calibration copy: {nbytes} bytes read and {nbytes} bytes written", flush=True)
+
+
+def const_probe(n):
+    """Constitutive kernels alone, for counter passes: cpfem_dstrss_batch (the
+    multi-pass schedule over the 6-, 8- and 12-wide tiers) on n seeded FCC
+    points near yield (random orientations, start stresses of 40 MPa, strain
+    increments of 1e-3, 5 % elastic), twice (the first is a warm-up)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from gpbatch import make_gp_batch
+    from cpfem_code_amd import capi
+    from cpfem_code_amd.problem import Control, fcc_aluminium
+    mat = fcc_aluminium()
+    b = make_gp_batch(mat, n, seed=101, elastic_frac=0.05, stress=40.0)
+    print(f"constitutive probe: {n} FCC points", flush=True)
+    for rep in range(2):
+        t0 = time.perf_counter()
+        g = capi.dstrss_batch(mat, Control(), 1e-3, **b)
+        wall = time.perf_counter() - t0
+        print(f"call {rep}: {wall * 1e3:.1f} ms (host staging included), algorithmic FLOPs "
+              f"{int(g['flops'].sum())}, plastic points {int((g['deff'] > 0).sum())}", flush=True)
 
 
 def graph_mode_leg(capi, n, ninc):
@@ -439,6 +475,8 @@ def main():
     ap.add_argument("--sweep-probe", type=int, default=0,
                     help="profiling mode: run exactly this many SOR sweeps per path on the first system and exit")
     ap.add_argument("--copy-mib", type=int, default=512)
+    ap.add_argument("--const-probe", type=int, default=0,
+                    help="profiling mode: the constitutive batch kernels alone on this many seeded points, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this library runs on one GPU: the z-slab sharding over several GPUs is a "
@@ -449,6 +487,9 @@ def main():
     from cpfem_code_amd.problem import Control, fcc_plane_strain, fcc_tension, hcp_compression
     if args.sweep_probe:
         sweep_probe(args.n or 64, args.sweep_probe, args.copy_mib)
+        return
+    if args.const_probe:
+        const_probe(args.const_probe)
         return
     name, cus = capi.device_info()
     import dataclasses
