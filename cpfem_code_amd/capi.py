@@ -235,6 +235,7 @@ def _bind_mesh_api(L):
     L.cpfem_get_stats.argtypes = [_CTX, C.POINTER(CpfemStats)]
     L.cpfem_autotune.argtypes = [_CTX, C.c_int32, _P, _P]
     L.cpfem_autotune_points.argtypes = [_CTX, C.c_int32, _P, _P]
+    L.cpfem_sweep_order.argtypes = [_CTX, _P, C.c_int32, C.c_int32]
     L.cpfem_calib_begin.argtypes = [_CTX, C.c_int32, C.c_int32, C.c_double, C.c_int32]
     L.cpfem_calib_record.argtypes = [_CTX, C.c_int32]
     L.cpfem_calib_history.argtypes = [_CTX, C.c_int32, _P]
@@ -260,6 +261,7 @@ CALIB_EQUILIBRIUM = 1
 # cpfem_autotune flags
 TUNE_LAUNCH = 1
 TUNE_OMEGA = 2
+TUNE_ORDER = 4
 
 
 # Codes after which the device Newton loop goes on (the caller decides, as the
@@ -402,14 +404,26 @@ class Solver:
         code = _check(self.L.cpfem_facsls(self.h, _ptr(sw), _ptr(err)), "cpfem_facsls", ok_codes=(904,))
         return code, int(sw[0]), float(err[0])
 
-    def autotune(self, launch=True, omega=True):
+    def autotune(self, launch=True, omega=True, order=False):
         """Tune the sweeps on the system of the last suvals: launch shape
-        (waves per workgroup) and/or relaxation factor.  Returns (omega, waves)."""
+        (waves per workgroup), relaxation factor and/or the order of the node
+        colours within a sweep.  Returns (omega, waves)."""
         om = np.zeros(1)
         wpb = np.zeros(1, np.int32)
-        flags = (TUNE_LAUNCH if launch else 0) | (TUNE_OMEGA if omega else 0)
+        flags = (TUNE_LAUNCH if launch else 0) | (TUNE_OMEGA if omega else 0) | (TUNE_ORDER if order else 0)
         _check(self.L.cpfem_autotune(self.h, flags, _ptr(om), _ptr(wpb)), "cpfem_autotune")
         return float(om[0]), int(wpb[0])
+
+    def sweep_order(self, order=None):
+        """The order of the node colours within a sweep; with order given, apply it."""
+        buf = np.zeros(64, np.int32)
+        if order is not None:
+            buf = np.ascontiguousarray(order, np.int32)
+            nc = self.L.cpfem_sweep_order(self.h, _ptr(buf), len(buf), 1)
+        else:
+            nc = self.L.cpfem_sweep_order(self.h, _ptr(buf), len(buf), 0)
+        _check(min(nc, 0), "cpfem_sweep_order")
+        return buf[:nc].tolist()
 
     def autotune_points(self, inc=0, first_iters=0, elems_per_block=0):
         """Constitutive first-pass budget and assembly elements per workgroup:

@@ -848,6 +848,35 @@ __device__ __forceinline__ void group_row_sums(const T* __restrict__ vg, const i
     }
 }
 
+// The same for PC > 0 with the group's column indices already loaded (cpre):
+// the colour kernel issues them before it reads its stop flag.
+template <typename T, typename V, int PC>
+__device__ __forceinline__ void group_row_sums_pre(const T* __restrict__ vg, const int (&cpre)[PC], int l,
+                                                   const V* __restrict__ y, T sum[3], T d[9], int& nd) {
+    T m[PC][9];
+    T yv[PC][3];
+#pragma unroll
+    for (int p = 0; p < PC; ++p)
+#pragma unroll
+        for (int e = 0; e < 9; ++e) m[p][e] = __builtin_nontemporal_load(vg + (p * 9 + e) * 64);
+#pragma unroll
+    for (int p = 0; p < PC; ++p)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) yv[p][a] = (T)y[3 * cpre[p] + a];
+#pragma unroll
+    for (int p = 0; p < PC; ++p) {
+        if (p == 0 && l == 0) {
+#pragma unroll
+            for (int e = 0; e < 9; ++e) d[e] = m[0][e];
+            nd = cpre[0];
+            continue;
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            sum[a] = sum[a] + m[p][3 * a + 0] * yv[p][0] + m[p][3 * a + 1] * yv[p][1] + m[p][3 * a + 2] * yv[p][2];
+    }
+}
+
 // Sum of the kRowLanes partial sums of row i onto its lane l == 0 (lane i).
 template <typename T>
 __device__ __forceinline__ void row_combine(T sum[3], int lane) {
@@ -872,19 +901,33 @@ template <typename T, typename V, bool ACC, int PC>
 __global__ __launch_bounds__(kBlock) void k_sor_color(const SorArgs A, const T* __restrict__ vals,
                                                       const V* __restrict__ b, V* __restrict__ y, double* partial,
                                                       int pslot0) {
-    if (*A.stop) return;   // uniform: the flag is written by earlier kernels only
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     const int lane = threadIdx.x & 63;
     const int w = xcd_block(A.per_xcd) * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
     const int l = lane / kGroupRows, i = lane - l * kGroupRows;
     const bool live = (l < kRowLanes) && (w * kGroupRows + i < A.nrows);
     const size_t g = (size_t)A.gbase + w;
+    // With the slots per lane fixed at compile time the group's column indices
+    // are loaded before the stop flag is read, so that the two memory round
+    // trips of a launch's critical path (flag, then columns, then the gather of
+    // y) overlap; a skipped launch reads only the columns.
+    int cpre[PC > 0 ? PC : 1];
+    if (PC > 0 && live) {
+        const int* cg = A.cols + g * (size_t)A.P * 64 + lane;
+#pragma unroll
+        for (int p = 0; p < (PC > 0 ? PC : 1); ++p) cpre[p] = __builtin_nontemporal_load(cg + p * 64);
+    }
+    if (*A.stop) return;   // uniform: the flag is written by earlier kernels only
     T sum[3] = {(T)0, (T)0, (T)0};
     T d[9];
     int nd = 0;
-    if (live)
-        group_row_sums<T, V, PC, false>(vals + g * (size_t)A.P * 576 + lane, A.cols + g * (size_t)A.P * 64 + lane,
-                                        A.P, l, y, sum, d, nd);
+    if (live) {
+        if constexpr (PC > 0)
+            group_row_sums_pre<T, V, PC>(vals + g * (size_t)A.P * 576 + lane, cpre, l, y, sum, d, nd);
+        else
+            group_row_sums<T, V, PC, false>(vals + g * (size_t)A.P * 576 + lane, A.cols + g * (size_t)A.P * 64 + lane,
+                                            A.P, l, y, sum, d, nd);
+    }
     row_combine<T>(sum, lane);
     if (live && l == 0) {
         T yn[3] = {(T)y[3 * nd], (T)y[3 * nd + 1], (T)y[3 * nd + 2]};
@@ -1457,6 +1500,14 @@ static int build_structure(Ctx* c, const cpfem_mesh* m, const std::vector<int>& 
     c->A.P = P;
     c->A.ncolor = ncol;
     c->A.ngroups = ngroups;
+    // colour order of the sweeps: the colour numbers, except with 8 colours
+    // the reflected Gray code with bit roles (1, 4, 2): 0 1 5 4 6 7 3 2.  On a
+    // brick mesh the greedy colours are node parity classes, and a sweep that
+    // visits them one axis flip at a time needs 20-30 % fewer sweeps than
+    // 0..7 (64^3 plane strain, 10^3-16^3 plane strain and tension, tuned omega)
+    c->corder.resize(ncol);
+    for (int k = 0; k < ncol; ++k) c->corder[k] = k;
+    if (ncol == 8) c->corder = {0, 1, 5, 4, 6, 7, 3, 2};
     c->A.nnzb = nnzb;
     const size_t vbytes = (c->precision == 1) ? sizeof(float) : sizeof(double);
     if (dalloc(c, &c->A.diag_off, nnod) || dalloc(c, &c->A.cols, cols.size()) ||
@@ -2204,7 +2255,8 @@ static void sor_sweep(Ctx* c, const T* vals, const V* b, V* y, double* partial, 
     S.stop = stop ? stop : c->flags + 2;
     S.omega_dev = omega_dev;
     int slot = 0;
-    for (int k = 0; k < c->A.ncolor; ++k) {
+    for (int q = 0; q < c->A.ncolor; ++q) {
+        const int k = c->corder[q];
         S.nrows = c->A.cstart[k + 1] - c->A.cstart[k];
         S.gbase = c->A.gstart[k];
         S.per_xcd = group_per_xcd(c->A.gstart[k + 1] - c->A.gstart[k], c->sor_wpb);
@@ -2688,6 +2740,28 @@ static double predicted_sweeps(double r1, double r2, int W, double tol) {
     return r2 <= tol ? W + log(tol / r1) / rate : 2.0 * W + log(tol / r2) / rate;
 }
 
+// Candidate colour orders of the order tuning: the order in force and, with
+// 8 colours, the six binary-reflected Gray codes of the colour numbers (the
+// bit flipped at step k is the lowest set bit of k, with the three bits given
+// the roles of a permutation).  The greedy colouring of a brick mesh numbered
+// x fastest gives colour px + 2 py + 4 pz (node parities), so these visit the
+// parity classes one axis flip at a time; measured on the 64^3 plane-strain
+// increments they need 20-30 % fewer sweeps than the colour-number order.
+static std::vector<std::vector<int>> order_candidates(const Ctx* c) {
+    std::vector<std::vector<int>> out{c->corder};
+    if (c->A.ncolor != 8) return out;
+    const int roles[6][3] = {{1, 2, 4}, {1, 4, 2}, {2, 1, 4}, {2, 4, 1}, {4, 1, 2}, {4, 2, 1}};
+    for (const auto& r : roles) {
+        std::vector<int> o(8);
+        for (int k = 0; k < 8; ++k) {
+            const int g = k ^ (k >> 1);   // bits b0 b1 b2 of the reflected code
+            o[k] = ((g & 1) ? r[0] : 0) | ((g & 2) ? r[1] : 0) | ((g & 4) ? r[2] : 0);
+        }
+        if (o != c->corder) out.push_back(o);
+    }
+    return out;
+}
+
 static int autotune_impl(Ctx* c, int flags) {
     if (!c->assembled) return CPFEM_E_ARG;
     round_sweep_matrix(c);
@@ -2750,6 +2824,36 @@ static int autotune_impl(Ctx* c, int flags) {
         c->ctl.sor_omega = best_om;
         drop_graphs(c);
     }
+    if (!rc && (flags & CPFEM_TUNE_ORDER)) {
+        // each candidate order: two windows from a zero start at the current
+        // omega; a candidate replaces the order in force only when it is
+        // predicted to need at least 3 % fewer sweeps
+        const int W = std::max(64, std::min(256, c->ctl.sor_maxit / 8)) / kSweepBatch * kSweepBatch;
+        const double tol = c->precision != 0 ? std::max(c->ctl.sor_tol, 1e-12) : std::max(c->ctl.sor_tol, 1e-14);
+        const auto cands = order_candidates(c);
+        std::vector<int> best = c->corder;
+        double best_n = 1e300;
+        for (size_t q = 0; q < cands.size() && !rc; ++q) {
+            c->corder = cands[q];
+            drop_graphs(c);
+            hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, s, c->ddx, (size_t)D, 0.0);
+            if (c->precision != 0)
+                (void)hipMemcpyAsync(c->xold, c->rhs, D * sizeof(double), hipMemcpyDeviceToDevice, s);
+            double r1 = 0.0, r2 = 0.0;
+            if (fail(tune_window(c, W, &r1))) break;
+            if (r1 > c->ctl.sor_tol && std::isfinite(r1))
+                if (fail(tune_window(c, W, &r2))) break;
+            c->stats.tune_sweeps += (r2 > 0.0 ? 2 : 1) * W;
+            const double n = predicted_sweeps(r1, r2, W, tol);
+            if (q == 0 || n < 0.97 * best_n) {
+                best_n = n;
+                best = cands[q];
+            }
+        }
+        c->corder = best;
+        drop_graphs(c);
+        drop_solve_graphs(c);
+    }
     (void)hipMemcpyAsync(c->ddx, saved, D * sizeof(double), hipMemcpyDeviceToDevice, s);
     const hipError_t e = hipStreamSynchronize(s);
     (void)hipFree(saved);
@@ -2761,7 +2865,7 @@ static int autotune_impl(Ctx* c, int flags) {
 }
 
 extern "C" int cpfem_autotune(cpfem_ctx* c, int32_t flags, double* omega, int32_t* waves_per_block) {
-    if (!c || (flags & ~(CPFEM_TUNE_LAUNCH | CPFEM_TUNE_OMEGA))) return CPFEM_E_ARG;
+    if (!c || (flags & ~(CPFEM_TUNE_LAUNCH | CPFEM_TUNE_OMEGA | CPFEM_TUNE_ORDER))) return CPFEM_E_ARG;
     if (flags == 0) {
         // apply the given settings
         const double om = omega ? *omega : 0.0;
@@ -2793,6 +2897,28 @@ extern "C" int cpfem_autotune(cpfem_ctx* c, int32_t flags, double* omega, int32_
     if (omega) *omega = c->ctl.sor_omega;
     if (waves_per_block) *waves_per_block = c->sor_wpb;
     return 0;
+}
+
+extern "C" int cpfem_sweep_order(cpfem_ctx* c, int32_t* order, int32_t n, int32_t set) {
+    if (!c || !order) return CPFEM_E_ARG;
+    const int nc = c->A.ncolor;
+    if (n < nc) return CPFEM_E_ARG;
+    if (!set) {
+        for (int k = 0; k < nc; ++k) order[k] = c->corder[k];
+        return nc;
+    }
+    if (n != nc) return CPFEM_E_ARG;
+    std::vector<int> seen(nc, 0), o(nc);
+    for (int k = 0; k < nc; ++k) {
+        if (order[k] < 0 || order[k] >= nc || seen[order[k]]++) return CPFEM_E_ARG;
+        o[k] = order[k];
+    }
+    if (o != c->corder) {
+        c->corder = o;
+        drop_graphs(c);
+        drop_solve_graphs(c);
+    }
+    return nc;
 }
 
 // Launch shapes of the point kernels: the outer iterations every point gets in

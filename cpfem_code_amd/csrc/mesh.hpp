@@ -126,6 +126,7 @@ struct Ctx {
     // elements by colour and, within a colour, by kind (ElemKind):
     // kind q of colour k is elem_by_color[ekstart[kKinds*k+q] .. ekstart[kKinds*k+q+1])
     std::vector<int> ekstart;
+    std::vector<int> corder;     // order in which a sweep relaxes the node colours (a permutation of 0..ncolor-1)
     int npts8 = 0, npts20 = 0;   // points of 8-node and of 20-node elements
     int npts10 = 0, npts15 = 0;  // points of 10-node and of 15-node elements
     int* elem_by_color = nullptr;

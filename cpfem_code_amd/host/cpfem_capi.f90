@@ -216,6 +216,14 @@ MODULE CPFEM_CAPI
             INTEGER(C_INT32_T), INTENT(OUT) :: waves_per_block
         END FUNCTION
 
+        ! order of the node colours within a sweep (set /= 0 applies order(1:n))
+        INTEGER(C_INT) FUNCTION cpfem_sweep_order(ctx, order, n, set) BIND(C, name='cpfem_sweep_order')
+            IMPORT :: C_INT, C_INT32_T, C_PTR
+            TYPE(C_PTR), VALUE :: ctx
+            INTEGER(C_INT32_T), INTENT(INOUT) :: order(*)
+            INTEGER(C_INT32_T), VALUE :: n, set
+        END FUNCTION
+
         ! calibration: record of the load path, replay loss and its adjoint gradient
         INTEGER(C_INT) FUNCTION cpfem_calib_begin(ctx, ninc, ckpt_every, toler, maxit) &
                 BIND(C, name='cpfem_calib_begin')

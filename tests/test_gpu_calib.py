@@ -40,7 +40,9 @@ def path_excluded():
     prob = fcc_tension(4, ninc=12, seed=4)
     print("calib: recording 4^3 FCC tension, 12 increments (non-converging points excluded)", flush=True)
     p = calibrate.record_path(prob, ninc=12, comp=2, ckpt_every=5)
-    nx = calibrate.exclude_nonconverging(p, calibrate.material_params(prob.materials[0]), rel=1e-6)
+    # the stencils of both step sizes the tests difference with (exclusions accumulate)
+    calibrate.exclude_nonconverging(p, calibrate.material_params(prob.materials[0]), rel=1e-6)
+    nx = calibrate.exclude_nonconverging(p, calibrate.material_params(prob.materials[0]), rel=1e-5)
     print(f"calib: {nx} of {prob.ngps} points excluded", flush=True)
     assert nx < prob.ngps // 2
     yield prob, p
