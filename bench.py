@@ -212,7 +212,7 @@ def leg_summary(recs, wall, warm, first_inc):
     }
 
 
-def roofline_of(prob, recs, traffic, key_sor, s_a):
+def roofline_of(prob, recs, traffic, key_sor, s_a, asm_t="double"):
     """Roofline objects of the constitutive phase and the sweep kernel of a leg."""
     nnzb = nnzb_structured(prob.dims)
     launches = sum(r["sweep_launches"] for r in recs)
@@ -234,9 +234,13 @@ def roofline_of(prob, recs, traffic, key_sor, s_a):
                        "one 'launch' = the constitutive phase of one ELVALS, event-timed on the library stream)"}
     asm_flops = 75968 * prob.nlmnt * nit
     t_a = sum(r["t_asm_ms"] for r in recs) * 1e-3
+    asm_keys = (f"k_assemble<{asm_t}, true>", f"k_gather_blocks<{asm_t}>", "k_gather_forces")
+    asm_traffic = sum(traffic[k] for k in asm_keys) if traffic and all(k in traffic for k in asm_keys) else None
     asm = {"bound": "fp64-valu", "achieved": asm_flops / t_a / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-           "frac": asm_flops / t_a / FP64_PEAK, "traffic": traffic.get("k_assemble<double>") if traffic else None,
-           "kernel": "k_assemble<double> (element assembly of one Newton iteration)"}
+           "frac": asm_flops / t_a / FP64_PEAK, "traffic": asm_traffic,
+           "launch_us": t_a / nit * 1e6, "launches": nit,
+           "kernel": f"{' + '.join(asm_keys)} (element blocks of every element, then the gathers of the matrix "
+                     "blocks and node forces: the assembly of one Newton iteration; traffic summed over the three)"}
     return {"sor": sor, "constitutive": const, "assembly": asm,
             "share_s": {"sor": sum(r["t_sor_ms"] for r in recs) * 1e-3, "constitutive": t_c, "assembly": t_a}}
 
@@ -539,7 +543,8 @@ def main():
     cpu = cpu_baseline(capi, args.cpu_seconds) if args.cpu_seconds > 0 else None
     h = legs["fp64"]
     roof = roofline_of(prob, recs64, traffic, "k_sor_color<double>", 8)
-    roof_m = roofline_of(prob, recsm, traffic, "k_sor_color<float>", 4)
+    roof_m = roofline_of(prob, recsm, traffic, "k_sor_color<float>", 4,
+                         "float" if args.mixed_precision == 1 else "double")
     dom = max(("sor", "constitutive", "assembly"), key=lambda k: roof["share_s"][k])
     line = {
         "metric": METRIC,

@@ -118,7 +118,7 @@ __global__ void k_point_sums(const double* __restrict__ erepmd, const long long*
 struct AsmArgs {
     int nel_color;
     int per_xcd;                 // logical blocks per XCD of the 8-node launch
-    const int* elems;            // elements of this colour
+    const int* elems;            // elements of this colour (element-block mode: element k is wave k)
     const int* elem_nodes;
     const int* elem_gps;
     const int* elem_lt;
@@ -130,9 +130,17 @@ struct AsmArgs {
     const double* t;             // [6][ngps]
     int ngps;
     double* fc;                  // [3*nnod]
+    double* efc;                 // element-block mode: [nel][8][3] forces of each element node
 };
 
-template <typename T>
+// BLOCKS = false: the element's blocks are added to the matrix through the
+// scatter map and its node forces to fc (one launch per element colour).
+// BLOCKS = true: one launch over all elements; the block of node pair
+// (a, b) of element e goes to vals[e*576 + a*72 + q*8 + b] (q: entry; the
+// 72 values of node a's pairs are contiguous, read by node a's matrix row
+// only) and the forces of node a to efc[(e*8 + a)*3 + c], with plain
+// stores; the gathers below then sum them into the matrix and fc.
+template <typename T, bool BLOCKS>
 __global__ __launch_bounds__(256) void k_assemble(const AsmArgs A, T* __restrict__ vals) {
     constexpr int EPB = 4;  // elements per block at most (blockDim.x / 64 of the launch: 1, 2 or 4)
     __shared__ T sh_dn[EPB][8][8][3];
@@ -140,13 +148,17 @@ __global__ __launch_bounds__(256) void k_assemble(const AsmArgs A, T* __restrict
     __shared__ T sh_d[EPB][8][36];
     __shared__ T sh_s[EPB][8][6];
     __shared__ double sh_x[EPB][8][3];
+    // internal forces: weight of each point and its stress times the node's
+    // shape-function gradient, sum_k s(i,k) dN_a/dx_k, for every node a
+    __shared__ double sh_wf[EPB][8];
+    __shared__ double sh_sf[EPB][8][8][3];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     // consecutive elements of the colour (neighbours in the mesh, whose
     // blocks share cache lines of the matrix) on one XCD, so that the partial
     // lines they write meet in that XCD's L2 before they are written back
     const int k = xcd_block(A.per_xcd) * (int)(blockDim.x >> 6) + wave;
     const bool live = k < A.nel_color;
-    const int e = live ? A.elems[k] : 0;
+    const int e = live ? (BLOCKS ? k : A.elems[k]) : 0;
     const int ng = live ? A.elem_np[e] : 0;
     const int lt = live ? A.elem_lt[e] : 13;
     if (live && lane < 24) {
@@ -170,8 +182,19 @@ __global__ __launch_bounds__(256) void k_assemble(const AsmArgs A, T* __restrict
         const int gp = A.elem_gps[8 * e + lane];
 #pragma unroll
         for (int q = 0; q < 36; ++q) sh_d[wave][lane][q] = (T)A.chi[q * n + gp];
+        double st[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) sh_s[wave][lane][q] = (T)A.t[q * n + gp];
+        for (int q = 0; q < 6; ++q) {
+            st[q] = A.t[q * n + gp];
+            sh_s[wave][lane][q] = (T)st[q];
+        }
+        sh_wf[wave][lane] = w;
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            sh_sf[wave][lane][a][0] = st[0] * sfd[a][0] + st[5] * sfd[a][1] + st[4] * sfd[a][2];
+            sh_sf[wave][lane][a][1] = st[1] * sfd[a][1] + st[3] * sfd[a][2] + st[5] * sfd[a][0];
+            sh_sf[wave][lane][a][2] = st[2] * sfd[a][2] + st[4] * sfd[a][0] + st[3] * sfd[a][1];
+        }
     }
     __syncthreads();
     if (!live) return;
@@ -197,36 +220,113 @@ __global__ __launch_bounds__(256) void k_assemble(const AsmArgs A, T* __restrict
 #pragma unroll
             for (int q = 0; q < 3; ++q) acc[p][q] = acc[p][q] + w * gm[p][q];
     }
-    const int off = A.scatter[A.scat_start[e] + lane];
+    if (BLOCKS) {
+        T* dst = vals + (size_t)e * 576 + i * 72 + j;
 #pragma unroll
-    for (int p = 0; p < 3; ++p)
+        for (int p = 0; p < 3; ++p)
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            T* dst = vals + off + (3 * p + q) * 64;
-            *dst = *dst + acc[p][q];
-        }
-    // internal forces of node i (Elvals.f90:122-126), lanes with j == 0
+            for (int q = 0; q < 3; ++q) dst[(3 * p + q) * 8] = acc[p][q];
+    } else {
+        const int off = A.scatter[A.scat_start[e] + lane];
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                T* dst = vals + off + (3 * p + q) * 64;
+                *dst = *dst + acc[p][q];
+            }
+    }
+    // internal forces of node i (Elvals.f90:122-126), lanes with j == 0, from
+    // the points' staged products, summed over the points in order
     if (j == 0) {
         double fl[3] = {0.0, 0.0, 0.0};
         for (int g = 0; g < ng; ++g) {
-            const int gp = A.elem_gps[8 * e + g];
-            double xl[8][3], sfd[8][3];
+            const double w = sh_wf[wave][g];
 #pragma unroll
-            for (int a = 0; a < 8; ++a)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) xl[a][c] = sh_x[wave][a][c];
-            const double w = brick_shape(lt, g, xl, sfd);
-            double s[6];
-#pragma unroll
-            for (int q = 0; q < 6; ++q) s[q] = A.t[q * n + gp];
-            fl[0] = fl[0] + w * (s[0] * sfd[i][0] + s[5] * sfd[i][1] + s[4] * sfd[i][2]);
-            fl[1] = fl[1] + w * (s[1] * sfd[i][1] + s[3] * sfd[i][2] + s[5] * sfd[i][0]);
-            fl[2] = fl[2] + w * (s[2] * sfd[i][2] + s[4] * sfd[i][0] + s[3] * sfd[i][1]);
+            for (int c = 0; c < 3; ++c) fl[c] = fl[c] + w * sh_sf[wave][g][i][c];
         }
-        const int nd = A.elem_nodes[kMaxNodes * e + i];
+        if (BLOCKS) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) A.fc[3 * nd + c] = A.fc[3 * nd + c] + fl[c];
+            for (int c = 0; c < 3; ++c) A.efc[((size_t)e * 8 + i) * 3 + c] = fl[c];
+        } else {
+            const int nd = A.elem_nodes[kMaxNodes * e + i];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) A.fc[3 * nd + c] = A.fc[3 * nd + c] + fl[c];
+        }
     }
+}
+
+// Gather assembly: the lists of the contributions of each matrix block and
+// each node force, built at context creation.  A segment is one lane slot p
+// of one row group (64 lanes, 9 entries each: the layout of the matrix
+// values); lane l of it finds its contributions at list[start + k*64 + l],
+// k < cnt (the largest count of the segment's lanes; -1 pads), in the order
+// of the element colours.
+struct GatherArgs {
+    int nseg;            // segments = waves of the launch
+    int per_xcd;         // logical blocks per XCD
+    const int* order;    // [nseg] segment of each wave: the colours' groups of one mesh region in turn
+    const int* start;    // [nseg]
+    const int* cnt;      // [nseg]
+    const int* list;     // element-block indices e*576 + a*72 + b
+};
+
+// Matrix values from the element blocks.  Each block starts from zero and adds
+// its contributions in element-colour order, the order in which the
+// colour-by-colour scatter adds them, so both give the same values bit for bit.
+// The lanes of a row read the node pairs (a, b) of one element node a: 576
+// contiguous bytes per element.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_blocks(const GatherArgs G, const T* __restrict__ eblk,
+                                                       T* __restrict__ vals) {
+    const int lane = threadIdx.x & 63;
+    const int w = xcd_block(G.per_xcd) * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= G.nseg) return;   // wave-uniform
+    const int sg = G.order[w];
+    const int* li = G.list + G.start[sg] + lane;
+    const int n = G.cnt[sg];
+    T acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q] = (T)0;
+    for (int k0 = 0; k0 < n; k0 += 4) {
+        int id[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) id[u] = (k0 + u < n) ? li[(k0 + u) * 64] : -1;
+        T v[4][9];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < 9; ++q) v[u][q] = id[u] >= 0 ? eblk[(size_t)id[u] + q * 8] : (T)0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (id[u] >= 0)
+#pragma unroll
+                for (int q = 0; q < 9; ++q) acc[q] = acc[q] + v[u][q];
+    }
+    T* dst = vals + (size_t)sg * 576 + lane;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) dst[q * 64] = acc[q];
+}
+
+// Node forces from the element node forces, in element-colour order; nodes
+// in chunks of 64 (one wave each), entry k of node nd at
+// list[start[nd/64] + k*64 + nd%64] (efc index, -1 pads).
+__global__ void k_gather_forces(int nnod, const int* __restrict__ start, const int* __restrict__ cnt,
+                                const int* __restrict__ list, const double* __restrict__ efc, double* __restrict__ fc) {
+    const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nd >= nnod) return;
+    const int ch = nd >> 6, lane = nd & 63;
+    const int n = cnt[ch];
+    const int* li = list + start[ch] + lane;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < n; ++k) {
+        const int id = li[k * 64];
+        if (id >= 0)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[c] = acc[c] + efc[(size_t)id + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) fc[3 * (size_t)nd + c] = acc[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -1354,6 +1454,8 @@ __global__ void k_reorient(int n, double* sv, int* twinsys, int* timestwinned, c
 // host side
 // ---------------------------------------------------------------------------
 static inline int nblk(size_t n, int b = kBlock) { return (int)((n + b - 1) / b); }
+// Blocks per XCD of a launch over ng row groups, wpb groups (waves) per block.
+static int group_per_xcd(int ng, int wpb = kWavesPerBlock) { return ((ng + wpb - 1) / wpb + kXcd - 1) / kXcd; }
 
 template <typename T>
 static hipError_t dalloc(Ctx* c, T** p, size_t n) {
@@ -1379,6 +1481,105 @@ static float elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, a, b);
     return ms;
+}
+
+// Gather assembly on meshes of 8-node bricks (CPFEM_ASSEMBLY=scatter keeps
+// the colour-by-colour scatter): for every matrix block and every node, the
+// list of its element contributions in the order of the element colours (ebc
+// holds the elements colour by colour), padded per segment of 64 lanes.
+static int build_gather(Ctx* c, int nel, int nnod, const std::vector<int>& en, const std::vector<int>& kind,
+                        const std::vector<int>& scat, const std::vector<int>& scat_start, const std::vector<int>& ebc) {
+    c->A.gather = false;
+    const char* am = getenv("CPFEM_ASSEMBLY");
+    if (am && strcmp(am, "scatter") == 0) return 0;
+    if (nel == 0 || (size_t)nel * 576 > 0x7fffffffull) return 0;
+    for (int e = 0; e < nel; ++e)
+        if (kind[e] != kElem8) return 0;
+    const int nseg = c->A.ngroups * c->A.P;
+    // matrix blocks: block entry 0 at offset off = segment*576 + lane
+    std::vector<int> cnt((size_t)nseg * 64, 0);
+    for (int e = 0; e < nel; ++e)
+        for (int pr = 0; pr < 64; ++pr) {
+            const int off = scat[scat_start[e] + pr];
+            cnt[(size_t)(off / 576) * 64 + off % 64]++;
+        }
+    std::vector<int> seg_cnt(nseg, 0), seg_start(nseg, 0);
+    size_t total = 0;
+    for (int sg = 0; sg < nseg; ++sg) {
+        int m = 0;
+        for (int l = 0; l < 64; ++l) m = std::max(m, cnt[(size_t)sg * 64 + l]);
+        seg_cnt[sg] = m;
+        if (total > 0x7fffffffull) return 0;
+        seg_start[sg] = (int)total;
+        total += (size_t)m * 64;
+    }
+    if (total > 0x7fffffffull) return 0;
+    std::vector<int> glist(std::max<size_t>(total, 1), -1);
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int t = 0; t < nel; ++t) {
+        const int e = ebc[t];
+        for (int pr = 0; pr < 64; ++pr) {
+            const int off = scat[scat_start[e] + pr];
+            const int sg = off / 576, l = off % 64;
+            const int k = cnt[(size_t)sg * 64 + l]++;
+            glist[(size_t)seg_start[sg] + (size_t)k * 64 + l] = e * 576 + (pr >> 3) * 72 + (pr & 7);
+        }
+    }
+    // wave order of the block gather: group j of every colour in turn (the
+    // colours' j-th groups cover neighbouring nodes, which share elements)
+    std::vector<int> order;
+    order.reserve(nseg);
+    const int ncol = c->A.ncolor, P = c->A.P;
+    int maxg = 0;
+    for (int k = 0; k < ncol; ++k) maxg = std::max(maxg, c->A.gstart[k + 1] - c->A.gstart[k]);
+    for (int j = 0; j < maxg; ++j)
+        for (int k = 0; k < ncol; ++k)
+            if (j < c->A.gstart[k + 1] - c->A.gstart[k])
+                for (int p = 0; p < P; ++p) order.push_back((c->A.gstart[k] + j) * P + p);
+    if ((int)order.size() != nseg) return CPFEM_E_ARG;
+    // node forces: chunks of 64 nodes
+    const int nch = (nnod + 63) / 64;
+    std::vector<int> fcnt(nnod, 0), fch_cnt(nch, 0), fch_start(nch, 0);
+    for (int e = 0; e < nel; ++e)
+        for (int a = 0; a < 8; ++a) fcnt[en[kMaxNodes * e + a]]++;
+    size_t ftotal = 0;
+    for (int ch = 0; ch < nch; ++ch) {
+        int m = 0;
+        for (int nd = ch * 64; nd < std::min(nnod, ch * 64 + 64); ++nd) m = std::max(m, fcnt[nd]);
+        fch_cnt[ch] = m;
+        fch_start[ch] = (int)ftotal;
+        ftotal += (size_t)m * 64;
+    }
+    if (ftotal > 0x7fffffffull) return 0;
+    std::vector<int> flist(std::max<size_t>(ftotal, 1), -1);
+    std::fill(fcnt.begin(), fcnt.end(), 0);
+    for (int t = 0; t < nel; ++t) {
+        const int e = ebc[t];
+        for (int a = 0; a < 8; ++a) {
+            const int nd = en[kMaxNodes * e + a];
+            const int k = fcnt[nd]++;
+            flist[(size_t)fch_start[nd / 64] + (size_t)k * 64 + nd % 64] = (e * 8 + a) * 3;
+        }
+    }
+    const size_t vbytes = (c->precision == 1) ? sizeof(float) : sizeof(double);
+    char* eb = nullptr;
+    if (dalloc(c, &eb, (size_t)nel * 576 * vbytes) || dalloc(c, &c->A.efc, (size_t)nel * 24) ||
+        dalloc(c, &c->A.g_order, nseg) || dalloc(c, &c->A.g_start, nseg) || dalloc(c, &c->A.g_cnt, nseg) ||
+        dalloc(c, &c->A.g_list, glist.size()) || dalloc(c, &c->A.f_start, nch) || dalloc(c, &c->A.f_cnt, nch) ||
+        dalloc(c, &c->A.f_list, flist.size()))
+        return CPFEM_E_HIP;
+    c->A.eblk = eb;
+    hipStream_t s = c->stream;
+    CPFEM_HIP(hipMemcpyAsync(c->A.g_order, order.data(), nseg * sizeof(int), hipMemcpyHostToDevice, s));
+    CPFEM_HIP(hipMemcpyAsync(c->A.g_start, seg_start.data(), nseg * sizeof(int), hipMemcpyHostToDevice, s));
+    CPFEM_HIP(hipMemcpyAsync(c->A.g_cnt, seg_cnt.data(), nseg * sizeof(int), hipMemcpyHostToDevice, s));
+    CPFEM_HIP(hipMemcpyAsync(c->A.g_list, glist.data(), glist.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    CPFEM_HIP(hipMemcpyAsync(c->A.f_start, fch_start.data(), nch * sizeof(int), hipMemcpyHostToDevice, s));
+    CPFEM_HIP(hipMemcpyAsync(c->A.f_cnt, fch_cnt.data(), nch * sizeof(int), hipMemcpyHostToDevice, s));
+    CPFEM_HIP(hipMemcpyAsync(c->A.f_list, flist.data(), flist.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    CPFEM_HIP(hipStreamSynchronize(s));
+    c->A.gather = true;
+    return 0;
 }
 
 // Sparsity (the node blocks SETPTA's row lists cover, Stif_sol.f90:98-149),
@@ -1525,7 +1726,7 @@ static int build_structure(Ctx* c, const cpfem_mesh* m, const std::vector<int>& 
                              c->stream));
     CPFEM_HIP(hipMemcpyAsync(c->elem_by_color, ebc.data(), nel * sizeof(int), hipMemcpyHostToDevice, c->stream));
     CPFEM_HIP(hipStreamSynchronize(c->stream));
-    return 0;
+    return build_gather(c, nel, nnod, en, kind, scat, scat_start, ebc);
 }
 
 
@@ -1945,8 +2146,52 @@ extern "C" int cpfem_update_node(cpfem_ctx* c) {
 // colour (the elements of one colour share no node), on the context's stream.
 static void enqueue_assembly(Ctx* c) {
     hipStream_t s = c->stream;
-    // zero fc and the matrix values
     const size_t D = 3 * (size_t)c->nnod, nv = c->A.nvals();
+    if (c->A.gather) {
+        // element blocks of every element in one launch, then every matrix
+        // block and node force sums its contributions (all values written:
+        // no zero fill)
+        AsmArgs AA;
+        AA.nel_color = c->nlmnt;
+        AA.elems = nullptr;
+        AA.elem_nodes = c->elem_nodes;
+        AA.elem_gps = c->elem_gps;
+        AA.elem_lt = c->elem_lt;
+        AA.elem_np = c->elem_np;
+        AA.scatter = nullptr;
+        AA.scat_start = nullptr;
+        AA.x = c->x;
+        AA.chi = c->chi;
+        AA.t = c->t;
+        AA.ngps = c->ngps;
+        AA.fc = c->fc;
+        AA.efc = c->A.efc;
+        const int epb = c->asm_epb;
+        AA.per_xcd = ((c->nlmnt + epb - 1) / epb + kXcd - 1) / kXcd;
+        GatherArgs G;
+        G.nseg = c->A.ngroups * c->A.P;
+        G.per_xcd = group_per_xcd(G.nseg);
+        G.order = c->A.g_order;
+        G.start = c->A.g_start;
+        G.cnt = c->A.g_cnt;
+        G.list = c->A.g_list;
+        if (c->precision == 1) {
+            hipLaunchKernelGGL((k_assemble<float, true>), dim3(AA.per_xcd * kXcd), dim3(64 * epb), 0, s, AA,
+                               (float*)c->A.eblk);
+            hipLaunchKernelGGL(k_gather_blocks<float>, dim3(G.per_xcd * kXcd), dim3(kBlock), 0, s, G,
+                               (const float*)c->A.eblk, (float*)c->A.vals);
+        } else {
+            hipLaunchKernelGGL((k_assemble<double, true>), dim3(AA.per_xcd * kXcd), dim3(64 * epb), 0, s, AA,
+                               (double*)c->A.eblk);
+            hipLaunchKernelGGL(k_gather_blocks<double>, dim3(G.per_xcd * kXcd), dim3(kBlock), 0, s, G,
+                               (const double*)c->A.eblk, (double*)c->A.vals);
+        }
+        hipLaunchKernelGGL(k_gather_forces, dim3(nblk(c->nnod)), dim3(kBlock), 0, s, c->nnod,
+                           (const int*)c->A.f_start, (const int*)c->A.f_cnt, (const int*)c->A.f_list,
+                           (const double*)c->A.efc, c->fc);
+        return;
+    }
+    // zero fc and the matrix values
     hipLaunchKernelGGL(k_fill<double>, dim3(nblk(D)), dim3(kBlock), 0, s, c->fc, D, 0.0);
     if (c->precision == 1) {
         hipLaunchKernelGGL(k_fill<float>, dim3(std::min(nblk(nv), 65536)), dim3(kBlock), 0, s, (float*)c->A.vals, nv,
@@ -1957,6 +2202,7 @@ static void enqueue_assembly(Ctx* c) {
     }
     AsmArgs AA;
     AA.per_xcd = 0;
+    AA.efc = nullptr;
     AA.elem_nodes = c->elem_nodes;
     AA.elem_gps = c->elem_gps;
     AA.elem_lt = c->elem_lt;
@@ -1981,9 +2227,9 @@ static void enqueue_assembly(Ctx* c) {
                 AA.per_xcd = ((cnt + epb - 1) / epb + kXcd - 1) / kXcd;
                 const int nb = AA.per_xcd * kXcd;
                 if (f32)
-                    hipLaunchKernelGGL(k_assemble<float>, dim3(nb), dim3(64 * epb), 0, s, AA, (float*)c->A.vals);
+                    hipLaunchKernelGGL((k_assemble<float, false>), dim3(nb), dim3(64 * epb), 0, s, AA, (float*)c->A.vals);
                 else
-                    hipLaunchKernelGGL(k_assemble<double>, dim3(nb), dim3(64 * epb), 0, s, AA, (double*)c->A.vals);
+                    hipLaunchKernelGGL((k_assemble<double, false>), dim3(nb), dim3(64 * epb), 0, s, AA, (double*)c->A.vals);
             } else if (q == kElem20) {
                 if (f32)
                     hipLaunchKernelGGL((k_assemble_tab<float, 20>), dim3(cnt), dim3(256), 0, s, AA, (float*)c->A.vals);
@@ -2242,8 +2488,6 @@ static const int kSweepBatch = 16;
 // Most batches between two residual checks of the residual-based stopping rule.
 static const int kMaxCheckBatches = 64;
 
-// Blocks per XCD of a launch over ng row groups, wpb groups (waves) per block.
-static int group_per_xcd(int ng, int wpb = kWavesPerBlock) { return ((ng + wpb - 1) / wpb + kXcd - 1) / kXcd; }
 
 template <typename T, typename V, bool ACC>
 static void sor_sweep(Ctx* c, const T* vals, const V* b, V* y, double* partial, const int* stop = nullptr,

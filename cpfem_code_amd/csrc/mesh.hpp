@@ -73,6 +73,19 @@ struct Matrix {
     int* scatter = nullptr;    // per element nd*nd entries from scat_start[e]: index of entry 0 of the
                                // block of node pair (a, b) at a*nd + b
     int* scat_start = nullptr; // [nel]
+    // gather assembly (meshes of 8-node bricks): the element blocks are stored
+    // element by element, then every matrix block and node force sums its
+    // contributions from lists (in element-colour order)
+    bool gather = false;
+    void* eblk = nullptr;      // [nel][8][9][8] double or float: entry q of node pair (a, b) at a*72 + q*8 + b
+    double* efc = nullptr;     // [nel][8][3] forces of each element node
+    int* g_order = nullptr;    // [ngroups*P] segment of each wave of the block gather
+    int* g_start = nullptr;    // [ngroups*P] first list entry of each segment
+    int* g_cnt = nullptr;      // [ngroups*P] list entries per lane of each segment
+    int* g_list = nullptr;     // element-block indices, 64 lanes per entry
+    int* f_start = nullptr;    // [ceil(nnod/64)] the same for the node forces
+    int* f_cnt = nullptr;
+    int* f_list = nullptr;     // efc indices
     size_t nnzb = 0;           // true (unpadded) blocks
     size_t nvals() const { return (size_t)ngroups * P * 576; }
 };
