@@ -977,7 +977,55 @@ __device__ __forceinline__ void group_row_sums_pre(const T* __restrict__ vg, con
     }
 }
 
+// FP64 row sums with three slots per lane in two phases: slots 0 and 1, then
+// slot 2 once their products are summed (the compiler barrier keeps slot 2's
+// loads behind them), so that at most two slots' values are live per lane.
+// The products are summed in the same order as in group_row_sums_pre.
+template <typename V>
+__device__ __forceinline__ void group_row_sums_2ph(const double* __restrict__ vg, const int (&cpre)[3], int l,
+                                                   const V* __restrict__ y, double sum[3], double d[9], int& nd) {
+    {
+        double m[2][9];
+        double yv[2][3];
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int e = 0; e < 9; ++e) m[p][e] = __builtin_nontemporal_load(vg + (p * 9 + e) * 64);
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) yv[p][a] = (double)y[3 * cpre[p] + a];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            if (p == 0 && l == 0) {
+#pragma unroll
+                for (int e = 0; e < 9; ++e) d[e] = m[0][e];
+                nd = cpre[0];
+                continue;
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                sum[a] = sum[a] + m[p][3 * a + 0] * yv[p][0] + m[p][3 * a + 1] * yv[p][1] + m[p][3 * a + 2] * yv[p][2];
+        }
+    }
+    asm volatile("" ::: "memory");
+    {
+        double m[9];
+        double yv[3];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) m[e] = __builtin_nontemporal_load(vg + (2 * 9 + e) * 64);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) yv[a] = (double)y[3 * cpre[2] + a];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            sum[a] = sum[a] + m[3 * a + 0] * yv[0] + m[3 * a + 1] * yv[1] + m[3 * a + 2] * yv[2];
+    }
+}
+
 // Sum of the kRowLanes partial sums of row i onto its lane l == 0 (lane i).
+// In FP64 each component is summed before the next one's shuffles are issued
+// (the empty asm takes the sum as an operand), which bounds the registers
+// that hold shuffled values to one component's.
 template <typename T>
 __device__ __forceinline__ void row_combine(T sum[3], int lane) {
 #pragma unroll
@@ -985,6 +1033,7 @@ __device__ __forceinline__ void row_combine(T sum[3], int lane) {
         T acc = sum[a];
 #pragma unroll
         for (int q = 1; q < kRowLanes; ++q) acc = acc + __shfl(sum[a], (lane + q * kGroupRows) & 63, 64);
+        if constexpr (sizeof(T) == 8) asm volatile("" : "+v"(acc));
         sum[a] = acc;
     }
 }
@@ -1003,7 +1052,9 @@ __global__ __launch_bounds__(kBlock) void k_sor_color(const SorArgs A, const T* 
                                                       int pslot0) {
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     const int lane = threadIdx.x & 63;
-    const int w = xcd_block(A.per_xcd) * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
+    // the wave's row group, uniform over the wave: kept in scalar registers so
+    // that loads can address it as a scalar base plus the lane's offset
+    const int w = __builtin_amdgcn_readfirstlane(xcd_block(A.per_xcd) * (int)(blockDim.x >> 6) + (threadIdx.x >> 6));
     const int l = lane / kGroupRows, i = lane - l * kGroupRows;
     const bool live = (l < kRowLanes) && (w * kGroupRows + i < A.nrows);
     const size_t g = (size_t)A.gbase + w;
@@ -1022,7 +1073,9 @@ __global__ __launch_bounds__(kBlock) void k_sor_color(const SorArgs A, const T* 
     T d[9];
     int nd = 0;
     if (live) {
-        if constexpr (PC > 0)
+        if constexpr (PC == 3 && sizeof(T) == 8)
+            group_row_sums_2ph<V>(vals + g * (size_t)A.P * 576 + lane, cpre, l, y, sum, d, nd);
+        else if constexpr (PC > 0)
             group_row_sums_pre<T, V, PC>(vals + g * (size_t)A.P * 576 + lane, cpre, l, y, sum, d, nd);
         else
             group_row_sums<T, V, PC, false>(vals + g * (size_t)A.P * 576 + lane, A.cols + g * (size_t)A.P * 64 + lane,
