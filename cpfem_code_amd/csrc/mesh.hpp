@@ -187,6 +187,7 @@ struct Ctx {
     bool om_trial = false;       // the last solve ran at a trial factor
     double om_best = 0.0, om_rate = 0.0, om_step = 0.01;
     int om_dir = 1;
+    int om_wins = 0;             // successful trials in a row (two double the step)
     bool assembled = false;      // A and rhs hold a system (set by SUVALS)
     bool vals32_fresh = false;   // precision 2: A.vals32 holds the rounded copy of the current A.vals
     // timing and counters of the last calls
