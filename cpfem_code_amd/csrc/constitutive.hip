@@ -189,8 +189,8 @@ __device__ __forceinline__ int slpfta_c(const DevMaterial<N>& M, const DevContro
 // The per-point kernel.  One lane per point, 64 lanes per block.
 template <int N, int AMAX>
 __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__ Mp, const DevControl C,
-                                              const double dt_arg, const ConstArgs A) {
-    const double dt = A.dt_dev ? *A.dt_dev : dt_arg;
+                                              const double dt_arg, const ConstArgs A_arg) {
+    const double dt = A_arg.dt_dev ? *A_arg.dt_dev : dt_arg;
     // LDS: the Schmid columns of the active systems (also the staging area of
     // the active-set compaction) and the active system numbers, interleaved by
     // lane; the compliance and the last slips live in per-point work arrays
@@ -198,389 +198,402 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
     __shared__ double sh_b[kSlots][64];
     __shared__ int sh_isa[AMAX][64];
     const int lane = threadIdx.x;
-    const int idx = blockIdx.x * 64 + lane;
-    const bool first = A.first != 0;
-    bool live;
-    int g = 0;
-    if (first) {
-        // every point, in the order of in_list when one is given
-        live = idx < A.n;
-        if (live) g = A.in_list ? A.in_list[idx] : idx;
-    } else {
-        live = idx < *A.in_count;
-        if (live) g = A.in_list[idx];
-    }
-    bool want_cont = false, want_ovf = false;
-    if (live) {
-        // material record of the point: |material code| (checked on the host).
-        // The kernels of a fixed count serve one material set, whose record is
-        // read through wave-uniform (scalar) loads; the generic ones index it
-        // per lane.
-        const int iel = A.mesh ? A.elem_le[A.gp_elem[g]] : A.iel[g];
-        const DevMaterial<N>& M = N == kNGeneric ? Mp[abs(iel) - 1] : Mp[0];
-        const int NS = nsys(M);
-        const size_t n = (size_t)A.n;
-        double* el = A.el + g;
-        double* bc = &sh_b[0][lane];
-        double* gpre = A.gp + g;
-        double de[6], r[3], st[6], t[6], s[6];
-        // strain and rotation increments come from the kinematics kernel (mesh
-        // mode) or the caller (batch mode); in mesh mode that kernel's error
-        // code (920, 905, 906) seeds the point's code
-        int kerr = 0;
-#pragma unroll
-        for (int i = 0; i < 6; ++i) de[i] = A.de[i * n + g];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) r[i] = A.r[i * n + g];
-        if (!first || A.mesh) kerr = A.ck_kerr[g];
-#pragma unroll
-        for (int i = 0; i < 6; ++i) st[i] = A.st[i * n + g];
-        double ph[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) ph[i] = A.sv[i * n + g] + A.dsv[i * n + g];
-        elastc<N>(M, ph[0], ph[1], ph[2], el, (int)n);
-        tens_rot(r, st, t);
-        int64_t flops = 0, transc = 0;
-        int iter = 0, inner = 0;   // outer and slip-Newton iterations so far
-        bool elastic = false;
+    const bool first = A_arg.first != 0;
+    // Points of the launch in chunks of 64, one per lane: the workgroup's own
+    // chunk (blockIdx) and, when the launch is persistent (A.queue non-null,
+    // a grid of resident workgroups), every further chunk taken from the
+    // queue counter in list order as soon as the wave has finished its last.
+    int base = blockIdx.x * 64;
+    for (;;) {
+        const ConstArgs& A = A_arg;
+        const int idx = base + lane;
+        bool live;
+        int g = 0;
         if (first) {
-            flops = 1575 + 12 + 120 + 28;
-            transc = 6;
-#pragma unroll
-            for (int i = 0; i < 6; ++i) s[i] = t[i] + A.ds[i * n + g];
-            const double sinit = fj2vom(true, s);
-            const double einit = fj2vom(false, de);
-            elastic = !(iel > 0 && sinit > 0.0 && einit > 0.0);
+            // every point, in the order of in_list when one is given
+            live = idx < A.n;
+            if (live) g = A.in_list ? A.in_list[idx] : idx;
         } else {
-#pragma unroll
-            for (int i = 0; i < 6; ++i) s[i] = A.ck_s[i * n + g];
-            iter = A.ck_iter[g];
-            inner = A.ck_inner[g];
-            flops = A.flops[g];
-            transc = A.transc[g];
+            live = idx < *A.in_count;
+            if (live) g = A.in_list[idx];
         }
-        if (elastic) {
-            // Sig_eps.f90:130-145: chi = S^-1, t = chi de (the start stress is not used)
-            double lu[6][6], chi[6][6];
-#pragma unroll
-            for (int i = 0; i < 6; ++i)
-#pragma unroll
-                for (int j = 0; j < 6; ++j) lu[i][j] = el[sym21(i, j) * n];
-            (void)factor<6>(lu);
-#pragma unroll
-            for (int i = 0; i < 6; ++i) {
-                double c[6];
-#pragma unroll
-                for (int k = 0; k < 6; ++k) c[k] = (k == i) ? 1.0 : 0.0;
-                solver<6>(lu, c);
-#pragma unroll
-                for (int j = 0; j < 6; ++j) chi[j][i] = c[j];
-            }
-#pragma unroll
-            for (int i = 0; i < 6; ++i) {
-                double sum = 0.0;
-#pragma unroll
-                for (int k = 0; k < 6; ++k) sum = sum + chi[i][k] * de[k];
-                A.t[i * n + g] = sum;
-                A.ds[i * n + g] = sum - st[i];
-            }
-#pragma unroll
-            for (int j = 0; j < 6; ++j)
-#pragma unroll
-                for (int i = 0; i < 6; ++i) A.chi[(i + 6 * j) * n + g] = chi[i][j];
-            A.dep[g] = 0.0;
-            A.erepmd[g] = 0.0;
-            flops += factor_flops(6) + 6 * solver_flops(6) + 72 + 12;
-            A.flops[g] = flops;
-            A.transc[g] = transc;
-            A.err[g] = kerr;
-            if (kerr) atomicMax(A.errmax, kerr);
-        } else {
-            // ---------------- CPSTRS (Cpstrs.f90:34-172) ----------------
-            double a[3][3], aug[3];
-            ctmatrx(ph[0], ph[1], ph[2], a, aug);
-            const double gamcap = 6.0 * fj2vom(false, de);
+        bool want_cont = false, want_ovf = false;
+        if (live) {
+            // material record of the point: |material code| (checked on the host).
+            // The kernels of a fixed count serve one material set, whose record is
+            // read through wave-uniform (scalar) loads; the generic ones index it
+            // per lane.
+            const int iel = A.mesh ? A.elem_le[A.gp_elem[g]] : A.iel[g];
+            const DevMaterial<N>& M = N == kNGeneric ? Mp[abs(iel) - 1] : Mp[0];
+            const int NS = nsys(M);
+            const size_t n = (size_t)A.n;
+            double* el = A.el + g;
+            double* bc = &sh_b[0][lane];
+            double* gpre = A.gp + g;
+            double de[6], r[3], st[6], t[6], s[6];
+            // strain and rotation increments come from the kinematics kernel (mesh
+            // mode) or the caller (batch mode); in mesh mode that kernel's error
+            // code (920, 905, 906) seeds the point's code
+            int kerr = 0;
+    #pragma unroll
+            for (int i = 0; i < 6; ++i) de[i] = A.de[i * n + g];
+    #pragma unroll
+            for (int i = 0; i < 3; ++i) r[i] = A.r[i * n + g];
+            if (!first || A.mesh) kerr = A.ck_kerr[g];
+    #pragma unroll
+            for (int i = 0; i < 6; ++i) st[i] = A.st[i * n + g];
+            double ph[3];
+    #pragma unroll
+            for (int i = 0; i < 3; ++i) ph[i] = A.sv[i * n + g] + A.dsv[i * n + g];
+            elastc<N>(M, ph[0], ph[1], ph[2], el, (int)n);
+            tens_rot(r, st, t);
+            int64_t flops = 0, transc = 0;
+            int iter = 0, inner = 0;   // outer and slip-Newton iterations so far
+            bool elastic = false;
             if (first) {
-                flops += 14 + 1 + 12 + 60 * (int64_t)NS;
-                transc += 6;
+                flops = 1575 + 12 + 120 + 28;
+                transc = 6;
+    #pragma unroll
+                for (int i = 0; i < 6; ++i) s[i] = t[i] + A.ds[i * n + g];
+                const double sinit = fj2vom(true, s);
+                const double einit = fj2vom(false, de);
+                elastic = !(iel > 0 && sinit > 0.0 && einit > 0.0);
+            } else {
+    #pragma unroll
+                for (int i = 0; i < 6; ++i) s[i] = A.ck_s[i * n + g];
+                iter = A.ck_iter[g];
+                inner = A.ck_inner[g];
+                flops = A.flops[g];
+                transc = A.transc[g];
             }
-            int twinsys = A.twinsys[g];
-            const int timestwinned = A.timestwinned[g];
-            const uint64_t rng_key = ((A.rng_epoch_dev ? *A.rng_epoch_dev : A.rng_epoch) << 32) ^ (uint64_t)g;
-            int err = kerr, tflag = 0;
-            Compact<AMAX> S;
-            double djac[6][6], epl[6];
-            double erepmd = 0.0;
-            bool done = false, ovf = false;
-            while (!done && iter < A.iter_limit) {
-                double s_top[6];   // stress at the top of this outer iteration (stall test)
-#pragma unroll
-                for (int i = 0; i < 6; ++i) s_top[i] = s[i];
-                // ---- SLPINI (crys-cons.f90:143-197) over all systems.  The
-                // loop stays rolled (one power per system) and stages each
-                // system's slip and resolved shear in the LDS slice of the
-                // Schmid columns, slots 2i and 2i+1; the active ones are then
-                // compacted in place (write slot <= read slot) to 2k, 2k+1.
-                double gammax = 0.0;
-#pragma unroll 1
-                for (int i = 0; i < NS; ++i) {
-                    double sdr[3], spn[3], b[6];
-                    slip_vectors(M, a, i, sdr, spn);
-                    schmid(sdr, spn, b);
-                    double sum = 0.0;
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) sum = sum + b[k] * s[k];
-                    const double s0i = A.sv[(3 + i) * n + g];
-                    double gi;
-                    if (M.any_twin && fabs(M.ssang[i]) > 1.5) {
-                        gi = 0.0;
-                        if ((sum > s0i) && (timestwinned != M.twinlim) && (twinsys == 0)) {
-                            if (twin_draw(rng_key, iter + 1, i) < M.twinprob) {
-                                tflag = 1;
-                                twinsys = i + 1;
-                            }
-                        }
-                    } else if (M.any_unidir && M.ssang[i] * sum / s0i < 0.0) {
-                        gi = 0.0;
-                    } else {
-                        gi = dt * spower((sum / s0i), (1.0 / M.am[i]));
-                    }
-                    bc[(2 * i + 0) * 64] = gi;
-                    bc[(2 * i + 1) * 64] = sum;
-                    if (fabs(gi) > gammax) gammax = fabs(gi);
-                }
-                const double cutoff = 1.e-6 * gammax;
-                int nsa = 0;
-#pragma unroll 1
-                for (int i = 0; i < NS; ++i) {
-                    const double gi = bc[(2 * i + 0) * 64];
-                    if (!(fabs(gi) < cutoff)) {
-                        if (nsa < AMAX) {
-                            const double ti = bc[(2 * i + 1) * 64];
-                            bc[(2 * nsa + 0) * 64] = gi;
-                            bc[(2 * nsa + 1) * 64] = ti;
-                            sh_isa[nsa][lane] = i;
-                        }
-                        ++nsa;
-                    }
-                }
-                if (nsa > AMAX) {
-                    ovf = true;
-                    break;
-                }
-                ++iter;
-                S.nsa = nsa;
-#pragma unroll kUnrollA<AMAX>
-                for (int k = 0; k < AMAX; ++k) {
-                    if (k < nsa) {
-                        const int i = sh_isa[k][lane];
-                        S.isa[k] = i;
-                        S.gam[k] = bc[(2 * k + 0) * 64];
-                        S.tau[k] = bc[(2 * k + 1) * 64];
-                        S.s0[k] = A.sv[(3 + i) * n + g];
-                    } else {
-                        S.isa[k] = 0;
-                        S.gam[k] = 0.0;
-                        S.tau[k] = 0.0;
-                        S.s0[k] = 1.0;
-                    }
-                }
-                flops += 5 * (int64_t)NS;
-                transc += 2 * (int64_t)NS;
-                // Schmid columns of the active systems (BTMATRX, crys_math.f90:75-115)
-#pragma unroll 1
-                for (int k = 0; k < nsa; ++k) {
-                    double sdr[3], spn[3], b[6];
-                    slip_vectors(M, a, sh_isa[k][lane], sdr, spn);
-                    schmid(sdr, spn, b);
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) bc[(6 * k + q) * 64] = b[q];
-                }
-                scale_it_c<N, AMAX>(M, S, s, gamcap, C.afactor, flops, transc);
-                inner += slpfta_c<N, AMAX>(M, C, S, dt, gpre, n, err, flops, transc);
-                scale_it_c<N, AMAX>(M, S, s, gamcap, C.afactor, flops, transc);
-                // plastic strain increment and strain-increment error (Cpstrs.f90:96-98)
-#pragma unroll
-                for (int i = 0; i < 6; ++i) {
-                    double sum = 0.0;
-#pragma unroll kUnrollA<AMAX>
-                    for (int k = 0; k < AMAX; ++k)
-                        if (k < nsa) sum = sum + bc[(6 * k + i) * 64] * S.gam[k];
-                    epl[i] = sum;
-                }
-                double erreps[6];
-                {
-                    double sm[6];
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) sm[k] = s[k] - t[k];
-#pragma unroll
-                    for (int i = 0; i < 6; ++i) {
-                        double sum = 0.0;
-#pragma unroll
-                        for (int k = 0; k < 6; ++k) sum = sum + el[sym21(i, k) * n] * sm[k];
-                        erreps[i] = A.de[i * n + g] - epl[i] - sum;
-                    }
-                }
-                if (factor<AMAX>(S.ah)) err = err > 910 ? err : 910;
-                // djac = B_a X^-1 B_a^T + S: one solve per column (Cpstrs.f90:101-119)
-#pragma unroll
-                for (int j = 0; j < 6; ++j) {
-                    double w[AMAX];
-#pragma unroll kUnrollA<AMAX>
-                    for (int k = 0; k < AMAX; ++k) w[k] = (k < nsa) ? bc[(6 * k + j) * 64] : 0.0;
-                    solver<AMAX>(S.ah, w);
-#pragma unroll
-                    for (int i = 0; i < 6; ++i) {
-                        double sum = 0.0;
-#pragma unroll kUnrollA<AMAX>
-                        for (int k = 0; k < AMAX; ++k)
-                            if (k < nsa) sum = sum + bc[(6 * k + i) * 64] * w[k];
-                        djac[i][j] = sum + el[sym21(i, j) * n];
-                    }
-                }
-                if (factor<6>(djac)) err = err > 910 ? err : 910;
-                solver<6>(djac, erreps);
-                erepmd = fj2vom(true, erreps) / fj2vom(true, s);
-                const double scale = 1.0 / (1.0 + C.alpha * erepmd);
-#pragma unroll
-                for (int i = 0; i < 6; ++i) erreps[i] = scale * erreps[i];
-#pragma unroll
-                for (int i = 0; i < 6; ++i) s[i] = s[i] + erreps[i];
-                done = !((iter < C.maxit) && (erepmd > C.toler));
-                if (!done && C.stall_tol > 0.0) {
-                    // opt-in: the stress no longer moves (a fixed point of SCALE_IT's
-                    // cap and the Newton step); the remaining iterations would only
-                    // change it at the rounding level
-                    double dmax = 0.0, smax = 0.0;
-#pragma unroll
-                    for (int i = 0; i < 6; ++i) {
-                        if (fabs(s[i] - s_top[i]) > dmax) dmax = fabs(s[i] - s_top[i]);
-                        if (fabs(s[i]) > smax) smax = fabs(s[i]);
-                    }
-                    if (dmax <= C.stall_tol * smax) done = true;
-                }
-                flops += 24 * (int64_t)NS + 90 + 72 * (int64_t)nsa + 36 + 44 + factor_flops(nsa) +
-                         6 * solver_flops(nsa) + factor_flops(6) + solver_flops(6) + 28;
-            }
-            if (tflag) atomicOr(A.tflag, 1);
-            if (done) {
-                // ---------------- finish CPSTRS and DSTRSS ----------------
-                const int nsa = S.nsa;
-#pragma unroll
-                for (int i = 0; i < NS; ++i) A.srate[i * n + g] = 0.0;
-#pragma unroll kUnrollA<AMAX>
-                for (int k = 0; k < AMAX; ++k)
-                    if (k < nsa) A.srate[S.isa[k] * n + g] = fabs(S.gam[k] / dt);
-                // ECHROT (crys_math.f90:137-167)
-                double rtn[3] = {0.0, 0.0, 0.0};
-#pragma unroll kUnrollA<AMAX>
-                for (int k = 0; k < AMAX; ++k) {
-                    if (k < nsa) {
-                        double sdr[3], spn[3];
-                        slip_vectors(M, a, S.isa[k], sdr, spn);
-                        rtn[0] = rtn[0] + ((sdr[1] * spn[2] - sdr[2] * spn[1]) / 2.0) * S.gam[k];
-                        rtn[1] = rtn[1] + ((sdr[2] * spn[0] - sdr[0] * spn[2]) / 2.0) * S.gam[k];
-                        rtn[2] = rtn[2] + ((sdr[0] * spn[1] - sdr[1] * spn[0]) / 2.0) * S.gam[k];
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < 3; ++q) rtn[q] = rtn[q] - A.r[q * n + g];
-                double rx[3];
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    double sum = 0.0;
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) sum = sum + a[i][k] * rtn[k];
-                    rx[i] = sum;
-                }
-                const double dp1 = (fabs(aug[2]) < 1.e-5) ? rtn[2] : (aug[0] * rx[0] + aug[1] * rx[1]) / aug[2];
-                const double dph = aug[1] * rx[0] - aug[0] * rx[1];
-                const double dp2 = rx[2] - dp1 * a[2][2];
-                A.dsv[0 * n + g] = dp1;
-                A.dsv[1 * n + g] = dph;
-                A.dsv[2 * n + g] = dp2;
-                // dsv(3+i) = s1(i) - s0(i), s1 of the last PLASTC over every system
-                {
-                    double th[AMAX];
-                    double sumss = 0.0;
-#pragma unroll kUnrollA<AMAX>
-                    for (int k = 0; k < AMAX; ++k)
-                        if (k < nsa) sumss = sumss + fabs(gpre[k * n]);
-                    const double rate = sumss / dt;
-#pragma unroll kUnrollA<AMAX>
-                    for (int k = 0; k < AMAX; ++k) {
-                        th[k] = 0.0;
-                        if (k < nsa) {
-                            const int j = M.uniform_par ? 0 : S.isa[k];
-                            const double sst = M.taus[j] * pow_fast(rate, M.nsat[j]);
-                            double arg = (sst != 0.0) ? 1.0 - (S.s0[k] / sst) : 0.0;
-                            if (arg < 0.0) arg = 0.0;
-                            th[k] = M.th4[j] + M.th0[j] * copysign(pow_fast(fabs(arg), M.alp[j]), arg);
-                        }
-                    }
-#pragma unroll
-                    for (int i = 0; i < NS; ++i) {
-                        const double s0i = A.sv[(3 + i) * n + g];
-                        double s1 = s0i;
-#pragma unroll kUnrollA<AMAX>
-                        for (int l = 0; l < AMAX; ++l)
-                            if (l < nsa) s1 = s1 + (latent(M, i, S.isa[l]) * th[l]) * fabs(gpre[l * n]);
-                        A.dsv[(3 + i) * n + g] = s1 - s0i;
-                    }
-                }
-                // chi = djac^-1 (Cpstrs.f90:154-161), symmetrised (Sig_eps.f90:122-128)
-                double chi[6][6];
-#pragma unroll
+            if (elastic) {
+                // Sig_eps.f90:130-145: chi = S^-1, t = chi de (the start stress is not used)
+                double lu[6][6], chi[6][6];
+    #pragma unroll
+                for (int i = 0; i < 6; ++i)
+    #pragma unroll
+                    for (int j = 0; j < 6; ++j) lu[i][j] = el[sym21(i, j) * n];
+                (void)factor<6>(lu);
+    #pragma unroll
                 for (int i = 0; i < 6; ++i) {
                     double c[6];
-#pragma unroll
+    #pragma unroll
                     for (int k = 0; k < 6; ++k) c[k] = (k == i) ? 1.0 : 0.0;
-                    solver<6>(djac, c);
-#pragma unroll
+                    solver<6>(lu, c);
+    #pragma unroll
                     for (int j = 0; j < 6; ++j) chi[j][i] = c[j];
                 }
-#pragma unroll
-                for (int i = 0; i < 6; ++i)
-#pragma unroll
-                    for (int j = i; j < 6; ++j) {
-                        const double d = (chi[i][j] + chi[j][i]) / 2.0;
-                        chi[i][j] = d;
-                        chi[j][i] = d;
-                    }
-#pragma unroll
-                for (int j = 0; j < 6; ++j)
-#pragma unroll
-                    for (int i = 0; i < 6; ++i) A.chi[(i + 6 * j) * n + g] = chi[i][j];
-#pragma unroll
+    #pragma unroll
                 for (int i = 0; i < 6; ++i) {
-                    A.t[i * n + g] = s[i];
-                    A.ds[i * n + g] = s[i] - A.st[i * n + g];
+                    double sum = 0.0;
+    #pragma unroll
+                    for (int k = 0; k < 6; ++k) sum = sum + chi[i][k] * de[k];
+                    A.t[i * n + g] = sum;
+                    A.ds[i * n + g] = sum - st[i];
                 }
-                A.dep[g] = fj2vom(false, epl);
-                A.erepmd[g] = erepmd;
-                flops += 6 * (int64_t)NS + 25 + 6 * solver_flops(6) + 82;
+    #pragma unroll
+                for (int j = 0; j < 6; ++j)
+    #pragma unroll
+                    for (int i = 0; i < 6; ++i) A.chi[(i + 6 * j) * n + g] = chi[i][j];
+                A.dep[g] = 0.0;
+                A.erepmd[g] = 0.0;
+                flops += factor_flops(6) + 6 * solver_flops(6) + 72 + 12;
                 A.flops[g] = flops;
                 A.transc[g] = transc;
-                A.err[g] = err;
-                A.twinsys[g] = twinsys;
-                if (err) atomicMax(A.errmax, err);
+                A.err[g] = kerr;
+                if (kerr) atomicMax(A.errmax, kerr);
             } else {
-                // checkpoint: stress at the next outer iteration and the work so far
-#pragma unroll
-                for (int i = 0; i < 6; ++i) A.ck_s[i * n + g] = s[i];
-                A.ck_iter[g] = iter;
-                A.ck_inner[g] = inner;
-                A.ck_kerr[g] = err;
-                A.flops[g] = flops;
-                A.transc[g] = transc;
-                A.twinsys[g] = twinsys;
-                want_ovf = ovf;
-                want_cont = !ovf;
+                // ---------------- CPSTRS (Cpstrs.f90:34-172) ----------------
+                double a[3][3], aug[3];
+                ctmatrx(ph[0], ph[1], ph[2], a, aug);
+                const double gamcap = 6.0 * fj2vom(false, de);
+                if (first) {
+                    flops += 14 + 1 + 12 + 60 * (int64_t)NS;
+                    transc += 6;
+                }
+                int twinsys = A.twinsys[g];
+                const int timestwinned = A.timestwinned[g];
+                const uint64_t rng_key = ((A.rng_epoch_dev ? *A.rng_epoch_dev : A.rng_epoch) << 32) ^ (uint64_t)g;
+                int err = kerr, tflag = 0;
+                Compact<AMAX> S;
+                double djac[6][6], epl[6];
+                double erepmd = 0.0;
+                bool done = false, ovf = false;
+                while (!done && iter < A.iter_limit) {
+                    double s_top[6];   // stress at the top of this outer iteration (stall test)
+    #pragma unroll
+                    for (int i = 0; i < 6; ++i) s_top[i] = s[i];
+                    // ---- SLPINI (crys-cons.f90:143-197) over all systems.  The
+                    // loop stays rolled (one power per system) and stages each
+                    // system's slip and resolved shear in the LDS slice of the
+                    // Schmid columns, slots 2i and 2i+1; the active ones are then
+                    // compacted in place (write slot <= read slot) to 2k, 2k+1.
+                    double gammax = 0.0;
+    #pragma unroll 1
+                    for (int i = 0; i < NS; ++i) {
+                        double sdr[3], spn[3], b[6];
+                        slip_vectors(M, a, i, sdr, spn);
+                        schmid(sdr, spn, b);
+                        double sum = 0.0;
+    #pragma unroll
+                        for (int k = 0; k < 6; ++k) sum = sum + b[k] * s[k];
+                        const double s0i = A.sv[(3 + i) * n + g];
+                        double gi;
+                        if (M.any_twin && fabs(M.ssang[i]) > 1.5) {
+                            gi = 0.0;
+                            if ((sum > s0i) && (timestwinned != M.twinlim) && (twinsys == 0)) {
+                                if (twin_draw(rng_key, iter + 1, i) < M.twinprob) {
+                                    tflag = 1;
+                                    twinsys = i + 1;
+                                }
+                            }
+                        } else if (M.any_unidir && M.ssang[i] * sum / s0i < 0.0) {
+                            gi = 0.0;
+                        } else {
+                            gi = dt * spower((sum / s0i), (1.0 / M.am[i]));
+                        }
+                        bc[(2 * i + 0) * 64] = gi;
+                        bc[(2 * i + 1) * 64] = sum;
+                        if (fabs(gi) > gammax) gammax = fabs(gi);
+                    }
+                    const double cutoff = 1.e-6 * gammax;
+                    int nsa = 0;
+    #pragma unroll 1
+                    for (int i = 0; i < NS; ++i) {
+                        const double gi = bc[(2 * i + 0) * 64];
+                        if (!(fabs(gi) < cutoff)) {
+                            if (nsa < AMAX) {
+                                const double ti = bc[(2 * i + 1) * 64];
+                                bc[(2 * nsa + 0) * 64] = gi;
+                                bc[(2 * nsa + 1) * 64] = ti;
+                                sh_isa[nsa][lane] = i;
+                            }
+                            ++nsa;
+                        }
+                    }
+                    if (nsa > AMAX) {
+                        ovf = true;
+                        break;
+                    }
+                    ++iter;
+                    S.nsa = nsa;
+    #pragma unroll kUnrollA<AMAX>
+                    for (int k = 0; k < AMAX; ++k) {
+                        if (k < nsa) {
+                            const int i = sh_isa[k][lane];
+                            S.isa[k] = i;
+                            S.gam[k] = bc[(2 * k + 0) * 64];
+                            S.tau[k] = bc[(2 * k + 1) * 64];
+                            S.s0[k] = A.sv[(3 + i) * n + g];
+                        } else {
+                            S.isa[k] = 0;
+                            S.gam[k] = 0.0;
+                            S.tau[k] = 0.0;
+                            S.s0[k] = 1.0;
+                        }
+                    }
+                    flops += 5 * (int64_t)NS;
+                    transc += 2 * (int64_t)NS;
+                    // Schmid columns of the active systems (BTMATRX, crys_math.f90:75-115)
+    #pragma unroll 1
+                    for (int k = 0; k < nsa; ++k) {
+                        double sdr[3], spn[3], b[6];
+                        slip_vectors(M, a, sh_isa[k][lane], sdr, spn);
+                        schmid(sdr, spn, b);
+    #pragma unroll
+                        for (int q = 0; q < 6; ++q) bc[(6 * k + q) * 64] = b[q];
+                    }
+                    scale_it_c<N, AMAX>(M, S, s, gamcap, C.afactor, flops, transc);
+                    inner += slpfta_c<N, AMAX>(M, C, S, dt, gpre, n, err, flops, transc);
+                    scale_it_c<N, AMAX>(M, S, s, gamcap, C.afactor, flops, transc);
+                    // plastic strain increment and strain-increment error (Cpstrs.f90:96-98)
+    #pragma unroll
+                    for (int i = 0; i < 6; ++i) {
+                        double sum = 0.0;
+    #pragma unroll kUnrollA<AMAX>
+                        for (int k = 0; k < AMAX; ++k)
+                            if (k < nsa) sum = sum + bc[(6 * k + i) * 64] * S.gam[k];
+                        epl[i] = sum;
+                    }
+                    double erreps[6];
+                    {
+                        double sm[6];
+    #pragma unroll
+                        for (int k = 0; k < 6; ++k) sm[k] = s[k] - t[k];
+    #pragma unroll
+                        for (int i = 0; i < 6; ++i) {
+                            double sum = 0.0;
+    #pragma unroll
+                            for (int k = 0; k < 6; ++k) sum = sum + el[sym21(i, k) * n] * sm[k];
+                            erreps[i] = A.de[i * n + g] - epl[i] - sum;
+                        }
+                    }
+                    if (factor<AMAX>(S.ah)) err = err > 910 ? err : 910;
+                    // djac = B_a X^-1 B_a^T + S: one solve per column (Cpstrs.f90:101-119)
+    #pragma unroll
+                    for (int j = 0; j < 6; ++j) {
+                        double w[AMAX];
+    #pragma unroll kUnrollA<AMAX>
+                        for (int k = 0; k < AMAX; ++k) w[k] = (k < nsa) ? bc[(6 * k + j) * 64] : 0.0;
+                        solver<AMAX>(S.ah, w);
+    #pragma unroll
+                        for (int i = 0; i < 6; ++i) {
+                            double sum = 0.0;
+    #pragma unroll kUnrollA<AMAX>
+                            for (int k = 0; k < AMAX; ++k)
+                                if (k < nsa) sum = sum + bc[(6 * k + i) * 64] * w[k];
+                            djac[i][j] = sum + el[sym21(i, j) * n];
+                        }
+                    }
+                    if (factor<6>(djac)) err = err > 910 ? err : 910;
+                    solver<6>(djac, erreps);
+                    erepmd = fj2vom(true, erreps) / fj2vom(true, s);
+                    const double scale = 1.0 / (1.0 + C.alpha * erepmd);
+    #pragma unroll
+                    for (int i = 0; i < 6; ++i) erreps[i] = scale * erreps[i];
+    #pragma unroll
+                    for (int i = 0; i < 6; ++i) s[i] = s[i] + erreps[i];
+                    done = !((iter < C.maxit) && (erepmd > C.toler));
+                    if (!done && C.stall_tol > 0.0) {
+                        // opt-in: the stress no longer moves (a fixed point of SCALE_IT's
+                        // cap and the Newton step); the remaining iterations would only
+                        // change it at the rounding level
+                        double dmax = 0.0, smax = 0.0;
+    #pragma unroll
+                        for (int i = 0; i < 6; ++i) {
+                            if (fabs(s[i] - s_top[i]) > dmax) dmax = fabs(s[i] - s_top[i]);
+                            if (fabs(s[i]) > smax) smax = fabs(s[i]);
+                        }
+                        if (dmax <= C.stall_tol * smax) done = true;
+                    }
+                    flops += 24 * (int64_t)NS + 90 + 72 * (int64_t)nsa + 36 + 44 + factor_flops(nsa) +
+                             6 * solver_flops(nsa) + factor_flops(6) + solver_flops(6) + 28;
+                }
+                if (tflag) atomicOr(A.tflag, 1);
+                if (done) {
+                    // ---------------- finish CPSTRS and DSTRSS ----------------
+                    const int nsa = S.nsa;
+    #pragma unroll
+                    for (int i = 0; i < NS; ++i) A.srate[i * n + g] = 0.0;
+    #pragma unroll kUnrollA<AMAX>
+                    for (int k = 0; k < AMAX; ++k)
+                        if (k < nsa) A.srate[S.isa[k] * n + g] = fabs(S.gam[k] / dt);
+                    // ECHROT (crys_math.f90:137-167)
+                    double rtn[3] = {0.0, 0.0, 0.0};
+    #pragma unroll kUnrollA<AMAX>
+                    for (int k = 0; k < AMAX; ++k) {
+                        if (k < nsa) {
+                            double sdr[3], spn[3];
+                            slip_vectors(M, a, S.isa[k], sdr, spn);
+                            rtn[0] = rtn[0] + ((sdr[1] * spn[2] - sdr[2] * spn[1]) / 2.0) * S.gam[k];
+                            rtn[1] = rtn[1] + ((sdr[2] * spn[0] - sdr[0] * spn[2]) / 2.0) * S.gam[k];
+                            rtn[2] = rtn[2] + ((sdr[0] * spn[1] - sdr[1] * spn[0]) / 2.0) * S.gam[k];
+                        }
+                    }
+    #pragma unroll
+                    for (int q = 0; q < 3; ++q) rtn[q] = rtn[q] - A.r[q * n + g];
+                    double rx[3];
+    #pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        double sum = 0.0;
+    #pragma unroll
+                        for (int k = 0; k < 3; ++k) sum = sum + a[i][k] * rtn[k];
+                        rx[i] = sum;
+                    }
+                    const double dp1 = (fabs(aug[2]) < 1.e-5) ? rtn[2] : (aug[0] * rx[0] + aug[1] * rx[1]) / aug[2];
+                    const double dph = aug[1] * rx[0] - aug[0] * rx[1];
+                    const double dp2 = rx[2] - dp1 * a[2][2];
+                    A.dsv[0 * n + g] = dp1;
+                    A.dsv[1 * n + g] = dph;
+                    A.dsv[2 * n + g] = dp2;
+                    // dsv(3+i) = s1(i) - s0(i), s1 of the last PLASTC over every system
+                    {
+                        double th[AMAX];
+                        double sumss = 0.0;
+    #pragma unroll kUnrollA<AMAX>
+                        for (int k = 0; k < AMAX; ++k)
+                            if (k < nsa) sumss = sumss + fabs(gpre[k * n]);
+                        const double rate = sumss / dt;
+    #pragma unroll kUnrollA<AMAX>
+                        for (int k = 0; k < AMAX; ++k) {
+                            th[k] = 0.0;
+                            if (k < nsa) {
+                                const int j = M.uniform_par ? 0 : S.isa[k];
+                                const double sst = M.taus[j] * pow_fast(rate, M.nsat[j]);
+                                double arg = (sst != 0.0) ? 1.0 - (S.s0[k] / sst) : 0.0;
+                                if (arg < 0.0) arg = 0.0;
+                                th[k] = M.th4[j] + M.th0[j] * copysign(pow_fast(fabs(arg), M.alp[j]), arg);
+                            }
+                        }
+    #pragma unroll
+                        for (int i = 0; i < NS; ++i) {
+                            const double s0i = A.sv[(3 + i) * n + g];
+                            double s1 = s0i;
+    #pragma unroll kUnrollA<AMAX>
+                            for (int l = 0; l < AMAX; ++l)
+                                if (l < nsa) s1 = s1 + (latent(M, i, S.isa[l]) * th[l]) * fabs(gpre[l * n]);
+                            A.dsv[(3 + i) * n + g] = s1 - s0i;
+                        }
+                    }
+                    // chi = djac^-1 (Cpstrs.f90:154-161), symmetrised (Sig_eps.f90:122-128)
+                    double chi[6][6];
+    #pragma unroll
+                    for (int i = 0; i < 6; ++i) {
+                        double c[6];
+    #pragma unroll
+                        for (int k = 0; k < 6; ++k) c[k] = (k == i) ? 1.0 : 0.0;
+                        solver<6>(djac, c);
+    #pragma unroll
+                        for (int j = 0; j < 6; ++j) chi[j][i] = c[j];
+                    }
+    #pragma unroll
+                    for (int i = 0; i < 6; ++i)
+    #pragma unroll
+                        for (int j = i; j < 6; ++j) {
+                            const double d = (chi[i][j] + chi[j][i]) / 2.0;
+                            chi[i][j] = d;
+                            chi[j][i] = d;
+                        }
+    #pragma unroll
+                    for (int j = 0; j < 6; ++j)
+    #pragma unroll
+                        for (int i = 0; i < 6; ++i) A.chi[(i + 6 * j) * n + g] = chi[i][j];
+    #pragma unroll
+                    for (int i = 0; i < 6; ++i) {
+                        A.t[i * n + g] = s[i];
+                        A.ds[i * n + g] = s[i] - A.st[i * n + g];
+                    }
+                    A.dep[g] = fj2vom(false, epl);
+                    A.erepmd[g] = erepmd;
+                    flops += 6 * (int64_t)NS + 25 + 6 * solver_flops(6) + 82;
+                    A.flops[g] = flops;
+                    A.transc[g] = transc;
+                    A.err[g] = err;
+                    A.twinsys[g] = twinsys;
+                    if (err) atomicMax(A.errmax, err);
+                } else {
+                    // checkpoint: stress at the next outer iteration and the work so far
+    #pragma unroll
+                    for (int i = 0; i < 6; ++i) A.ck_s[i * n + g] = s[i];
+                    A.ck_iter[g] = iter;
+                    A.ck_inner[g] = inner;
+                    A.ck_kerr[g] = err;
+                    A.flops[g] = flops;
+                    A.transc[g] = transc;
+                    A.twinsys[g] = twinsys;
+                    want_ovf = ovf;
+                    want_cont = !ovf;
+                }
             }
         }
+        list_append(want_cont, g, A.cont_list, A.cont_count);
+        list_append(want_ovf, g, A.ovf_list, A.ovf_count);
+        if (!A.queue) break;
+        int next = 0;
+        if (lane == 0) next = atomicAdd(A.queue, 64);
+        base = __shfl(next, 0, 64);
+        if (base >= (first ? A.n : *A.in_count)) break;
     }
-    list_append(want_cont, g, A.cont_list, A.cont_count);
-    list_append(want_ovf, g, A.ovf_list, A.ovf_count);
 }
 
 template <int N>
@@ -783,11 +796,39 @@ static hipError_t sort_points(ConstSchedule& S, const int* list, int n, const lo
     return hipGetLastError();
 }
 
+// Workgroups of k_const<N, AMAX> that the device holds at once (0 if unknown).
 template <int N, int AMAX>
-static void launch_one(const void* dmat, const DevControl& C, double dt, const ConstArgs& A, int nthreads,
-                       hipStream_t stream) {
+static int resident_blocks() {
+    static int cached = -1;
+    if (cached < 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_const<N, AMAX>, 64, 0) != hipSuccess)
+            cus = per_cu = 0;
+        cached = cus * per_cu;
+    }
+    return cached;
+}
+
+// One pass over nthreads list entries (an upper bound when the count is on
+// the device).  When that needs more workgroups than the device holds at once,
+// the launch is persistent: a grid of resident workgroups that take the list's
+// chunks from the queue counter S.counts[4] (set here to the grid's own
+// chunks), so that each wave starts its next chunk as soon as it finishes one.
+template <int N, int AMAX>
+static void launch_one(const void* dmat, const DevControl& C, double dt, ConstArgs A, int nthreads,
+                       const ConstSchedule& S, hipStream_t stream) {
     if (nthreads <= 0) return;
-    hipLaunchKernelGGL((k_const<N, AMAX>), dim3((nthreads + 63) / 64), dim3(64), 0, stream,
+    int grid = (nthreads + 63) / 64;
+    A.queue = nullptr;
+    const int cap = S.queue ? resident_blocks<N, AMAX>() : 0;
+    if (cap > 0 && cap < grid) {
+        grid = cap;
+        A.queue = S.counts + 4;
+        hipLaunchKernelGGL(k_set1, dim3(1), dim3(64), 0, stream, A.queue, grid * 64);
+    }
+    hipLaunchKernelGGL((k_const<N, AMAX>), dim3(grid), dim3(64), 0, stream,
                        reinterpret_cast<const DevMaterial<N>*>(dmat), C, dt, A);
 }
 
@@ -839,7 +880,7 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
         if (timed && hipEventElapsedTime(&ms, S.ev[a], S.ev[b]) == hipSuccess) S.pass_ms[q] += ms;
     };
     mark(0);
-    launch_one<N, kAmax>(dmat, C, dt, A, A.n, stream);
+    launch_one<N, kAmax>(dmat, C, dt, A, A.n, S, stream);
     mark(1);
     if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
     span(0, 0, 1);
@@ -854,7 +895,7 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
         A.cont_count = S.counts + 1;
         A.iter_limit = C.maxit;
         mark(1);
-        launch_one<N, kAmax>(dmat, C, dt, A, S.host_counts[0], stream);
+        launch_one<N, kAmax>(dmat, C, dt, A, S.host_counts[0], S, stream);
         mark(2);
         if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
         span(1, 1, 2);
@@ -881,7 +922,7 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
         // one (the split is exact: a point resumes from its checkpoint)
         A.iter_limit = wide_limit < C.maxit ? wide_limit : C.maxit;
         mark(2);
-        launch_one<N, kAmaxMid>(dmat, C, dt, A, S.host_counts[2], stream);
+        launch_one<N, kAmaxMid>(dmat, C, dt, A, S.host_counts[2], S, stream);
         if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
         if (S.host_counts[1] > 0) {
             if ((e = sort_points(S, S.list_b, S.host_counts[1], nullptr, stream)) != hipSuccess) return e;
@@ -889,7 +930,7 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
             A.in_count = S.counts + 1;
             A.cont_count = S.counts + 3;   // no point continues at iter_limit = maxit
             A.iter_limit = C.maxit;
-            launch_one<N, kAmaxMid>(dmat, C, dt, A, S.host_counts[1], stream);
+            launch_one<N, kAmaxMid>(dmat, C, dt, A, S.host_counts[1], S, stream);
         }
         mark(3);
         if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
@@ -905,7 +946,7 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
             A.ovf_count = S.counts + 2;
             hipLaunchKernelGGL(k_set1, dim3(1), dim3(64), 0, stream, S.counts + 2, 0);
             mark(3);
-            launch_one<N, kAmaxWide>(dmat, C, dt, A, S.host_counts[0], stream);
+            launch_one<N, kAmaxWide>(dmat, C, dt, A, S.host_counts[0], S, stream);
             mark(4);
             if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
             span(3, 3, 4);
@@ -923,7 +964,7 @@ static hipError_t schedule(const void* dmat, const DevControl& C, double dt, Con
                     A.ovf_list = S.list_a;
                     A.ovf_count = S.counts + 0;
                     mark(3);
-                    launch_one<N, kAmaxGeneric>(dmat, C, dt, A, S.host_counts[2], stream);
+                    launch_one<N, kAmaxGeneric>(dmat, C, dt, A, S.host_counts[2], S, stream);
                     mark(4);
                     if ((e = read_counts<N>(S, stream)) != hipSuccess) return e;
                     span(3, 3, 4);
@@ -989,7 +1030,7 @@ static hipError_t schedule_dev(const void* dmat, const DevControl& C, double dt,
     A.ovf_count = S.counts + 2;
     const int fi = S.first_iters > 0 ? S.first_iters : 8;
     A.iter_limit = fi < C.maxit ? fi : C.maxit;
-    launch_one<N, kAmax>(dmat, C, dt, A, n, stream);
+    launch_one<N, kAmax>(dmat, C, dt, A, n, S, stream);
     // pass 2: the unfinished points
     A.first = 0;
     if ((e = sort_points(S, S.list_a, n, nullptr, stream, false, S.counts + 0)) != hipSuccess) return e;
@@ -998,7 +1039,7 @@ static hipError_t schedule_dev(const void* dmat, const DevControl& C, double dt,
     A.cont_list = S.list_b;
     A.cont_count = S.counts + 1;
     A.iter_limit = C.maxit;
-    launch_one<N, kAmax>(dmat, C, dt, A, n, stream);
+    launch_one<N, kAmax>(dmat, C, dt, A, n, S, stream);
     // tier 2: the overflow of tier 1, by the previous update's work on a mesh
     hipLaunchKernelGGL(k_tier_prep, dim3(1), dim3(64), 0, stream, S.counts, report);
     if ((e = sort_points(S, S.list_ovf, n, nullptr, stream, A.mesh != 0, S.counts + 3)) != hipSuccess) return e;
@@ -1010,26 +1051,26 @@ static hipError_t schedule_dev(const void* dmat, const DevControl& C, double dt,
     A.ovf_count = S.counts + 0;
     const int wide_limit = S.wide_iters > 0 ? S.wide_iters : fi;
     A.iter_limit = wide_limit < C.maxit ? wide_limit : C.maxit;
-    launch_one<N, kAmaxMid>(dmat, C, dt, A, n, stream);
+    launch_one<N, kAmaxMid>(dmat, C, dt, A, n, S, stream);
     if ((e = sort_points(S, S.list_b, n, nullptr, stream, false, S.counts + 1)) != hipSuccess) return e;
     A.in_list = S.sorted;
     A.in_count = S.counts + 1;
     A.cont_count = S.counts + 3;
     A.iter_limit = C.maxit;
-    launch_one<N, kAmaxMid>(dmat, C, dt, A, n, stream);
+    launch_one<N, kAmaxMid>(dmat, C, dt, A, n, S, stream);
     // tier 3: the overflow of tier 2
     A.in_list = S.list_a;
     A.in_count = S.counts + 0;
     A.ovf_list = S.list_ovf;
     A.ovf_count = S.counts + 2;
-    launch_one<N, kAmaxWide>(dmat, C, dt, A, n, stream);
+    launch_one<N, kAmaxWide>(dmat, C, dt, A, n, S, stream);
     if constexpr (N == kNGeneric) {
         hipLaunchKernelGGL(k_tier4_prep, dim3(1), dim3(64), 0, stream, S.counts);
         A.in_list = S.list_ovf;
         A.in_count = S.counts + 3;
         A.ovf_list = S.list_a;
         A.ovf_count = S.counts + 0;
-        launch_one<N, kAmaxGeneric>(dmat, C, dt, A, n, stream);
+        launch_one<N, kAmaxGeneric>(dmat, C, dt, A, n, S, stream);
         hipLaunchKernelGGL(k_width_report, dim3(1), dim3(64), 0, stream, (const int*)S.counts, 0, report);
     } else {
         hipLaunchKernelGGL(k_width_report, dim3(1), dim3(64), 0, stream, (const int*)S.counts, 2, report);
@@ -1059,8 +1100,30 @@ hipError_t run_constitutive(int nss, const void* dmat, const DevControl& C, doub
     }
 }
 
+// Resident workgroups of every compiled kernel, looked up once outside any
+// stream capture.
+static void prime_resident() {
+    (void)resident_blocks<12, kAmax>();
+    (void)resident_blocks<12, kAmaxMid>();
+    (void)resident_blocks<12, kAmaxWide>();
+    (void)resident_blocks<18, kAmax>();
+    (void)resident_blocks<18, kAmaxMid>();
+    (void)resident_blocks<18, kAmaxWide>();
+    (void)resident_blocks<kNGeneric, kAmax>();
+    (void)resident_blocks<kNGeneric, kAmaxMid>();
+    (void)resident_blocks<kNGeneric, kAmaxWide>();
+    (void)resident_blocks<kNGeneric, kAmaxGeneric>();
+}
+
+static bool queue_enabled() {
+    const char* v = getenv("CPFEM_CONST_QUEUE");
+    return !(v && atoi(v) == 0);
+}
+
 hipError_t alloc_schedule(ConstSchedule& S, int n) {
     hipError_t e;
+    S.queue = queue_enabled();
+    prime_resident();
     if ((e = hipMalloc((void**)&S.ck_s, sizeof(double) * 6 * (size_t)n)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&S.ck_iter, sizeof(int) * (size_t)n)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&S.ck_kerr, sizeof(int) * (size_t)n)) != hipSuccess) return e;
@@ -1073,7 +1136,7 @@ hipError_t alloc_schedule(ConstSchedule& S, int n) {
     if ((e = hipMalloc((void**)&S.list_ovf, sizeof(int) * (size_t)n)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&S.el, sizeof(double) * 21 * (size_t)n)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&S.gp, sizeof(double) * kConstMaxWidth * (size_t)n)) != hipSuccess) return e;
-    if ((e = hipMalloc((void**)&S.counts, sizeof(int) * 4)) != hipSuccess) return e;
+    if ((e = hipMalloc((void**)&S.counts, sizeof(int) * 8)) != hipSuccess) return e;
     if ((e = hipHostMalloc((void**)&S.host_counts, sizeof(int) * 4, 0)) != hipSuccess) return e;
     for (int k = 0; k < 5; ++k)
         if ((e = hipEventCreate(&S.ev[k])) != hipSuccess) return e;
@@ -1379,8 +1442,10 @@ extern "C" int cpfem_dstrss_batch_noalloc(const cpfem_material* mat, const cpfem
     S.list_ovf = cv.take<int>(N);
     S.el = cv.take<double>(21 * N);
     S.gp = cv.take<double>((size_t)kConstMaxWidth * N);
-    S.counts = cv.take<int>(4);
+    S.counts = cv.take<int>(8);
     S.host_counts = g_noalloc_counts[dev];
+    S.queue = queue_enabled();
+    prime_resident();
     if (cv.off > kScratchBytes) return CPFEM_E_ARG;   // cannot happen within kNoAllocMaxPoints
     hipStream_t s0 = 0;
     // material record, built in host stack memory

@@ -336,6 +336,7 @@ struct ConstArgs {
     int* ovf_list;             // points whose active set exceeds the kernel's width
     int* ovf_count;
     int iter_limit;            // outer iterations allowed in total after this launch
+    int* queue;                // non-null: persistent launch, next chunk of the list at *queue
 };
 
 // wave-aggregated append of this lane's point to a list

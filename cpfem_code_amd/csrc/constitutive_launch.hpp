@@ -27,13 +27,15 @@ struct ConstSchedule {
     int* list_ovf = nullptr;   // [n] active set wider than the first kernel
     double* el = nullptr;      // [21][n] compliance in specimen axes of the current pass
     double* gp = nullptr;      // [kConstMaxWidth][n] slips of the last hardening update
-    int* counts = nullptr;     // [4] device
+    int* counts = nullptr;     // [8] device: list counts [0..3], work-queue counter [4]
     int* host_counts = nullptr;// [4] pinned
     hipEvent_t ev[5] = {};     // around the passes of the last run (timing)
     int first_iters = 8;       // outer iterations of the first pass over every point (cpfem_autotune)
     int wide_iters = 0;        // outer iterations (counted from the point's start) before the 8-wide tier's
                                // unfinished points are compacted and resumed (0: first_iters)
     double pass_ms[4] = {};    // device time of the last run: pass 1, pass 2, 8-wide, 12(+)-wide tiers
+    bool queue = true;         // persistent launches draining the point lists (CPFEM_CONST_QUEUE=0: one
+                               // workgroup per 64 list entries, drained by the hardware dispatcher)
 };
 
 DevControl make_control(const cpfem_control& c);
