@@ -1194,10 +1194,13 @@ __global__ void k_sor_check(const double* __restrict__ partial, int nblocks, dou
             s0 += sh[t][0];
             s1 += sh[t][1];
         }
-        const double err = s1 > 0.0 ? s0 / s1 : 0.0;
+        // a non-finite |y|^2 (NaN or Inf in the iterate) makes the measure NaN
+        const double err = !isfinite(s1) ? s1 - s1 : (s1 > 0.0 ? s0 / s1 : 0.0);
         *err_out = err;
         *sweeps += 1;
-        if (err < tol2 || *sweeps >= maxit) *stop = 1;
+        // a non-finite measure (a matrix or right-hand side with NaN or Inf
+        // entries, or sweeps that diverge) ends the solve at once
+        if (err < tol2 || *sweeps >= maxit || !isfinite(err)) *stop = 1;
     }
 }
 
@@ -1347,7 +1350,8 @@ __global__ __launch_bounds__(kBlock) void k_solve_check(const double* __restrict
     double r[4];
     reduce_partials(partial, nblocks, r);
     if (threadIdx.x != 0) return;
-    const double rel = r[1] > 0.0 ? sqrt(r[0] / r[1]) : 0.0;
+    // a right-hand side with NaN or Inf entries makes the measure NaN, not 0
+    const double rel = !isfinite(r[1]) ? r[1] - r[1] : (r[1] > 0.0 ? sqrt(r[0] / r[1]) : 0.0);
     const int sweeps = nd->sweeps;
     const double prev_rel = nd->prev_rel;
     const int prev_sweeps = nd->prev_sweeps;
@@ -1358,6 +1362,14 @@ __global__ __launch_bounds__(kBlock) void k_solve_check(const double* __restrict
     if (rel < tol) {
         if (rel > 0.0 && sweeps > 0) nd->final_rate = log(rel) / (double)sweeps;
         nd->stop = 1;
+        return;
+    }
+    if (!isfinite(rel)) {
+        // NaN or Inf in the system or diverging sweeps: the solve ends at once
+        // and the iteration's gated Newton update is skipped, so that the state
+        // stays as it was (the host reports 904 and ends NRITER)
+        nd->stop = 1;
+        nd->updated = 1;
         return;
     }
     int b = check_batches(rel, prev_rel, sweeps, prev_sweeps, tol, batch, max_batches);
@@ -2694,6 +2706,7 @@ static double residual(Ctx* c, const double* b, const double* y, double* res) {
     const int nb = residual_enqueue(c, b, y, res);
     double r[4];
     reduce_now(c, nb, r);
+    if (!std::isfinite(r[1])) return std::nan("");
     return r[1] > 0.0 ? sqrt(r[0] / r[1]) : 0.0;
 }
 
@@ -2907,7 +2920,7 @@ static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out, bool warm = fal
         }
         sweeps = hf[1];
         CPFEM_HIP(hipMemcpy(&err, c->red.result + 4, sizeof(double), hipMemcpyDeviceToHost));
-        if (err > 10.0 * tol2) code = 904;
+        if (!(err <= 10.0 * tol2)) code = 904;
     } else {
         // relative Jacobi-scaled residual, checked after a number of 16-sweep
         // batches predicted from the observed geometric convergence rate (at
@@ -2936,7 +2949,7 @@ static int facsls_impl(Ctx* c, int* sweeps_out, double* err_out, bool warm = fal
         err = h.rel;
         final_rate = h.final_rate;
         c->last_batches = sweeps / kSweepBatch;
-        if (err > 10.0 * c->ctl.sor_tol) code = 904;
+        if (!(err <= 10.0 * c->ctl.sor_tol)) code = 904;
     }
     CPFEM_HIP(hipEventRecord(c->ev[6], s));
     CPFEM_HIP(hipEventSynchronize(c->ev[6]));
@@ -3419,7 +3432,7 @@ static int graph_iteration(Ctx* c, int inc, double* sdx) {
     S.sweep_launches += (int64_t)h.sweeps * c->A.ncolor;
     c->last_batches = h.sweeps / kSweepBatch;
     int code = 0;
-    if (h.rel > 10.0 * c->ctl.sor_tol) code = 904;
+    if (!(h.rel <= 10.0 * c->ctl.sor_tol)) code = 904;
     if (code > S.err) S.err = code;
     if (code == 0) refine_omega(c, h.final_rate);
     S.sdx = h.sdx;
@@ -3449,6 +3462,7 @@ extern "C" int cpfem_nriter(cpfem_ctx* c, int32_t inc, cpfem_stats* st) {
             double sdx = 0.0;
             code = graph_iteration(c, inc, &sdx);
             if (code < 0 || (code > 500 && code != 987 && code != 910 && code != 904)) break;
+            if (!std::isfinite(c->stats.sor_error)) break;   // no update was applied
             more = (sdx >= c->ctl.toler) && (nit < c->ctl.maxit);
             continue;
         }
@@ -3464,6 +3478,8 @@ extern "C" int cpfem_nriter(cpfem_ctx* c, int32_t inc, cpfem_stats* st) {
         if (code) break;
         code = facsls_impl(c, &sweeps, &serr);
         if (code < 0 || (code > 500 && code != 904)) break;
+        // a solution with NaN or Inf entries is not applied: NRITER ends with 904
+        if (!std::isfinite(serr)) break;
         code = nr_update_impl(c, &sdx, &alp2);
         if (code) break;
         c->stats.sumcpiter = sumcp / (double)c->ngps;
