@@ -249,7 +249,10 @@ int cpfem_update_node(cpfem_ctx *ctx);
 int cpfem_elvals(cpfem_ctx *ctx, int32_t inc, double *sumcpiter, int32_t *tflag);
 /* SUVALS (Bconds.f90:19) and the free-node force norm of Stif_sol.f90:46-51. */
 int cpfem_suvals(cpfem_ctx *ctx, int32_t inc, double *sfc);
-/* FACSLS (Stif_sol.f90:161): solve for ddx. */
+/* FACSLS (Stif_sol.f90:161): solve for ddx.  A stopping measure that is not
+ * finite (NaN or Inf in the system, or diverging sweeps) ends the solve at
+ * the next check with 904 and a NaN *error; cpfem_nriter then ends without
+ * applying that correction. */
 int cpfem_facsls(cpfem_ctx *ctx, int32_t *sweeps, double *error);
 /* Norms, damping and update of Stif_sol.f90:55-74: x -= alp2*ddx, dx -= alp2*ddx. */
 int cpfem_nr_update(cpfem_ctx *ctx, double *sdx, double *alp2);
