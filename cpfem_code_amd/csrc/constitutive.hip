@@ -189,8 +189,8 @@ __device__ __forceinline__ int slpfta_c(const DevMaterial<N>& M, const DevContro
 // The per-point kernel.  One lane per point, 64 lanes per block.
 template <int N, int AMAX>
 __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__ Mp, const DevControl C,
-                                              const double dt_arg, const ConstArgs A_arg) {
-    const double dt = A_arg.dt_dev ? *A_arg.dt_dev : dt_arg;
+                                              const double dt_arg, const ConstArgs A) {
+    const double dt = A.dt_dev ? *A.dt_dev : dt_arg;
     // LDS: the Schmid columns of the active systems (also the staging area of
     // the active-set compaction) and the active system numbers, interleaved by
     // lane; the compliance and the last slips live in per-point work arrays
@@ -198,14 +198,13 @@ __global__ __launch_bounds__(64) void k_const(const DevMaterial<N>* __restrict__
     __shared__ double sh_b[kSlots][64];
     __shared__ int sh_isa[AMAX][64];
     const int lane = threadIdx.x;
-    const bool first = A_arg.first != 0;
+    const bool first = A.first != 0;
     // Points of the launch in chunks of 64, one per lane: the workgroup's own
     // chunk (blockIdx) and, when the launch is persistent (A.queue non-null,
     // a grid of resident workgroups), every further chunk taken from the
     // queue counter in list order as soon as the wave has finished its last.
     int base = blockIdx.x * 64;
     for (;;) {
-        const ConstArgs& A = A_arg;
         const int idx = base + lane;
         bool live;
         int g = 0;
